@@ -1,0 +1,233 @@
+"""Benchmark: 1080p denoised frames/s (forward + online backward) and p99 frame latency, one GPU.
+
+Workload: a 16-frame 1920x1080 synthetic render stream (global pan U[-3,3] px/frame, 10% random
+disocclusion), one per-frame online training step per frame: cross-regression, reprojection,
+erosion + temporal screening, bandwidth CNN forward (5 conv3x3, 32 ch, BF16 operands, FP32
+accumulation), fused spatial filter + temporal accumulation + loss + backward, CNN backward, Adam.
+Inputs are generated on the GPU and resident in HBM before the timed region.
+
+Prints one JSON line. `roofline` is the dominant custom kernel, timed with HIP events on the stream
+it runs on; `cpu_baseline` is the FP64 oracle (plus a torch-CPU CNN step) on a bounded crop.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "cross-denoiser_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PUBLISHED_FPS = 1000.0 / 22.26  # paper Table 3 total per 1080p frame (RTX 4090)
+
+# Algorithmic work per pixel (both halves where applicable), FMA = 2 flops; see DESIGN.md.
+ALGO = {
+    "regression": {"flop_per_px": 5200.0, "bytes_per_px": 65.0},
+    "filter_train": {"flop_per_px": 14700.0, "bytes_per_px": 152.0},
+    "recon": {"flop_per_px": 2100.0, "bytes_per_px": 89.0},
+    "reproject": {"flop_per_px": 150.0, "bytes_per_px": 153.0},
+    "screening": {"flop_per_px": 1200.0, "bytes_per_px": 108.0},
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--frames", type=int, default=16)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    return ap.parse_args()
+
+
+def stage_times(den, frames, reps, torch):
+    """Average duration of each stage, HIP events on the stream the kernels run on."""
+    import ctypes
+
+    from xdn import pipeline
+    from xdn._lib import check, lib, ptr
+    b = den.buf
+    B, H, W = b.shape
+    f = frames[1]
+    st = torch.cuda.current_stream()
+    s = st.cuda_stream
+    prv = b.hist[0]
+    out_log, ols1, ols2 = b.hist[1]
+    ins = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in prv])
+    outs = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in b.rp])
+    dims = (ctypes.c_int * 3)(3, 3, 3)
+    param = den.net.forward(ols1, ols2, b.prev_out, f["albedo"], f["normal"])
+    den.net._out = None
+
+    def t_cr():
+        check(lib.xdn_cross_regression_from_color(ptr(f["color"]), ptr(f["color2"]), ptr(f["albedo"]),
+                                                  ptr(f["normal"]), ptr(b.y1), ptr(b.y2), ptr(ols1), ptr(ols2),
+                                                  ptr(b.var1), ptr(b.var2), ptr(b.beta), b.beta_bytes, B, H, W, 17, 4,
+                                                  s), "cr")
+
+    def t_rp():
+        check(lib.xdn_reproject(ptr(f["mvec"]), 3, ins, dims, ptr(f["linearZ"]), ptr(den.prev_linz), ptr(f["normal"]),
+                                ptr(den.prev_nrm), ptr(f["pnFwidth"]), ptr(b.ones), ptr(b.ones), ptr(b.success), outs,
+                                B, H, W, s), "rp")
+
+    def t_ts():
+        check(lib.xdn_erode_temporal_screening(ptr(ols1), ptr(ols2), ptr(b.rp[0]), ptr(b.rp[1]), ptr(b.rp[2]),
+                                               ptr(b.success), ptr(b.succ_e), ptr(b.prev_out), ptr(b.prev_ols1),
+                                               ptr(b.prev_ols2), ptr(b.success_prev), B, H, W, 7, 1, s), "ts")
+
+    def t_ft():
+        check(lib.xdn_filter_train(ptr(ols1), ptr(ols2), ptr(f["albedo"]), ptr(f["normal"]), ptr(param),
+                                   ptr(b.succ_e), ptr(b.success_prev), ptr(b.prev_out), ptr(f["opacity"]),
+                                   ptr(b.prev_ols1), ptr(b.prev_ols2), ptr(out_log), ptr(b.out), ptr(b.dparam), None,
+                                   ptr(b.avg_loss), ptr(b.ws), b.ws_bytes, B, H, W, 11, s), "ft")
+
+    def t_cnn():
+        den.net.forward(ols1, ols2, b.prev_out, f["albedo"], f["normal"])
+        den.net.backward(b.dparam)
+        for p in den.net.params:
+            p.grad = None
+
+    out = {}
+    for name, fn in (("cross_regression", t_cr), ("reproject", t_rp), ("screening", t_ts), ("filter_train", t_ft),
+                     ("cnn_fwd_bwd", t_cnn)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps
+    return out
+
+
+def cpu_baseline(torch, H, W):
+    """Oracle (FP64, all host threads) + torch-CPU CNN step on a 256x256 crop, scaled to 1080p."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from xdn import cnn as cnn_mod
+    from xdn import synth
+    h, w = 256, 256
+    spec = synth.StreamSpec(h, w, seed=1, pan=(1.0, -1.0), disocclusion=0.1)
+    f0, f1 = synth.make_frame(spec, 0), synth.make_frame(spec, 1)
+    n = lambda t: t.numpy()
+    t0 = time.perf_counter()
+    y1, y2 = O.prepare_input(n(f1["color"])).astype(np.float32), O.prepare_input(n(f1["color2"])).astype(np.float32)
+    o1, o2, _, _ = O.cross_regression(y1, y2, n(f1["albedo"]), n(f1["normal"]))
+    o1, o2 = o1.astype(np.float32), o2.astype(np.float32)
+    s, rp = O.reproject(n(f1["mvec"]), [o1, o1, o2], n(f1["linearZ"]), n(f0["linearZ"]), n(f1["normal"]),
+                        n(f0["normal"]), n(f1["pnFwidth"]), n(f1["opacity"]), n(f1["opacity"]))
+    se = O.erosion(s)
+    pv, p1, p2, sp = O.temporal_screening(o1, o2, *[r.astype(np.float32) for r in rp], se)
+    band = np.ones((1, h, w, 5), np.float32)
+    acc = O.spatial_filter(o1, o2, n(f1["albedo"]), n(f1["normal"]), band)
+    O.spatial_filter_grad(acc[0].astype(np.float32), acc[1].astype(np.float32), acc[2].astype(np.float32),
+                          acc[3].astype(np.float32), o1, o2, n(f1["albedo"]), n(f1["normal"]), band)
+    t_filter = time.perf_counter() - t0
+    # CNN forward + backward in FP32 on the CPU
+    params = [p.clone().requires_grad_(True) for wb in cnn_mod.init_params() for p in
+              (wb[0].permute(3, 2, 0, 1).contiguous(), wb[1])]
+    x = torch.randn(1, 15, h, w)
+    t0 = time.perf_counter()
+    hcur = x
+    for i in range(5):
+        hcur = torch.nn.functional.conv2d(hcur, params[2 * i], params[2 * i + 1], padding=1)
+        if i < 4:
+            hcur = torch.relu(hcur)
+    hcur.sum().backward()
+    t_cnn = time.perf_counter() - t0
+    scale = (H * W) / (h * w)
+    per_frame = (t_filter + t_cnn) * scale
+    return {"value": round(1.0 / per_frame, 4), "unit": "frames/s", "cores": int(torch.get_num_threads()),
+            "kind": "port",
+            "sample": f"one {h}x{w} frame: FP64 oracle filtering path (cross-regression, reprojection, erosion, "
+                      f"screening, spatial filter + band gradient; {os.cpu_count()} host threads) + torch-CPU FP32 "
+                      f"CNN fwd+bwd; time scaled by pixel count to {W}x{H}",
+            "filter_s": round(t_filter, 3), "cnn_s": round(t_cnn, 3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by render stream, "
+              "see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from xdn import pipeline, synth
+
+    dev = torch.device("cuda:0")
+    H, W = args.height, args.width
+    spec = synth.StreamSpec(H, W, seed=args.seed, pan=None, pan_max=3.0, disocclusion=0.10)
+    frames = [synth.make_frame(spec, t, device=dev) for t in range(args.frames)]
+    den = pipeline.Denoiser(H, W, device=dev)
+    torch.cuda.synchronize()
+    print(f"[bench] {H}x{W}, {args.frames} frames resident, warmup {args.warmup}", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        den.step(frames[i % args.frames])
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(st)
+    for i in range(args.steps):
+        den.step(frames[(args.warmup + i) % args.frames])
+        evs[i + 1].record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)])
+    gpu_ms = evs[0].elapsed_time(evs[-1])
+    secs = max(wall, gpu_ms / 1e3)
+    fps = args.steps / secs
+    print(f"[bench] timed {args.steps} frames: {fps:.1f} frames/s", file=sys.stderr, flush=True)
+
+    stages = stage_times(den, frames, args.kernel_reps, torch)
+    print(f"[bench] stages (ms): {stages}", file=sys.stderr, flush=True)
+    # dominant custom kernel
+    dom = max(("filter_train", "cross_regression"), key=lambda k: stages[k])
+    npx = H * W
+    if dom == "filter_train":
+        flops = ALGO["filter_train"]["flop_per_px"] * npx
+        name = "sf_train_tile (fused spatial filter + aggregation + backward)"
+    else:
+        flops = (ALGO["regression"]["flop_per_px"] + ALGO["recon"]["flop_per_px"]) * npx
+        name = "cross_regression (variance + regression_rows + recon_tile)"
+    achieved = flops / (stages[dom] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": name,
+                "note": "VALU-bound FP32 kernel; peak is the published FP32 rate"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(torch, H, W)
+    line = {
+        "metric": "1080p denoised frames/sec (fwd+online bwd) + p99 frame latency, 1 GPU",
+        "value": round(fps, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * secs / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(fps / PUBLISHED_FPS, 3), "dtype": "fp32 (filtering) + bf16 MFMA/fp32 acc (CNN)",
+        "data": "synthetic",
+        "config": {"workload": f"{W}x{H} {args.frames}-frame stream, online training step per frame "
+                               "(cnn5 5x conv3x3 32ch)", "height": H, "width": W, "frames": args.frames,
+                   "parallelism": "single stream"},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

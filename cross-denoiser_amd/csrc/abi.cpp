@@ -1,0 +1,373 @@
+// extern "C" entry points of libxdn.so (declared in include/xdn.h): argument validation, batch
+// splitting into launches of at most XDN_MAX_FRAMES frames, error reporting.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/xdn.h"
+#include "xdn_common.h"
+#include "xdn_kernels.h"
+#include "xdn_misc.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) return fail(XDN_EHIP, "%s: %s", what, hipGetErrorString(e));
+    return XDN_OK;
+}
+
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline size_t px(int H, int W) { return (size_t)H * (size_t)W; }
+
+int check_shape(int B, int H, int W)
+{
+    if (B < 0 || H <= 0 || W <= 0) return fail(XDN_EINVAL, "invalid shape [%d, %d, %d]", B, H, W);
+    return XDN_OK;
+}
+
+template <class Fill, class Launch>
+int sf_run(int B, int H, int W, int win, Fill fill, Launch launch, const char *what)
+{
+    for (int b0 = 0; b0 < B; b0 += XDN_MAX_FRAMES)
+    {
+        const int n = B - b0 < XDN_MAX_FRAMES ? B - b0 : XDN_MAX_FRAMES;
+        std::vector<xdn::SfFrame> fr(n);
+        for (int i = 0; i < n; ++i)
+        {
+            memset(&fr[i], 0, sizeof(xdn::SfFrame));
+            fill(fr[i], (size_t)(b0 + i));
+            fr[i].H = H;
+            fr[i].W = W;
+        }
+        if (int e = hip_check(launch(n, fr.data()), what)) return e;
+    }
+    return XDN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *xdn_version(void) { return "xdn 0.1 gfx950"; }
+
+const char *xdn_last_error(void) { return g_err; }
+
+size_t xdn_cross_regression_workspace_size(int batch, int height, int width, int sparsity)
+{
+    if (batch <= 0 || height <= 0 || width <= 0 || sparsity <= 0) return 0;
+    return (size_t)batch * (size_t)(height / sparsity) * (size_t)(width / sparsity) * 36 * sizeof(float);
+}
+
+static int cross_regression_impl(const float *color, const float *color2, const float *imga, const float *imgb,
+                                 const float *albedo, const float *normal, float *y1, float *y2, float *outa,
+                                 float *outb, float *vara, float *varb, void *ws, size_t ws_bytes, int B, int H,
+                                 int W, int win, int s, int rs, void *stream)
+{
+    if (int e = check_shape(B, H, W)) return e;
+    if (win < 1 || s < 1 || rs < 1) return fail(XDN_EINVAL, "win_size, sparsity and recon_scale must be >= 1");
+    if (rs > s) return fail(XDN_EINVAL, "recon_scale must be less than or equal to sparsity");
+    if (H % rs != 0 || W % rs != 0) return fail(XDN_EINVAL, "Image height and width must be divisible by recon_scale");
+    if (H % s != 0 || W % s != 0) return fail(XDN_EINVAL, "Image height and width must be divisible by sparsity");
+    if (H <= win / 2 || W <= win / 2) return fail(XDN_EINVAL, "Image height and width must exceed win_size/2");
+    if (ws_bytes < xdn_cross_regression_workspace_size(B, H, W, s))
+        return fail(XDN_EINVAL, "workspace too small: %zu < %zu", ws_bytes, xdn_cross_regression_workspace_size(B, H, W, s));
+    const bool fused = color != nullptr;
+    const size_t N = px(H, W), Nr = px(H / rs, W / rs), Nb = (size_t)(H / s) * (W / s) * 36;
+    for (int b0 = 0; b0 < B; b0 += XDN_MAX_FRAMES / 2)
+    {
+        const int n = B - b0 < XDN_MAX_FRAMES / 2 ? B - b0 : XDN_MAX_FRAMES / 2;
+        std::vector<xdn::CrFrame> fr(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const size_t b = (size_t)(b0 + i);
+            xdn::CrFrame &f = fr[i];
+            f.color = fused ? color + b * N * 3 : nullptr;
+            f.color2 = fused ? color2 + b * N * 3 : nullptr;
+            f.y1 = fused ? y1 + b * N * 3 : const_cast<float *>(imga + b * N * 3);
+            f.y2 = fused ? y2 + b * N * 3 : const_cast<float *>(imgb + b * N * 3);
+            f.var1 = vara + b * N;
+            f.var2 = varb + b * N;
+            f.alb = albedo + b * N * 3;
+            f.nrm = normal + b * N * 3;
+            f.beta = static_cast<float *>(ws) + b * Nb;
+            f.outa = outa + b * Nr * 3;
+            f.outb = outb + b * Nr * 3;
+            f.H = H;
+            f.W = W;
+        }
+        if (int e = hip_check(xdn::launch_variance(n, fr.data(), 3, fused, S(stream)), "variance")) return e;
+        if (int e = hip_check(xdn::launch_regression(n, fr.data(), win, s, S(stream)), "regression")) return e;
+        if (int e = hip_check(xdn::launch_recon(n, fr.data(), win, s, rs, S(stream)), "recon")) return e;
+    }
+    return XDN_OK;
+}
+
+int xdn_cross_regression(const float *imga, const float *imgb, const float *albedo, const float *normal, float *outa,
+                         float *outb, float *out_vara, float *out_varb, void *workspace, size_t workspace_bytes,
+                         int batch, int height, int width, int win_size, int sparsity, int recon_scale, void *stream)
+{
+    return cross_regression_impl(nullptr, nullptr, imga, imgb, albedo, normal, nullptr, nullptr, outa, outb, out_vara,
+                                 out_varb, workspace, workspace_bytes, batch, height, width, win_size, sparsity,
+                                 recon_scale, stream);
+}
+
+int xdn_cross_regression_from_color(const float *color, const float *color2, const float *albedo, const float *normal,
+                                    float *y1, float *y2, float *outa, float *outb, float *out_vara, float *out_varb,
+                                    void *workspace, size_t workspace_bytes, int batch, int height, int width,
+                                    int win_size, int sparsity, void *stream)
+{
+    if (!color || !color2 || !y1 || !y2) return fail(XDN_EINVAL, "color, color2, y1 and y2 are required");
+    return cross_regression_impl(color, color2, nullptr, nullptr, albedo, normal, y1, y2, outa, outb, out_vara,
+                                 out_varb, workspace, workspace_bytes, batch, height, width, win_size, sparsity, 1,
+                                 stream);
+}
+
+static int sf_common_check(int B, int H, int W, int win)
+{
+    if (int e = check_shape(B, H, W)) return e;
+    if (win < 1) return fail(XDN_EINVAL, "win_size must be >= 1");
+    if (win / 2 < 1) return fail(XDN_EINVAL, "win_size must be >= 2");
+    if (H <= win / 2 || W <= win / 2) return fail(XDN_EINVAL, "Image height and width must exceed win_size/2");
+    return XDN_OK;
+}
+
+int xdn_spatial_filter(const float *imga, const float *imgb, const float *albedo, const float *normal,
+                       const float *band, float *out_acca, float *out_accb, float *out_wgtsuma, float *out_wgtsumb,
+                       int batch, int height, int width, int win_size, void *stream)
+{
+    if (int e = sf_common_check(batch, height, width, win_size)) return e;
+    const size_t N = px(height, width);
+    return sf_run(
+        batch, height, width, win_size,
+        [&](xdn::SfFrame &f, size_t b) {
+            f.ia = imga + b * N * 3; f.ib = imgb + b * N * 3; f.alb = albedo + b * N * 3; f.nrm = normal + b * N * 3;
+            f.band = band + b * N * 5;
+            f.accA = out_acca + b * N * 3; f.accB = out_accb + b * N * 3;
+            f.wsA = out_wgtsuma + b * N; f.wsB = out_wgtsumb + b * N;
+        },
+        [&](int n, const xdn::SfFrame *fr) { return xdn::launch_sf_forward(n, fr, win_size, S(stream)); },
+        "spatial_filter");
+}
+
+int xdn_spatial_filter_grad(const float *grad_acca, const float *grad_accb, const float *grad_wgtsuma,
+                            const float *grad_wgtsumb, const float *imga, const float *imgb, const float *img_albedo,
+                            const float *img_normal, const float *band, float *out_grad_band, int batch, int height,
+                            int width, int win_size, void *stream)
+{
+    if (int e = sf_common_check(batch, height, width, win_size)) return e;
+    const size_t N = px(height, width);
+    return sf_run(
+        batch, height, width, win_size,
+        [&](xdn::SfFrame &f, size_t b) {
+            f.ia = imga + b * N * 3; f.ib = imgb + b * N * 3; f.alb = img_albedo + b * N * 3;
+            f.nrm = img_normal + b * N * 3; f.band = band + b * N * 5;
+            f.gaccA = grad_acca + b * N * 3; f.gaccB = grad_accb + b * N * 3;
+            f.gwA = grad_wgtsuma + b * N; f.gwB = grad_wgtsumb + b * N;
+            f.gband = out_grad_band + b * N * 5;
+        },
+        [&](int n, const xdn::SfFrame *fr) { return xdn::launch_sf_grad_band(n, fr, win_size, S(stream)); },
+        "spatial_filter_grad");
+}
+
+int xdn_spatial_filter_grad_inputs(const float *grad_acca, const float *grad_accb, const float *grad_wgtsuma,
+                                   const float *grad_wgtsumb, const float *imga, const float *imgb,
+                                   const float *img_albedo, const float *img_normal, const float *band,
+                                   float *grad_imga, float *grad_imgb, float *grad_albedo, float *grad_normal,
+                                   int batch, int height, int width, int win_size, void *stream)
+{
+    if (int e = sf_common_check(batch, height, width, win_size)) return e;
+    if (win_size / 2 > 7) return fail(XDN_EINVAL, "input gradients support win_size <= 15");
+    const size_t N = px(height, width);
+    auto off = [](float *p, size_t o) { return p ? p + o : nullptr; };
+    return sf_run(
+        batch, height, width, win_size,
+        [&](xdn::SfFrame &f, size_t b) {
+            f.ia = imga + b * N * 3; f.ib = imgb + b * N * 3; f.alb = img_albedo + b * N * 3;
+            f.nrm = img_normal + b * N * 3; f.band = band + b * N * 5;
+            f.gaccA = grad_acca + b * N * 3; f.gaccB = grad_accb + b * N * 3;
+            f.gwA = grad_wgtsuma + b * N; f.gwB = grad_wgtsumb + b * N;
+            f.gia = off(grad_imga, b * N * 3); f.gib = off(grad_imgb, b * N * 3);
+            f.galb = off(grad_albedo, b * N * 3); f.gnrm = off(grad_normal, b * N * 3);
+        },
+        [&](int n, const xdn::SfFrame *fr) { return xdn::launch_sf_grad_inputs(n, fr, win_size, S(stream)); },
+        "spatial_filter_grad_inputs");
+}
+
+int xdn_reproject(const float *mvec, int n_list, const float *const *inputs, const int *dims, const float *linear_z,
+                  const float *prev_linear_z, const float *normal, const float *prev_normal, const float *pn_fwidth,
+                  const float *opacity, const float *prev_opacity, uint8_t *success, float *const *outputs, int batch,
+                  int height, int width, void *stream)
+{
+    if (int e = check_shape(batch, height, width)) return e;
+    if (n_list < 0) return fail(XDN_EINVAL, "n_list must be >= 0");
+    for (int l = 0; l < n_list; ++l)
+        if (dims[l] < 1 || dims[l] > 4) return fail(XDN_EINVAL, "input_list[%d] has %d channels (1..4 supported)", l, dims[l]);
+    const size_t N = px(height, width);
+    // buffers beyond XDN_MAX_LIST go to further launches (the mask is recomputed, identically)
+    int l0 = 0;
+    do
+    {
+        const int nl = n_list - l0 < XDN_MAX_LIST ? n_list - l0 : XDN_MAX_LIST;
+        for (int b0 = 0; b0 < batch; b0 += XDN_MAX_FRAMES)
+        {
+            const int n = batch - b0 < XDN_MAX_FRAMES ? batch - b0 : XDN_MAX_FRAMES;
+            std::vector<xdn::RpFrame> fr(n);
+            for (int i = 0; i < n; ++i)
+            {
+                const size_t b = (size_t)(b0 + i);
+                xdn::RpFrame &f = fr[i];
+                memset(&f, 0, sizeof(f));
+                f.mvec = mvec + b * N * 3; f.linZ = linear_z + b * N * 3; f.prevLinZ = prev_linear_z + b * N * 3;
+                f.nrm = normal + b * N * 3; f.prevNrm = prev_normal + b * N * 3; f.pnFw = pn_fwidth + b * N * 3;
+                f.opac = opacity + b * N; f.prevOpac = prev_opacity + b * N; f.success = success + b * N;
+                for (int l = 0; l < nl; ++l)
+                {
+                    f.in[l] = inputs[l0 + l] + b * N * dims[l0 + l];
+                    f.out[l] = outputs[l0 + l] + b * N * dims[l0 + l];
+                }
+                f.H = height;
+                f.W = width;
+            }
+            if (int e = hip_check(xdn::launch_reproject(n, fr.data(), nl, dims + l0, S(stream)), "reproject")) return e;
+        }
+        l0 += nl;
+    } while (l0 < n_list);
+    return XDN_OK;
+}
+
+static int screening_impl(const float *c1, const float *c2, const float *prev, const float *prev1, const float *prev2,
+                          const float *rsucc, const uint8_t *mask, float *eroded, float *o, float *o1, float *o2,
+                          float *osucc, int B, int H, int W, int win_curr, int win_prev, void *stream)
+{
+    if (int e = check_shape(B, H, W)) return e;
+    if (win_curr < 1 || win_prev < 1) return fail(XDN_EINVAL, "win_curr and win_prev must be >= 1");
+    const size_t N = px(H, W);
+    for (int b0 = 0; b0 < B; b0 += XDN_MAX_FRAMES)
+    {
+        const int n = B - b0 < XDN_MAX_FRAMES ? B - b0 : XDN_MAX_FRAMES;
+        std::vector<xdn::TsFrame> fr(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const size_t b = (size_t)(b0 + i);
+            xdn::TsFrame &f = fr[i];
+            f.c1 = c1 + b * N * 3; f.c2 = c2 + b * N * 3; f.prev = prev + b * N * 3;
+            f.prev1 = prev1 + b * N * 3; f.prev2 = prev2 + b * N * 3;
+            f.rsucc = rsucc ? rsucc + b * N : nullptr;
+            f.succ_mask = mask ? mask + b * N : nullptr;
+            f.succ_e = eroded ? eroded + b * N : nullptr;
+            f.o = o + b * N * 3; f.o1 = o1 + b * N * 3; f.o2 = o2 + b * N * 3; f.osucc = osucc + b * N;
+            f.H = H;
+            f.W = W;
+        }
+        if (int e = hip_check(xdn::launch_screening(n, fr.data(), win_curr, win_prev, mask != nullptr, S(stream)),
+                              "temporal_screening"))
+            return e;
+    }
+    return XDN_OK;
+}
+
+int xdn_temporal_screening(const float *current1, const float *current2, const float *prev, const float *prev1,
+                           const float *prev2, const float *reproj_success, float *output, float *output1,
+                           float *output2, float *success, int batch, int height, int width, int win_curr,
+                           int win_prev, void *stream)
+{
+    if (!reproj_success) return fail(XDN_EINVAL, "reproj_success is required");
+    return screening_impl(current1, current2, prev, prev1, prev2, reproj_success, nullptr, nullptr, output, output1,
+                          output2, success, batch, height, width, win_curr, win_prev, stream);
+}
+
+int xdn_erode_temporal_screening(const float *current1, const float *current2, const float *prev, const float *prev1,
+                                 const float *prev2, const uint8_t *success_mask, float *eroded, float *output,
+                                 float *output1, float *output2, float *success, int batch, int height, int width,
+                                 int win_curr, int win_prev, void *stream)
+{
+    if (!success_mask) return fail(XDN_EINVAL, "success_mask is required");
+    return screening_impl(current1, current2, prev, prev1, prev2, nullptr, success_mask, eroded, output, output1,
+                          output2, success, batch, height, width, win_curr, win_prev, stream);
+}
+
+size_t xdn_filter_train_workspace_size(int batch, int height, int width)
+{
+    if (batch <= 0 || height <= 0 || width <= 0) return 0;
+    return (size_t)batch * (size_t)xdn::filter_train_tiles(height, width) * sizeof(float);
+}
+
+int xdn_filter_train(const float *ols1, const float *ols2, const float *albedo, const float *normal, const float *param,
+                     const float *success, const float *success_prev, const float *prev_out, const float *opacity,
+                     const float *prev_ols1, const float *prev_ols2, float *out_log, float *out, float *dparam,
+                     float *loss, float *avg_loss, void *workspace, size_t workspace_bytes, int batch, int height,
+                     int width, int win_size, void *stream)
+{
+    if (int e = sf_common_check(batch, height, width, win_size)) return e;
+    if (win_size / 2 != 5) return fail(XDN_EINVAL, "the fused training kernel supports win_size 11 (half 5) only");
+    if (workspace_bytes < xdn_filter_train_workspace_size(batch, height, width))
+        return fail(XDN_EINVAL, "workspace too small");
+    const size_t N = px(height, width);
+    const int T = xdn::filter_train_tiles(height, width);
+    float *parts = static_cast<float *>(workspace);
+    for (int b0 = 0; b0 < batch; b0 += XDN_MAX_FRAMES)
+    {
+        const int n = batch - b0 < XDN_MAX_FRAMES ? batch - b0 : XDN_MAX_FRAMES;
+        std::vector<xdn::TrainFrame> fr(n);
+        std::vector<const float *> pp(n);
+        std::vector<int> cnt(n);
+        std::vector<float> sc(n);
+        std::vector<float *> outp(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const size_t b = (size_t)(b0 + i);
+            xdn::TrainFrame &f = fr[i];
+            f.ols1 = ols1 + b * N * 3; f.ols2 = ols2 + b * N * 3; f.alb = albedo + b * N * 3; f.nrm = normal + b * N * 3;
+            f.param = param + b * N * 6; f.succ_e = success + b * N; f.succ_prev = success_prev + b * N;
+            f.prev_out = prev_out + b * N * 3; f.opacity = opacity + b * N;
+            f.prev_ols1 = prev_ols1 + b * N * 3; f.prev_ols2 = prev_ols2 + b * N * 3;
+            f.out_log = out_log + b * N * 3; f.out = out + b * N * 3; f.dparam = dparam + b * N * 6;
+            f.loss = loss ? loss + b * N * 3 : nullptr;
+            f.loss_part = parts + b * T;
+            f.H = height;
+            f.W = width;
+            pp[i] = f.loss_part;
+            cnt[i] = T;
+            sc[i] = (float)(1.0 / (3.0 * (double)N));
+            outp[i] = avg_loss + b;
+        }
+        if (int e = hip_check(xdn::launch_filter_train(n, fr.data(), win_size, S(stream)), "filter_train")) return e;
+        if (avg_loss)
+            if (int e = hip_check(xdn::launch_sums(n, pp.data(), cnt.data(), sc.data(), outp.data(), S(stream)), "loss sum"))
+                return e;
+    }
+    return XDN_OK;
+}
+
+int xdn_adam(float *param, const float *grad, float *m, float *v, int count, int step, float lr, float beta1,
+             float beta2, float eps, void *stream)
+{
+    if (count < 0 || step < 1) return fail(XDN_EINVAL, "count must be >= 0 and step >= 1");
+    xdn::AdamJob j{};
+    j.param = param;
+    j.grad = const_cast<float *>(grad);
+    j.m = m;
+    j.v = v;
+    j.count = count;
+    j.alpha = (float)((double)lr * sqrt(1.0 - pow((double)beta2, step)) / (1.0 - pow((double)beta1, step)));
+    return hip_check(xdn::launch_adam(1, &j, beta1, beta2, eps, S(stream)), "adam");
+}
+
+}  // extern "C"

@@ -1,0 +1,634 @@
+// Cross-regression on gfx950: input variance, per-center weighted ridge regression and block
+// reconstruction (the cross_regression op, regression.cu.cc:244-500).
+//
+// Data layout in HBM: images are NHWC float32 with 3 channels (12 B/px) exactly as the op
+// boundary hands them over. The regression coefficients are an internal workspace laid out as
+// [B][H/s][W/s][2 halves][6 coeffs][3 channels] float32 (144 B per center).
+//
+// Kernels
+//   variance_kernel    3x3 clamped neighbourhood variance (EstInputVarianceKernel :244-276),
+//                      HBM-bound, 1 px/thread. Optionally fuses log1p of the raw colour
+//                      (scripts/main.py:207-212) and writes the log image.
+//   regression_rows    fast path (win 17, sparsity 4): one workgroup = 64 centers of one center
+//                      row, both halves; the 17 tap rows stream through a double-buffered LDS
+//                      row (4 float4 planes per pixel) and each (center, half) is served by two
+//                      lanes that split the 17 tap columns 9/8 and are reduced with a lane swap.
+//                      61 FP32 accumulators per lane; 6x6 Cholesky per channel spread over the
+//                      lane pair. VALU-bound.
+//   recon_tile         fast path (win 17, sparsity 4, recon_scale 1): 16x16 pixels per
+//                      workgroup; the <= 8x8 centers the tile touches are staged in LDS (both
+//                      halves' coefficients and features), one pixel per lane, both halves.
+//   regression_generic / recon_generic: any win_size / sparsity / recon_scale, direct loads.
+#include "xdn_common.h"
+#include "xdn_kernels.h"
+
+namespace xdn {
+
+// ---------------------------------------------------------------------------------------------
+// Variance (and optional log1p prologue)
+// ---------------------------------------------------------------------------------------------
+struct VarFrame {
+    const float *in;   // y (already log) or raw colour when log_in is set
+    float *y_out;      // log1p(colour) when log_in is set, else unused
+    float *var;        // [H][W]
+    int H, W;
+};
+struct VarBatch {
+    TileMap map;
+    VarFrame f[XDN_MAX_FRAMES];
+    int log_in;
+};
+
+constexpr int VT_X = 32, VT_Y = 8;
+
+__global__ __launch_bounds__(256) void variance_kernel(VarBatch b)
+{
+    __shared__ float s[3][VT_Y + 2][VT_X + 2];
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const VarFrame &F = b.f[L.f];
+    const int H = F.H, W = F.W;
+    const int x0 = L.tx * VT_X, y0 = L.ty * VT_Y;
+    for (int i = threadIdx.x; i < (VT_X + 2) * (VT_Y + 2); i += 256)
+    {
+        const int ly = i / (VT_X + 2), lx = i - ly * (VT_X + 2);
+        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+        float3 v = make_float3(0.f, 0.f, 0.f);
+        if (gx >= 0 && gy >= 0 && gx < W && gy < H)
+        {
+            v = ld3(F.in, (size_t)gy * W + gx);
+            if (b.log_in)
+            {
+                v = make_float3(log1pf(v.x), log1pf(v.y), log1pf(v.z));
+                if (lx >= 1 && ly >= 1 && lx <= VT_X && ly <= VT_Y) st3(F.y_out, (size_t)gy * W + gx, v);
+            }
+        }
+        s[0][ly][lx] = v.x;
+        s[1][ly][lx] = v.y;
+        s[2][ly][lx] = v.z;
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % VT_X, ly = threadIdx.x / VT_X;
+    const int cx = x0 + lx, cy = y0 + ly;
+    if (cx >= W || cy >= H) return;
+    float acc[3] = {0.f, 0.f, 0.f};
+    int n = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+        {
+            const int gx = cx + dx, gy = cy + dy;
+            if ((dx | dy) == 0 || gx < 0 || gy < 0 || gx >= W || gy >= H) continue;
+            for (int k = 0; k < 3; ++k) acc[k] += s[k][ly + 1 + dy][lx + 1 + dx];
+            ++n;
+        }
+    const float inv = 1.f / (float)n;
+    float v = 0.f;
+    for (int k = 0; k < 3; ++k)
+    {
+        const float d = acc[k] * inv - s[k][ly + 1][lx + 1];
+        v += d * d;
+    }
+    F.var[(size_t)cy * W + cx] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Regression
+// ---------------------------------------------------------------------------------------------
+struct RegFrame {
+    const float *y1, *y2, *var1, *var2, *alb, *nrm;
+    float *beta;  // [Hs][Ws][2][18]
+    int H, W;
+};
+struct RegBatch {
+    TileMap map;
+    RegFrame f[XDN_MAX_FRAMES];
+    int half_win, sparsity;  // generic kernel only
+};
+
+// Upper-triangular 6x6 normal equations of one channel plus the right-hand side.
+struct Sys6 {
+    float a[21];  // row-major upper triangle: (0,0) (0,1) .. (0,5) (1,1) .. (5,5)
+    float x[6];
+};
+
+__device__ __forceinline__ constexpr int up6(int r, int c) { return r * 6 - (r * (r - 1)) / 2 + (c - r); }
+
+// Ridge + Cholesky (cholesky_color :206-218 with the SQRT clamp :197-204) + forward/backward
+// substitution (:220-242) for one channel; all indices compile-time.
+__device__ __forceinline__ void solve6(const Sys6 &s, float beta[6])
+{
+    float L[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j)
+        {
+            float acc = 0.f;
+#pragma unroll
+            for (int k = 0; k < j; ++k) acc += L[i][k] * L[j][k];
+            const float aji = s.a[up6(j, i)] + (i == j ? 1e-3f : 0.f);
+            if (i == j)
+                L[i][i] = sqrtf(fmaxf(1e-5f, aji - acc));
+            else
+                L[i][j] = (1.f / L[j][j]) * (aji - acc);
+        }
+    float w[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+    {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < i; ++k) acc += L[i][k] * w[k];
+        w[i] = (s.x[i] - acc) / L[i][i];
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; --i)
+    {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 5; k > i; --k) acc += L[k][i] * beta[k];
+        beta[i] = (w[i] - acc) / L[i][i];
+    }
+}
+
+// The 61 distinct accumulators of the three per-channel 6x6 systems. Feature order per channel:
+// [1, d1_c, d2_c, n0, n1, n2]; entries among {0,3,4,5} do not depend on the channel.
+struct RegAcc {
+    float s00, s03, s04, s05, s33, s34, s35, s44, s45, s55;  // shared
+    float c01[3], c02[3], c11[3], c12[3], c22[3], c13[3], c14[3], c15[3], c23[3], c24[3], c25[3];
+    float x0[3], x1[3], x2[3], x3[3], x4[3], x5[3];
+
+    __device__ __forceinline__ void zero()
+    {
+        s00 = s03 = s04 = s05 = s33 = s34 = s35 = s44 = s45 = s55 = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            c01[c] = c02[c] = c11[c] = c12[c] = c22[c] = c13[c] = c14[c] = c15[c] = c23[c] = c24[c] =
+                c25[c] = x0[c] = x1[c] = x2[c] = x3[c] = x4[c] = x5[c] = 0.f;
+    }
+
+    // One tap: weight w, 1/(std_c + std_i + 1e-2), own-half colour difference dw (tap - center),
+    // albedo difference d2, normal difference n, target colour t.
+    __device__ __forceinline__ void add(float w, float inv, float3 dw, float3 d2, float3 n, float3 t)
+    {
+        const float wn0 = w * n.x, wn1 = w * n.y, wn2 = w * n.z;
+        s00 += w;
+        s03 += wn0;
+        s04 += wn1;
+        s05 += wn2;
+        s33 += wn0 * n.x;
+        s34 += wn0 * n.y;
+        s35 += wn0 * n.z;
+        s44 += wn1 * n.y;
+        s45 += wn1 * n.z;
+        s55 += wn2 * n.z;
+        const float d1v[3] = {dw.x * inv, dw.y * inv, dw.z * inv};
+        const float d2v[3] = {d2.x, d2.y, d2.z};
+        const float tv[3] = {t.x, t.y, t.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            const float d1 = d1v[c], dd2 = d2v[c], tt = tv[c];
+            const float wd1 = w * d1, wd2 = w * dd2;
+            c01[c] += wd1;
+            c02[c] += wd2;
+            c11[c] += wd1 * d1;
+            c12[c] += wd1 * dd2;
+            c22[c] += wd2 * dd2;
+            c13[c] += wd1 * n.x;
+            c14[c] += wd1 * n.y;
+            c15[c] += wd1 * n.z;
+            c23[c] += wd2 * n.x;
+            c24[c] += wd2 * n.y;
+            c25[c] += wd2 * n.z;
+            x0[c] += w * tt;
+            x1[c] += wd1 * tt;
+            x2[c] += wd2 * tt;
+            x3[c] += wn0 * tt;
+            x4[c] += wn1 * tt;
+            x5[c] += wn2 * tt;
+        }
+    }
+
+    // System of channel c (c compile-time).
+    __device__ __forceinline__ void system(int c, Sys6 &s) const
+    {
+        // row 0
+        s.a[up6(0, 0)] = s00; s.a[up6(0, 1)] = c01[c]; s.a[up6(0, 2)] = c02[c];
+        s.a[up6(0, 3)] = s03; s.a[up6(0, 4)] = s04; s.a[up6(0, 5)] = s05;
+        s.a[up6(1, 1)] = c11[c]; s.a[up6(1, 2)] = c12[c]; s.a[up6(1, 3)] = c13[c];
+        s.a[up6(1, 4)] = c14[c]; s.a[up6(1, 5)] = c15[c];
+        s.a[up6(2, 2)] = c22[c]; s.a[up6(2, 3)] = c23[c]; s.a[up6(2, 4)] = c24[c]; s.a[up6(2, 5)] = c25[c];
+        s.a[up6(3, 3)] = s33; s.a[up6(3, 4)] = s34; s.a[up6(3, 5)] = s35;
+        s.a[up6(4, 4)] = s44; s.a[up6(4, 5)] = s45;
+        s.a[up6(5, 5)] = s55;
+        s.x[0] = x0[c]; s.x[1] = x1[c]; s.x[2] = x2[c]; s.x[3] = x3[c]; s.x[4] = x4[c]; s.x[5] = x5[c];
+    }
+};
+
+__device__ __forceinline__ float swap_add(float v) { return v + __shfl_xor(v, 1); }
+
+__device__ __forceinline__ void reduce_pair(RegAcc &a)
+{
+    a.s00 = swap_add(a.s00); a.s03 = swap_add(a.s03); a.s04 = swap_add(a.s04); a.s05 = swap_add(a.s05);
+    a.s33 = swap_add(a.s33); a.s34 = swap_add(a.s34); a.s35 = swap_add(a.s35); a.s44 = swap_add(a.s44);
+    a.s45 = swap_add(a.s45); a.s55 = swap_add(a.s55);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+        a.c01[c] = swap_add(a.c01[c]); a.c02[c] = swap_add(a.c02[c]); a.c11[c] = swap_add(a.c11[c]);
+        a.c12[c] = swap_add(a.c12[c]); a.c22[c] = swap_add(a.c22[c]); a.c13[c] = swap_add(a.c13[c]);
+        a.c14[c] = swap_add(a.c14[c]); a.c15[c] = swap_add(a.c15[c]); a.c23[c] = swap_add(a.c23[c]);
+        a.c24[c] = swap_add(a.c24[c]); a.c25[c] = swap_add(a.c25[c]);
+        a.x0[c] = swap_add(a.x0[c]); a.x1[c] = swap_add(a.x1[c]); a.x2[c] = swap_add(a.x2[c]);
+        a.x3[c] = swap_add(a.x3[c]); a.x4[c] = swap_add(a.x4[c]); a.x5[c] = swap_add(a.x5[c]);
+    }
+}
+
+// Regression weight (regressionWgt :278-287, +1e-7 at :432) with the +1e-4 folded into cvar_e.
+__device__ __forceinline__ float reg_weight(float3 dw, float cvar_e, float ivar)
+{
+    const float n2 = dw.x * dw.x + dw.y * dw.y + dw.z * dw.z;
+    return fast_exp2(-kLog2e * n2 * fast_rcp(cvar_e + ivar)) + 1e-7f;
+}
+
+constexpr int RG_CX = 64;       // centers per workgroup
+constexpr int RG_S = 4;         // sparsity of the fast path
+constexpr int RG_H = 8;         // half window of the fast path
+constexpr int RG_ROW = RG_CX * RG_S + 2 * RG_H;
+
+__global__ __launch_bounds__(256) void regression_rows(RegBatch b)
+{
+    // [buffer][plane][pixel]; planes: (y1, var1), (y2, var2), (albedo, std1), (normal, std2)
+    __shared__ float4 lds[2][4][RG_ROW];
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const RegFrame &F = b.f[L.f];
+    const int H = F.H, W = F.W, Ws = W / RG_S;
+    const int by = L.ty, bx0 = L.tx * RG_CX;
+    const int t = threadIdx.x;
+    const int half = t >> 7;           // wave-uniform: waves 0-1 half A, waves 2-3 half B
+    const int j = t & 127;
+    const int ci = j >> 1, split = j & 1;
+    const int bx = bx0 + ci;
+    const bool active = bx < Ws;
+    const int cx = (active ? bx : Ws - 1) * RG_S, cy = by * RG_S;
+    const int px0 = bx0 * RG_S - RG_H;  // pixel x of LDS column 0
+
+    const float *feat = half ? F.y2 : F.y1;
+    const float *fvar = half ? F.var2 : F.var1;
+    const size_t c = (size_t)cy * W + cx;
+    const float3 cw = ld3(feat, c), ca = ld3(F.alb, c), cn = ld3(F.nrm, c);
+    const float cvar = fvar[c];
+    const float cvar_e = cvar + 1e-4f;
+    const float cstd_e = sqrtf(cvar) + 1e-2f;
+
+    RegAcc acc;
+    acc.zero();
+
+    const int ncols = min(RG_ROW, W + 2 * RG_H - bx0 * RG_S);  // columns actually needed
+    const int own = half, oth = 1 - half;
+    // first tap column of this lane: split 0 covers dx in [-8, 0], split 1 covers [1, 8]
+    const int col0 = ci * RG_S + (split ? RG_H + 1 : 0);
+    const int ntap = split ? RG_H : RG_H + 1;
+
+    for (int r = 0; r <= 2 * RG_H; ++r)
+    {
+        const int buf = r & 1;
+        const int gy = mirror(cy + r - RG_H, H);
+        const size_t row = (size_t)gy * W;
+        for (int i = t; i < ncols; i += 256)
+        {
+            const size_t p = row + mirror(px0 + i, W);
+            const float3 a = ld3(F.y1, p), bb = ld3(F.y2, p), al = ld3(F.alb, p), nr = ld3(F.nrm, p);
+            const float v1 = F.var1[p], v2 = F.var2[p];
+            lds[buf][0][i] = make_float4(a.x, a.y, a.z, v1);
+            lds[buf][1][i] = make_float4(bb.x, bb.y, bb.z, v2);
+            lds[buf][2][i] = make_float4(al.x, al.y, al.z, sqrtf(v1));
+            lds[buf][3][i] = make_float4(nr.x, nr.y, nr.z, sqrtf(v2));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RG_H + 1; ++k)
+        {
+            if (k < ntap)
+            {
+                const int col = col0 + k;
+                const float4 P = lds[buf][own][col];
+                const float4 T = lds[buf][oth][col];
+                const float4 A = lds[buf][2][col];
+                const float4 N = lds[buf][3][col];
+                const float istd = half ? N.w : A.w;
+                const float3 dw = make_float3(P.x - cw.x, P.y - cw.y, P.z - cw.z);
+                const float w = reg_weight(dw, cvar_e, P.w);
+                const float inv = fast_rcp(cstd_e + istd);
+                acc.add(w, inv, dw, make_float3(A.x - ca.x, A.y - ca.y, A.z - ca.z),
+                        make_float3(N.x - cn.x, N.y - cn.y, N.z - cn.z), make_float3(T.x, T.y, T.z));
+            }
+        }
+    }
+    reduce_pair(acc);
+    if (!active) return;
+    // Lane pair solves the three channels: split 0 -> channels 0 and 1, split 1 -> channel 2.
+    float *out = F.beta + (((size_t)by * Ws + bx) * 2 + half) * 18;
+    {
+        Sys6 s0, s2;
+        acc.system(0, s0);
+        acc.system(2, s2);
+        Sys6 s;
+#pragma unroll
+        for (int k = 0; k < 21; ++k) s.a[k] = split ? s2.a[k] : s0.a[k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s.x[k] = split ? s2.x[k] : s0.x[k];
+        float be[6];
+        solve6(s, be);
+        const int ch = split ? 2 : 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k * 3 + ch] = be[k];
+    }
+    if (split == 0)
+    {
+        Sys6 s1;
+        acc.system(1, s1);
+        float be[6];
+        solve6(s1, be);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k * 3 + 1] = be[k];
+    }
+}
+
+// Any window / sparsity: one lane per (center, half), direct loads.
+__global__ __launch_bounds__(256) void regression_generic(RegBatch b)
+{
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const RegFrame &F = b.f[L.f];
+    const int S = b.sparsity, HW = b.half_win;
+    const int H = F.H, W = F.W, Ws = W / S, Hs = H / S;
+    const int half = threadIdx.x >> 7;
+    const int bx = L.tx * 16 + (threadIdx.x & 15);
+    const int by = L.ty * 8 + ((threadIdx.x & 127) >> 4);
+    if (bx >= Ws || by >= Hs) return;
+    const float *feat = half ? F.y2 : F.y1;
+    const float *tgt = half ? F.y1 : F.y2;
+    const float *fvar = half ? F.var2 : F.var1;
+    const int cx = bx * S, cy = by * S;
+    const size_t c = (size_t)cy * W + cx;
+    const float3 cw = ld3(feat, c), ca = ld3(F.alb, c), cn = ld3(F.nrm, c);
+    const float cvar = fvar[c], cvar_e = cvar + 1e-4f, cstd_e = sqrtf(cvar) + 1e-2f;
+    RegAcc acc;
+    acc.zero();
+    for (int iy = cy - HW; iy <= cy + HW; ++iy)
+        for (int ix = cx - HW; ix <= cx + HW; ++ix)
+        {
+            const size_t i = (size_t)mirror(iy, H) * W + mirror(ix, W);
+            const float3 iw = ld3(feat, i), ia = ld3(F.alb, i), in = ld3(F.nrm, i), it = ld3(tgt, i);
+            const float ivar = fvar[i];
+            const float3 dw = make_float3(iw.x - cw.x, iw.y - cw.y, iw.z - cw.z);
+            const float w = reg_weight(dw, cvar_e, ivar);
+            const float inv = fast_rcp(cstd_e + sqrtf(ivar));
+            acc.add(w, inv, dw, make_float3(ia.x - ca.x, ia.y - ca.y, ia.z - ca.z),
+                    make_float3(in.x - cn.x, in.y - cn.y, in.z - cn.z), it);
+        }
+    float *out = F.beta + (((size_t)by * Ws + bx) * 2 + half) * 18;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+    {
+        Sys6 s;
+        acc.system(ch, s);
+        float be[6];
+        solve6(s, be);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) out[k * 3 + ch] = be[k];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Block reconstruction
+// ---------------------------------------------------------------------------------------------
+struct RecFrame {
+    const float *beta;  // [Hs][Ws][2][18]
+    const float *y1, *y2, *var1, *var2, *alb, *nrm;
+    float *outa, *outb;  // [H/rs][W/rs][3]
+    int H, W;
+};
+struct RecBatch {
+    TileMap map;
+    RecFrame f[XDN_MAX_FRAMES];
+    int half_win, sparsity, recon_scale;  // generic kernel only
+};
+
+constexpr int RC_T = 16;                 // pixels per tile side
+constexpr int RC_C = 8;                  // centers per tile side (fast path)
+constexpr int RC_STRIDE = 13;            // float4 per staged center
+
+// Per-center contribution for one half: pred = beta . delta with delta = pixel - center.
+__device__ __forceinline__ void recon_term(const float4 *cb, float3 dwv, float inv, float3 d2, float3 n,
+                                           float w, float3 &acc)
+{
+    // cb: 18 floats of beta [k][ch] as 4.5 float4 starting at a 16-B boundary or at +8 B; we
+    // read through a float pointer to keep it simple.
+    const float *bt = reinterpret_cast<const float *>(cb);
+    const float d1x = dwv.x * inv, d1y = dwv.y * inv, d1z = dwv.z * inv;
+    const float px = bt[0] + bt[3] * d1x + bt[6] * d2.x + bt[9] * n.x + bt[12] * n.y + bt[15] * n.z;
+    const float py = bt[1] + bt[4] * d1y + bt[7] * d2.y + bt[10] * n.x + bt[13] * n.y + bt[16] * n.z;
+    const float pz = bt[2] + bt[5] * d1z + bt[8] * d2.z + bt[11] * n.x + bt[14] * n.y + bt[17] * n.z;
+    acc.x += w * px;
+    acc.y += w * py;
+    acc.z += w * pz;
+}
+
+__global__ __launch_bounds__(256) void recon_tile(RecBatch b)
+{
+    // staged center record (13 float4): beta A (18) + beta B (18) = 9 float4,
+    // (y1, var1), (y2, var2), (alb, std1), (nrm, std2)
+    __shared__ float4 cs[RC_C * RC_C][RC_STRIDE];
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const RecFrame &F = b.f[L.f];
+    const int H = F.H, W = F.W, Ws = W / RG_S;
+    const int x0 = L.tx * RC_T, y0 = L.ty * RC_T;
+    // tile center window [clo, clo+RC_C) x [rlo, rlo+RC_C), a superset of every pixel's range
+    const int clo = max(0, (x0 - RG_H + RG_S - 1 + 4 * RG_S) / RG_S - 4);
+    const int rlo = max(0, (y0 - RG_H + RG_S - 1 + 4 * RG_S) / RG_S - 4);
+    const int chi = min((W - 1) / RG_S, (x0 + RC_T - 1 + RG_H) / RG_S);
+    const int rhi = min((H - 1) / RG_S, (y0 + RC_T - 1 + RG_H) / RG_S);
+    const int ncx = chi - clo + 1, ncy = rhi - rlo + 1;
+    for (int i = threadIdx.x; i < ncx * ncy * 9; i += 256)
+    {
+        const int cidx = i / 9, q = i - cidx * 9;
+        const int ry = cidx / ncx, rx = cidx - ry * ncx;
+        const float4 *src = reinterpret_cast<const float4 *>(F.beta + ((size_t)(rlo + ry) * Ws + (clo + rx)) * 36);
+        cs[ry * RC_C + rx][q] = src[q];
+    }
+    for (int i = threadIdx.x; i < ncx * ncy; i += 256)
+    {
+        const int ry = i / ncx, rx = i - ry * ncx;
+        const size_t p = (size_t)((rlo + ry) * RG_S) * W + (size_t)((clo + rx) * RG_S);
+        const float3 a = ld3(F.y1, p), bb = ld3(F.y2, p), al = ld3(F.alb, p), nr = ld3(F.nrm, p);
+        const float v1 = F.var1[p], v2 = F.var2[p];
+        float4 *d = cs[ry * RC_C + rx];
+        d[9] = make_float4(a.x, a.y, a.z, v1);
+        d[10] = make_float4(bb.x, bb.y, bb.z, v2);
+        d[11] = make_float4(al.x, al.y, al.z, sqrtf(v1));
+        d[12] = make_float4(nr.x, nr.y, nr.z, sqrtf(v2));
+    }
+    __syncthreads();
+    const int cx = x0 + (threadIdx.x & 15), cy = y0 + (threadIdx.x >> 4);
+    if (cx >= W || cy >= H) return;
+    const size_t c = (size_t)cy * W + cx;
+    const float3 c1 = ld3(F.y1, c), c2 = ld3(F.y2, c), ca = ld3(F.alb, c), cn = ld3(F.nrm, c);
+    const float v1 = F.var1[c], v2 = F.var2[c];
+    const float v1e = v1 + 1e-4f, v2e = v2 + 1e-4f;
+    const float s1e = sqrtf(v1) + 1e-2f, s2e = sqrtf(v2) + 1e-2f;
+    const int sx = max(0, idivup(cx - RG_H, RG_S)), ex = min((W - 1) / RG_S, (cx + RG_H) / RG_S);
+    const int sy = max(0, idivup(cy - RG_H, RG_S)), ey = min((H - 1) / RG_S, (cy + RG_H) / RG_S);
+    float3 accA = make_float3(0.f, 0.f, 0.f), accB = accA;
+    float sumA = 0.f, sumB = 0.f;
+    for (int iy = sy; iy <= ey; ++iy)
+        for (int ix = sx; ix <= ex; ++ix)
+        {
+            const float4 *rec = cs[(iy - rlo) * RC_C + (ix - clo)];
+            const float4 P1 = rec[9], P2 = rec[10], A = rec[11], N = rec[12];
+            const float3 d2 = make_float3(ca.x - A.x, ca.y - A.y, ca.z - A.z);
+            const float3 n = make_float3(cn.x - N.x, cn.y - N.y, cn.z - N.z);
+            const float3 dwA = make_float3(c1.x - P1.x, c1.y - P1.y, c1.z - P1.z);
+            const float3 dwB = make_float3(c2.x - P2.x, c2.y - P2.y, c2.z - P2.z);
+            const float wA = reg_weight(dwA, v1e, P1.w), wB = reg_weight(dwB, v2e, P2.w);
+            recon_term(rec, dwA, fast_rcp(s1e + A.w), d2, n, wA, accA);
+            recon_term(reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(rec) + 18), dwB,
+                       fast_rcp(s2e + N.w), d2, n, wB, accB);
+            sumA += wA;
+            sumB += wB;
+        }
+    const float iA = 1.f / fmaxf(sumA, 1e-7f), iB = 1.f / fmaxf(sumB, 1e-7f);
+    st3(F.outa, c, make_float3(fmaxf(0.f, accA.x * iA), fmaxf(0.f, accA.y * iA), fmaxf(0.f, accA.z * iA)));
+    st3(F.outb, c, make_float3(fmaxf(0.f, accB.x * iB), fmaxf(0.f, accB.y * iB), fmaxf(0.f, accB.z * iB)));
+}
+
+__global__ __launch_bounds__(256) void recon_generic(RecBatch b)
+{
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const RecFrame &F = b.f[L.f];
+    const int S = b.sparsity, HW = b.half_win, RS = b.recon_scale;
+    const int H = F.H, W = F.W, Ws = W / S, Hr = H / RS, Wr = W / RS;
+    const int rx = L.tx * 16 + (threadIdx.x & 15), ry = L.ty * 16 + (threadIdx.x >> 4);
+    if (rx >= Wr || ry >= Hr) return;
+    const int cx = rx * RS, cy = ry * RS;
+    const size_t c = (size_t)cy * W + cx;
+    const float3 c1 = ld3(F.y1, c), c2 = ld3(F.y2, c), ca = ld3(F.alb, c), cn = ld3(F.nrm, c);
+    const float v1 = F.var1[c], v2 = F.var2[c];
+    const float v1e = v1 + 1e-4f, v2e = v2 + 1e-4f;
+    const float s1e = sqrtf(v1) + 1e-2f, s2e = sqrtf(v2) + 1e-2f;
+    const int sx = max(0, idivup(cx - HW, S)), ex = min((W - 1) / S, (cx + HW) / S);
+    const int sy = max(0, idivup(cy - HW, S)), ey = min((H - 1) / S, (cy + HW) / S);
+    float3 accA = make_float3(0.f, 0.f, 0.f), accB = accA;
+    float sumA = 0.f, sumB = 0.f;
+    for (int iy = sy; iy <= ey; ++iy)
+        for (int ix = sx; ix <= ex; ++ix)
+        {
+            const float *bt = F.beta + ((size_t)iy * Ws + ix) * 36;
+            const size_t i = (size_t)(iy * S) * W + (size_t)(ix * S);
+            const float3 p1 = ld3(F.y1, i), p2 = ld3(F.y2, i), ia = ld3(F.alb, i), in = ld3(F.nrm, i);
+            const float iv1 = F.var1[i], iv2 = F.var2[i];
+            const float3 d2 = make_float3(ca.x - ia.x, ca.y - ia.y, ca.z - ia.z);
+            const float3 n = make_float3(cn.x - in.x, cn.y - in.y, cn.z - in.z);
+            const float3 dwA = make_float3(c1.x - p1.x, c1.y - p1.y, c1.z - p1.z);
+            const float3 dwB = make_float3(c2.x - p2.x, c2.y - p2.y, c2.z - p2.z);
+            const float wA = reg_weight(dwA, v1e, iv1), wB = reg_weight(dwB, v2e, iv2);
+            recon_term(reinterpret_cast<const float4 *>(bt), dwA, fast_rcp(s1e + sqrtf(iv1)), d2, n, wA, accA);
+            recon_term(reinterpret_cast<const float4 *>(bt + 18), dwB, fast_rcp(s2e + sqrtf(iv2)), d2, n, wB, accB);
+            sumA += wA;
+            sumB += wB;
+        }
+    const size_t o = (size_t)ry * Wr + rx;
+    const float iA = 1.f / fmaxf(sumA, 1e-7f), iB = 1.f / fmaxf(sumB, 1e-7f);
+    st3(F.outa, o, make_float3(fmaxf(0.f, accA.x * iA), fmaxf(0.f, accA.y * iA), fmaxf(0.f, accA.z * iA)));
+    st3(F.outb, o, make_float3(fmaxf(0.f, accB.x * iB), fmaxf(0.f, accB.y * iB), fmaxf(0.f, accB.z * iB)));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------------------------
+static int div_up(int a, int b) { return (a + b - 1) / b; }
+
+hipError_t launch_variance(int n, const CrFrame *fr, int which, bool log_in, hipStream_t st)
+{
+    // which: bit0 -> half A (y1/var1), bit1 -> half B (y2/var2); both halves share one launch
+    VarBatch b{};
+    int tx[XDN_MAX_FRAMES], ty[XDN_MAX_FRAMES];
+    int m = 0;
+    for (int i = 0; i < n; ++i)
+        for (int h = 0; h < 2; ++h)
+        {
+            if (!(which & (1 << h))) continue;
+            VarFrame &v = b.f[m];
+            v.in = log_in ? (h ? fr[i].color2 : fr[i].color) : (h ? fr[i].y2 : fr[i].y1);
+            v.y_out = h ? fr[i].y2 : fr[i].y1;
+            v.var = h ? fr[i].var2 : fr[i].var1;
+            v.H = fr[i].H;
+            v.W = fr[i].W;
+            tx[m] = div_up(v.W, VT_X);
+            ty[m] = div_up(v.H, VT_Y);
+            ++m;
+        }
+    b.map = make_tilemap(m, tx, ty);
+    b.log_in = log_in ? 1 : 0;
+    if (b.map.total == 0) return hipSuccess;
+    hipLaunchKernelGGL(variance_kernel, dim3(b.map.total), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_regression(int n, const CrFrame *fr, int win, int s, hipStream_t st)
+{
+    RegBatch b{};
+    int tx[XDN_MAX_FRAMES], ty[XDN_MAX_FRAMES];
+    const bool fast = (win / 2 == RG_H && s == RG_S);
+    for (int i = 0; i < n; ++i)
+    {
+        RegFrame &r = b.f[i];
+        r.y1 = fr[i].y1; r.y2 = fr[i].y2; r.var1 = fr[i].var1; r.var2 = fr[i].var2;
+        r.alb = fr[i].alb; r.nrm = fr[i].nrm; r.beta = fr[i].beta; r.H = fr[i].H; r.W = fr[i].W;
+        const int Ws = r.W / s, Hs = r.H / s;
+        tx[i] = fast ? div_up(Ws, RG_CX) : div_up(Ws, 16);
+        ty[i] = fast ? Hs : div_up(Hs, 8);
+    }
+    b.map = make_tilemap(n, tx, ty);
+    b.half_win = win / 2;
+    b.sparsity = s;
+    if (b.map.total == 0) return hipSuccess;
+    if (fast)
+        hipLaunchKernelGGL(regression_rows, dim3(b.map.total), dim3(256), 0, st, b);
+    else
+        hipLaunchKernelGGL(regression_generic, dim3(b.map.total), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_recon(int n, const CrFrame *fr, int win, int s, int rs, hipStream_t st)
+{
+    RecBatch b{};
+    int tx[XDN_MAX_FRAMES], ty[XDN_MAX_FRAMES];
+    const bool fast = (win / 2 == RG_H && s == RG_S && rs == 1);
+    for (int i = 0; i < n; ++i)
+    {
+        RecFrame &r = b.f[i];
+        r.beta = fr[i].beta; r.y1 = fr[i].y1; r.y2 = fr[i].y2; r.var1 = fr[i].var1; r.var2 = fr[i].var2;
+        r.alb = fr[i].alb; r.nrm = fr[i].nrm; r.outa = fr[i].outa; r.outb = fr[i].outb;
+        r.H = fr[i].H; r.W = fr[i].W;
+        tx[i] = div_up(r.W / rs, 16);
+        ty[i] = div_up(r.H / rs, 16);
+    }
+    b.map = make_tilemap(n, tx, ty);
+    b.half_win = win / 2;
+    b.sparsity = s;
+    b.recon_scale = rs;
+    if (b.map.total == 0) return hipSuccess;
+    if (fast)
+        hipLaunchKernelGGL(recon_tile, dim3(b.map.total), dim3(256), 0, st, b);
+    else
+        hipLaunchKernelGGL(recon_generic, dim3(b.map.total), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+}  // namespace xdn

@@ -1,0 +1,240 @@
+// Temporal screening on gfx950 (TemporalScreening, temporal.cu.cc:57-198) with the 3x3 erosion of
+// the reprojection success mask (scripts/main.py:270-276) optionally fused in.
+//
+// A pixel keeps its reprojected history for a set X in {avg, 1, 2} iff for every channel
+// min_prev < max_curr and min_curr < max_prev (strict), the current extrema taken over the
+// win_curr x win_curr window with out-of-image taps skipped and the previous ones over win_prev
+// (only where the eroded success is >= 1). Outputs are selections of the inputs, so the result is
+// exact given identical inputs; the set average is computed as (c1 + c2) * 0.5f in FP32.
+//
+// screening_tile (win_curr 7, win_prev 1): 32x8 pixels per workgroup; the tile plus a 3-px halo of
+// (c1, c2, avg) goes to LDS, then a separable min/max (7-tap row pass into LDS, 7-tap column pass)
+// gives the 18 extrema per pixel with 14 instead of 49 taps. HBM-bound.
+// screening_generic: any window sizes, direct loads.
+#include "xdn_common.h"
+#include "xdn_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace xdn {
+
+struct TsBatch {
+    TileMap map;
+    TsFrame f[XDN_MAX_FRAMES];
+    int win_curr, win_prev, fuse;
+};
+static_assert(sizeof(TsBatch) <= 4096, "kernel argument too large");
+
+constexpr float SCR_BIG = 10000000.f;  // FLT_MAX as redefined by the reference (temporal.cu.cc:29)
+constexpr int TS_TX = 32, TS_TY = 8, TS_R = 3;
+
+// Eroded success at (x, y): min over the in-bounds 3x3 neighbourhood of the u8 mask.
+__device__ __forceinline__ float eroded(const uint8_t *m, int x, int y, int W, int H)
+{
+    float r = 1.f;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+        {
+            const int gx = x + dx, gy = y + dy;
+            if (gx < 0 || gy < 0 || gx >= W || gy >= H) continue;
+            r = fminf(r, m[(size_t)gy * W + gx] ? 1.f : 0.f);
+        }
+    return r;
+}
+
+struct MinMax9 {
+    float mn[9], mx[9];  // sets avg, 1, 2 x channels
+};
+
+__device__ __forceinline__ void emit(const TsFrame &F, size_t c, float sc, const MinMax9 *cur, bool center_ok)
+{
+    const float3 a = ld3(F.c1, c), b = ld3(F.c2, c);
+    const float3 m = make_float3((a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f);
+    if (sc + 1e-4f < 1.f)
+    {
+        st3(F.o, c, m);
+        st3(F.o1, c, a);
+        st3(F.o2, c, b);
+        F.osucc[c] = 0.f;
+        return;
+    }
+    const float3 p = ld3(F.prev, c), p1 = ld3(F.prev1, c), p2 = ld3(F.prev2, c);
+    const float pv[9] = {p.x, p.y, p.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z};
+    bool keep[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+    {
+        bool k = true;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            // 1x1 previous window: its extrema are the center value when it is usable, else the
+            // initial (+BIG, -BIG) of the reference
+            const float pmn = center_ok ? pv[3 * s + ch] : SCR_BIG;
+            const float pmx = center_ok ? pv[3 * s + ch] : -SCR_BIG;
+            k = k && (pmn < cur->mx[3 * s + ch]) && (cur->mn[3 * s + ch] < pmx);
+        }
+        keep[s] = k;
+    }
+    st3(F.o, c, keep[0] ? p : m);
+    st3(F.o1, c, keep[1] ? p1 : a);
+    st3(F.o2, c, keep[2] ? p2 : b);
+    F.osucc[c] = keep[0] ? 1.f : 0.f;
+}
+
+__global__ __launch_bounds__(256) void screening_tile(TsBatch b)
+{
+    constexpr int EX = TS_TX + 2 * TS_R, EY = TS_TY + 2 * TS_R;
+    __shared__ float v[9][EY][EX];          // avg.xyz, c1.xyz, c2.xyz
+    __shared__ float hmn[9][EY][TS_TX];     // row-pass minima
+    __shared__ float hmx[9][EY][TS_TX];     // row-pass maxima
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const TsFrame &F = b.f[L.f];
+    const int H = F.H, W = F.W;
+    const int x0 = L.tx * TS_TX, y0 = L.ty * TS_TY;
+    for (int i = threadIdx.x; i < EX * EY; i += 256)
+    {
+        const int ly = i / EX, lx = i - ly * EX;
+        const int gx = min(max(x0 + lx - TS_R, 0), W - 1), gy = min(max(y0 + ly - TS_R, 0), H - 1);
+        const size_t p = (size_t)gy * W + gx;
+        const float3 a = ld3(F.c1, p), bb = ld3(F.c2, p);
+        v[0][ly][lx] = (a.x + bb.x) * 0.5f; v[1][ly][lx] = (a.y + bb.y) * 0.5f; v[2][ly][lx] = (a.z + bb.z) * 0.5f;
+        v[3][ly][lx] = a.x; v[4][ly][lx] = a.y; v[5][ly][lx] = a.z;
+        v[6][ly][lx] = bb.x; v[7][ly][lx] = bb.y; v[8][ly][lx] = bb.z;
+    }
+    __syncthreads();
+    // row pass over the in-bounds columns of each window row
+    for (int i = threadIdx.x; i < EY * TS_TX; i += 256)
+    {
+        const int ly = i / TS_TX, lx = i - ly * TS_TX;
+        const int gx = x0 + lx;
+        const int lo = max(gx - TS_R, 0) - (x0 - TS_R), hi = min(gx + TS_R, W - 1) - (x0 - TS_R);
+        float mn[9], mx[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) mn[k] = SCR_BIG, mx[k] = -SCR_BIG;
+        for (int xx = lo; xx <= hi; ++xx)
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+            {
+                const float t = v[k][ly][xx];
+                mn[k] = fminf(mn[k], t);
+                mx[k] = fmaxf(mx[k], t);
+            }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) hmn[k][ly][lx] = mn[k], hmx[k][ly][lx] = mx[k];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % TS_TX, ly = threadIdx.x / TS_TX;
+    const int cx = x0 + lx, cy = y0 + ly;
+    if (cx >= W || cy >= H) return;
+    const int lo = max(cy - TS_R, 0) - (y0 - TS_R), hi = min(cy + TS_R, H - 1) - (y0 - TS_R);
+    MinMax9 cur;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) cur.mn[k] = SCR_BIG, cur.mx[k] = -SCR_BIG;
+    for (int yy = lo; yy <= hi; ++yy)
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+        {
+            cur.mn[k] = fminf(cur.mn[k], hmn[k][yy][lx]);
+            cur.mx[k] = fmaxf(cur.mx[k], hmx[k][yy][lx]);
+        }
+    const size_t c = (size_t)cy * W + cx;
+    const float sc = b.fuse ? eroded(F.succ_mask, cx, cy, W, H) : F.rsucc[c];
+    if (b.fuse && F.succ_e) F.succ_e[c] = sc;
+    // win_prev == 1: the previous extrema are the center values, taken iff sc >= 1
+    emit(F, c, sc, &cur, sc >= 1.f);
+}
+
+__global__ __launch_bounds__(256) void screening_generic(TsBatch b)
+{
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const TsFrame &F = b.f[L.f];
+    const int H = F.H, W = F.W;
+    const int cx = L.tx * TS_TX + (threadIdx.x % TS_TX), cy = L.ty * TS_TY + threadIdx.x / TS_TX;
+    if (cx >= W || cy >= H) return;
+    const size_t c = (size_t)cy * W + cx;
+    const float sc = b.fuse ? eroded(F.succ_mask, cx, cy, W, H) : F.rsucc[c];
+    if (b.fuse && F.succ_e) F.succ_e[c] = sc;
+    const int hc = b.win_curr / 2, hp = b.win_prev / 2;
+    MinMax9 cur, prv;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) cur.mn[k] = prv.mn[k] = SCR_BIG, cur.mx[k] = prv.mx[k] = -SCR_BIG;
+    if (!(sc + 1e-4f < 1.f))
+    {
+        for (int iy = cy - hc; iy <= cy + hc; ++iy)
+            for (int ix = cx - hc; ix <= cx + hc; ++ix)
+            {
+                if (ix < 0 || ix >= W || iy < 0 || iy >= H) continue;
+                const size_t i = (size_t)iy * W + ix;
+                const float3 a = ld3(F.c1, i), bb = ld3(F.c2, i);
+                const float t[9] = {(a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f, (a.z + bb.z) * 0.5f, a.x, a.y, a.z,
+                                    bb.x, bb.y, bb.z};
+#pragma unroll
+                for (int k = 0; k < 9; ++k) cur.mn[k] = fminf(cur.mn[k], t[k]), cur.mx[k] = fmaxf(cur.mx[k], t[k]);
+            }
+        for (int iy = cy - hp; iy <= cy + hp; ++iy)
+            for (int ix = cx - hp; ix <= cx + hp; ++ix)
+            {
+                if (ix < 0 || ix >= W || iy < 0 || iy >= H) continue;
+                const size_t i = (size_t)iy * W + ix;
+                const float si = b.fuse ? eroded(F.succ_mask, ix, iy, W, H) : F.rsucc[i];
+                if (si < 1.f) continue;
+                const float3 p = ld3(F.prev, i), p1 = ld3(F.prev1, i), p2 = ld3(F.prev2, i);
+                const float t[9] = {p.x, p.y, p.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z};
+#pragma unroll
+                for (int k = 0; k < 9; ++k) prv.mn[k] = fminf(prv.mn[k], t[k]), prv.mx[k] = fmaxf(prv.mx[k], t[k]);
+            }
+    }
+    const float3 a = ld3(F.c1, c), bb = ld3(F.c2, c);
+    const float3 m = make_float3((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f, (a.z + bb.z) * 0.5f);
+    if (sc + 1e-4f < 1.f)
+    {
+        st3(F.o, c, m);
+        st3(F.o1, c, a);
+        st3(F.o2, c, bb);
+        F.osucc[c] = 0.f;
+        return;
+    }
+    bool keep[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+    {
+        bool k = true;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+            k = k && (prv.mn[3 * s + ch] < cur.mx[3 * s + ch]) && (cur.mn[3 * s + ch] < prv.mx[3 * s + ch]);
+        keep[s] = k;
+    }
+    st3(F.o, c, keep[0] ? ld3(F.prev, c) : m);
+    st3(F.o1, c, keep[1] ? ld3(F.prev1, c) : a);
+    st3(F.o2, c, keep[2] ? ld3(F.prev2, c) : bb);
+    F.osucc[c] = keep[0] ? 1.f : 0.f;
+}
+
+static int div_up(int a, int b) { return (a + b - 1) / b; }
+
+hipError_t launch_screening(int n, const TsFrame *fr, int win_curr, int win_prev, bool fuse_erosion, hipStream_t st)
+{
+    TsBatch b{};
+    int tx[XDN_MAX_FRAMES], ty[XDN_MAX_FRAMES];
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        tx[i] = div_up(fr[i].W, TS_TX);
+        ty[i] = div_up(fr[i].H, TS_TY);
+    }
+    b.map = make_tilemap(n, tx, ty);
+    b.win_curr = win_curr;
+    b.win_prev = win_prev;
+    b.fuse = fuse_erosion ? 1 : 0;
+    if (b.map.total == 0) return hipSuccess;
+    if (win_curr / 2 == TS_R && win_prev / 2 == 0)
+        hipLaunchKernelGGL(screening_tile, dim3(b.map.total), dim3(256), 0, st, b);
+    else
+        hipLaunchKernelGGL(screening_generic, dim3(b.map.total), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+}  // namespace xdn

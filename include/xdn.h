@@ -1,0 +1,152 @@
+/*
+ * xdn.h -- C ABI of libxdn.so, the gfx950 (MI355X) implementation of cross-denoiser's per-frame
+ * filtering path.
+ *
+ * Conventions
+ *   - Every image is a contiguous NHWC float32 device buffer [B, H, W, C] (C stated per argument),
+ *     the layout of the reference ops (scripts/ops.py). B images are processed independently.
+ *   - `stream` is a hipStream_t (NULL = default stream). All work is enqueued on it; no entry
+ *     point synchronizes, allocates device memory or keeps device state between calls, so every
+ *     call can be captured in a hipGraph.
+ *   - Return value: 0 on success, XDN_EINVAL for bad arguments (nothing launched), XDN_EHIP for a
+ *     HIP launch error. xdn_last_error() gives the message of the last failure on the calling
+ *     thread; argument messages repeat the reference's text where it has one.
+ *   - Pointers may not alias unless stated.
+ */
+#ifndef XDN_H
+#define XDN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XDN_OK 0
+#define XDN_EINVAL (-1)
+#define XDN_EHIP (-2)
+
+/* Library version string. */
+const char *xdn_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *xdn_last_error(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * CrossRegression op: replaces REGISTER_OP("CrossRegression") / CrossRegressionOp::Compute
+ * (ops/regression.cc:25-57, 125-196) and CrossRegressionFunc (ops/regression.cu.cc:477-500).
+ *   imga, imgb, albedo, normal  [B,H,W,3]
+ *   outa, outb                  [B,H/recon_scale,W/recon_scale,3]
+ *   out_vara, out_varb          [B,H,W,1]
+ *   workspace                   >= xdn_cross_regression_workspace_size(B,H,W,sparsity) bytes
+ *                               (the regression coefficients; replaces the static betaA/betaB
+ *                               temporaries of regression.cc:174-180)
+ * Errors (InvalidArgument in the reference): recon_scale > sparsity (regression.cc:134-135),
+ * H or W not divisible by recon_scale (:159-160); additionally H, W divisible by sparsity and
+ * H, W > win_size/2 (the reference reads unwritten or out-of-bounds memory otherwise).
+ * ------------------------------------------------------------------------------------------- */
+size_t xdn_cross_regression_workspace_size(int batch, int height, int width, int sparsity);
+int xdn_cross_regression(const float *imga, const float *imgb, const float *albedo, const float *normal,
+                         float *outa, float *outb, float *out_vara, float *out_varb, void *workspace,
+                         size_t workspace_bytes, int batch, int height, int width, int win_size, int sparsity,
+                         int recon_scale, void *stream);
+
+/* Same op with prepare_input (scripts/main.py:207-212) fused in: takes the raw colours, writes
+ * y1 = log1p(color), y2 = log1p(color2) [B,H,W,3] and then runs the op on them. */
+int xdn_cross_regression_from_color(const float *color, const float *color2, const float *albedo,
+                                    const float *normal, float *y1, float *y2, float *outa, float *outb,
+                                    float *out_vara, float *out_varb, void *workspace, size_t workspace_bytes,
+                                    int batch, int height, int width, int win_size, int sparsity, void *stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * SpatialFilter op: replaces REGISTER_OP("SpatialFilter") / SpatialFilterOp::Compute
+ * (ops/regression.cc:59-86, 235-282) and SpatialFilterFunc (ops/regression.cu.cc:502-517).
+ *   imga, imgb, albedo, normal  [B,H,W,3];  band [B,H,W,5]
+ *   out_acca, out_accb          [B,H,W,3];  out_wgtsuma, out_wgtsumb [B,H,W,1]
+ * No per-tap state is stored (the reference writes 2x[B,H,W,16] gradBand temporaries here).
+ * ------------------------------------------------------------------------------------------- */
+int xdn_spatial_filter(const float *imga, const float *imgb, const float *albedo, const float *normal,
+                       const float *band, float *out_acca, float *out_accb, float *out_wgtsuma,
+                       float *out_wgtsumb, int batch, int height, int width, int win_size, void *stream);
+
+/* SpatialFilterGrad op: replaces REGISTER_OP("SpatialFilterGrad") / SpatialFilterGradOp::Compute
+ * (ops/regression.cc:88-103, 283-323) and SpatialFilterGradFunc (ops/regression.cu.cc:519-534).
+ * grad_band [B,H,W,5] = d L / d band, recomputing the window weights instead of reading moments
+ * stored by the forward pass. */
+int xdn_spatial_filter_grad(const float *grad_acca, const float *grad_accb, const float *grad_wgtsuma,
+                            const float *grad_wgtsumb, const float *imga, const float *imgb, const float *img_albedo,
+                            const float *img_normal, const float *band, float *out_grad_band, int batch, int height,
+                            int width, int win_size, void *stream);
+
+/* Gradient of SpatialFilter w.r.t. its image inputs, which the reference's registered gradient
+ * leaves as None (scripts/ops.py:23-32). Any of the four outputs [B,H,W,3] may be NULL. */
+int xdn_spatial_filter_grad_inputs(const float *grad_acca, const float *grad_accb, const float *grad_wgtsuma,
+                                   const float *grad_wgtsumb, const float *imga, const float *imgb,
+                                   const float *img_albedo, const float *img_normal, const float *band,
+                                   float *grad_imga, float *grad_imgb, float *grad_albedo, float *grad_normal,
+                                   int batch, int height, int width, int win_size, void *stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Reproject op: replaces REGISTER_OP("Reproject") / ReprojectOp::Compute (ops/reproject.cc:34-69,
+ * 87-178) and ReprojectFunc (ops/reproject.cu.cc:240-273).
+ *   mvec, linear_z, prev_linear_z, normal, prev_normal, pn_fwidth [B,H,W,3]
+ *   opacity, prev_opacity [B,H,W,1];  success [B,H,W,1] bool (1 byte)
+ *   inputs[l] / outputs[l] [B,H,W,dims[l]] for l < n_list, 1 <= dims[l] <= 4; the pointer
+ *   arrays are host memory read during the call (no device pointer table, no host sync).
+ * ------------------------------------------------------------------------------------------- */
+int xdn_reproject(const float *mvec, int n_list, const float *const *inputs, const int *dims,
+                  const float *linear_z, const float *prev_linear_z, const float *normal, const float *prev_normal,
+                  const float *pn_fwidth, const float *opacity, const float *prev_opacity, uint8_t *success,
+                  float *const *outputs, int batch, int height, int width, void *stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * TemporalScreening op: replaces REGISTER_OP("TemporalScreening") / TemporalScreeningOp::Compute
+ * (ops/temporal.cu.cc:204-285).
+ *   current1, current2, prev, prev1, prev2 [B,H,W,3]; reproj_success [B,H,W,1] float
+ *   output, output1, output2 [B,H,W,3]; success [B,H,W,1] float
+ * ------------------------------------------------------------------------------------------- */
+int xdn_temporal_screening(const float *current1, const float *current2, const float *prev, const float *prev1,
+                           const float *prev2, const float *reproj_success, float *output, float *output1,
+                           float *output2, float *success, int batch, int height, int width, int win_curr,
+                           int win_prev, void *stream);
+
+/* The same op preceded by the 3x3 erosion of the boolean reprojection mask
+ * (scripts/main.py:270-276, tf.nn.erosion2d + 1): takes `success_mask` [B,H,W,1] u8 and also
+ * writes the eroded mask to `eroded` [B,H,W,1] float (may be NULL). */
+int xdn_erode_temporal_screening(const float *current1, const float *current2, const float *prev,
+                                 const float *prev1, const float *prev2, const uint8_t *success_mask, float *eroded,
+                                 float *output, float *output1, float *output2, float *success, int batch,
+                                 int height, int width, int win_curr, int win_prev, void *stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Fused spatiotemporal filtering step of train_step (scripts/main.py:291-302): the SpatialFilter
+ * forward with band = 1/(x^2+1e-4), alpha = sigmoid(x) of the raw network output `param`
+ * [B,H,W,6] (main.py:110-113), aggregation_loss (main.py:214-248) and its gradient back to
+ * `param`. Replaces SpatialFilter + aggregation_loss + SpatialFilterGrad + the band/alpha
+ * head's backward. Labels follow main.py:283-286 (label1 = ols2, label2 = ols1,
+ * prev_label1 = prev_ols2, prev_label2 = prev_ols1).
+ *   ols1, ols2, albedo, normal, prev_out, prev_ols1, prev_ols2 [B,H,W,3]
+ *   success (eroded), success_prev, opacity [B,H,W,1]
+ *   out_log, out [B,H,W,3]; dparam [B,H,W,6] = d mean(loss of the image) / d param;
+ *   loss [B,H,W,3] (may be NULL); avg_loss [B] (mean loss per image)
+ *   workspace >= xdn_filter_train_workspace_size(B,H,W) bytes
+ * ------------------------------------------------------------------------------------------- */
+size_t xdn_filter_train_workspace_size(int batch, int height, int width);
+int xdn_filter_train(const float *ols1, const float *ols2, const float *albedo, const float *normal,
+                     const float *param, const float *success, const float *success_prev, const float *prev_out,
+                     const float *opacity, const float *prev_ols1, const float *prev_ols2, float *out_log, float *out,
+                     float *dparam, float *loss, float *avg_loss, void *workspace, size_t workspace_bytes, int batch,
+                     int height, int width, int win_size, void *stream);
+
+/* Keras Adam step (tf.keras.optimizers.Adam, scripts/main.py:81,303) on a flat parameter buffer:
+ * m += (g-m)(1-b1); v += (g^2-v)(1-b2); p -= alpha m/(sqrt(v)+eps) with
+ * alpha = lr sqrt(1-b2^step)/(1-b1^step), step counted from 1. */
+int xdn_adam(float *param, const float *grad, float *m, float *v, int count, int step, float lr, float beta1,
+             float beta2, float eps, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XDN_H */

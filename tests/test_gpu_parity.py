@@ -1,0 +1,294 @@
+"""GPU parity of every op against the FP64 oracle, through the C ABI (libxdn.so).
+
+Tolerances (stated per output, relative L2 = ||gpu - oracle|| / ||oracle||):
+  filtered radiance and sums (spatial filter, fused step)       <= 1e-4 (the stated bar); observed ~1e-6
+  cross-regression outputs (FP32 normal equations + Cholesky)   <= 1e-4
+  gradients (FP32 accumulation of 121-tap sums)                 <= 1e-4
+  reprojection success mask and screening selections            bit-exact
+  reprojected values (FP32 bilinear vs FP64)                    <= 1e-6
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import log_images, rand_img, rel_l2, to_dev, to_np, unit_normals
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ops(dev):
+    from xdn import ops as xops
+    return xops
+
+
+def _report(name, err):
+    print(f"  {name}: rel L2 vs FP64 oracle = {err:.3e}", flush=True)
+
+
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("B,H,W", [(1, 64, 64), (1, 48, 40), (2, 36, 52), (1, 256, 256), (1, 24, 300)])
+def test_cross_regression(ops, dev, B, H, W):
+    ya, yb = log_images(H, W, seed=H * W, B=B)
+    alb, nrm = rand_img(H, W, seed=3, B=B), unit_normals(H, W, seed=4, B=B)
+    ref = O.cross_regression(ya, yb, alb, nrm, 17, 4, 1)
+    got = ops.cross_regression(to_dev(ya, dev), to_dev(yb, dev), to_dev(alb, dev), to_dev(nrm, dev), win_size=17,
+                               sparsity=4)
+    for name, g, r in zip(("outa", "outb", "out_vara", "out_varb"), got, ref):
+        err = rel_l2(to_np(g), r)
+        _report(f"cross_regression {name} {B}x{H}x{W}", err)
+        assert err <= TOL, name
+    assert got[0].shape == (B, H, W, 3) and got[2].shape == (B, H, W, 1)
+
+
+@pytest.mark.parametrize("win,s,rs", [(13, 2, 1), (17, 4, 2), (9, 4, 4)])
+def test_cross_regression_generic_attrs(ops, dev, win, s, rs):
+    H, W = 32, 40
+    ya, yb = log_images(H, W, seed=7)
+    alb, nrm = rand_img(H, W, seed=5), unit_normals(H, W, seed=6)
+    ref = O.cross_regression(ya, yb, alb, nrm, win, s, rs)
+    got = ops.cross_regression(to_dev(ya, dev), to_dev(yb, dev), to_dev(alb, dev), to_dev(nrm, dev), win_size=win,
+                               sparsity=s, recon_scale=rs)
+    for name, g, r in zip(("outa", "outb", "out_vara", "out_varb"), got, ref):
+        err = rel_l2(to_np(g), r)
+        _report(f"cross_regression {name} win={win} s={s} rs={rs}", err)
+        assert err <= TOL, name
+
+
+def test_cross_regression_errors(ops, dev):
+    x = torch.zeros(1, 32, 32, 3, device=dev)
+    with pytest.raises(ValueError, match="recon_scale must be less than or equal to sparsity"):
+        ops.cross_regression(x, x, x, x, win_size=17, sparsity=2, recon_scale=4)
+    y = torch.zeros(1, 30, 32, 3, device=dev)
+    with pytest.raises(ValueError, match="divisible by recon_scale"):
+        ops.cross_regression(y, y, y, y, win_size=17, sparsity=4, recon_scale=4)
+
+
+def test_cross_regression_from_color(ops, dev):
+    H, W = 64, 72
+    r = np.random.default_rng(0)
+    c1 = r.lognormal(0, 1, (1, H, W, 3)).astype(np.float32)
+    c2 = r.lognormal(0, 1, (1, H, W, 3)).astype(np.float32)
+    alb, nrm = rand_img(H, W, seed=1), unit_normals(H, W, seed=2)
+    y1, y2, oa, ob, va, vb = ops.cross_regression_from_color(to_dev(c1, dev), to_dev(c2, dev), to_dev(alb, dev),
+                                                             to_dev(nrm, dev), 17, 4)
+    assert rel_l2(to_np(y1), O.prepare_input(c1)) < 1e-6
+    ref = O.cross_regression(np.log1p(c1.astype(np.float64)).astype(np.float32),
+                             np.log1p(c2.astype(np.float64)).astype(np.float32), alb, nrm, 17, 4, 1)
+    for name, g, rr in zip(("outa", "outb", "vara", "varb"), (oa, ob, va, vb), ref):
+        err = rel_l2(to_np(g), rr)
+        _report(f"cross_regression_from_color {name}", err)
+        assert err <= TOL
+
+
+# ------------------------------------------------------------------------------------------------
+def sf_case(H, W, seed, B=1, band_lo=0.1, band_hi=10.0):
+    ia, ib = log_images(H, W, seed=seed, B=B)
+    alb, nrm = rand_img(H, W, seed=seed + 1, B=B), unit_normals(H, W, seed=seed + 2, B=B)
+    r = np.random.default_rng(seed + 3)
+    band = np.exp(r.uniform(np.log(band_lo), np.log(band_hi), (B, H, W, 5))).astype(np.float32)
+    return ia, ib, alb, nrm, band
+
+
+@pytest.mark.parametrize("B,H,W,win", [(1, 64, 64, 11), (1, 37, 29, 11), (2, 48, 80, 11), (1, 512, 512, 11),
+                                       (1, 30, 33, 7), (1, 20, 24, 15)])
+def test_spatial_filter(ops, dev, B, H, W, win):
+    args = sf_case(H, W, seed=H + W, B=B)
+    ref = O.spatial_filter(*args, win_size=win)
+    got = ops.spatial_filter(*[to_dev(a, dev) for a in args], win_size=win)
+    for name, g, r in zip(("acca", "accb", "wgtsuma", "wgtsumb"), got, ref):
+        err = rel_l2(to_np(g), r)
+        _report(f"spatial_filter {name} {B}x{H}x{W} win={win}", err)
+        assert err <= TOL, name
+
+
+def _cots(B, H, W, seed):
+    r = np.random.default_rng(seed)
+    return [r.normal(size=(B, H, W, c)).astype(np.float32) for c in (3, 3, 1, 1)]
+
+
+@pytest.mark.parametrize("B,H,W,win", [(1, 64, 64, 11), (1, 37, 29, 11), (2, 24, 40, 11), (1, 33, 31, 9)])
+def test_spatial_filter_grad(ops, dev, B, H, W, win):
+    args = sf_case(H, W, seed=2 * H + W, B=B)
+    cots = _cots(B, H, W, 9)
+    ref = O.spatial_filter_grad(*cots, *args, win_size=win)
+    got = ops.spatial_filter_grad(*[to_dev(c, dev) for c in cots], *[to_dev(a, dev) for a in args], win_size=win)
+    err = rel_l2(to_np(got), ref)
+    _report(f"spatial_filter_grad {B}x{H}x{W} win={win}", err)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("H,W,win", [(32, 32, 11), (23, 29, 11), (16, 40, 7)])
+def test_spatial_filter_grad_inputs(ops, dev, H, W, win):
+    args = sf_case(H, W, seed=H + 3 * W, band_lo=0.1, band_hi=3.0)
+    cots = _cots(1, H, W, 10)
+    ref = O.spatial_filter_grad_inputs(*cots, *args, win_size=win)
+    got = ops.spatial_filter_grad_inputs(*[to_dev(c, dev) for c in cots], *[to_dev(a, dev) for a in args],
+                                         win_size=win)
+    for name, g, r in zip(("imga", "imgb", "albedo", "normal"), got, ref):
+        err = rel_l2(to_np(g), r)
+        _report(f"spatial_filter_grad_inputs {name} {H}x{W} win={win}", err)
+        assert err <= TOL, name
+
+
+def test_spatial_filter_autograd(ops, dev):
+    H, W = 40, 44
+    args = [to_dev(a, dev) for a in sf_case(H, W, seed=77)]
+    for a in args:
+        a.requires_grad_(True)
+    outs = ops.spatial_filter(*args, win_size=11)
+    cots = [to_dev(c, dev) for c in _cots(1, H, W, 11)]
+    L = sum((o * c).sum() for o, c in zip(outs, cots))
+    L.backward()
+    gb = ops.spatial_filter_grad(*cots, *[a.detach() for a in args], win_size=11)
+    assert torch.equal(args[4].grad, gb)
+    ref = O.spatial_filter_grad_inputs(*[to_np(c) for c in cots], *[to_np(a.detach()) for a in args])
+    for a, r in zip(args[:4], ref):
+        assert rel_l2(to_np(a.grad), r) <= TOL
+
+
+# ------------------------------------------------------------------------------------------------
+def reproject_case(H, W, seed, pan=(1.3, -0.6), dis=0.1, B=1):
+    from xdn import synth
+    out = []
+    for b in range(B):
+        spec = synth.StreamSpec(H, W, seed=seed, stream=b, pan=pan, disocclusion=dis)
+        f0, f1 = synth.make_frame(spec, 0), synth.make_frame(spec, 1)
+        out.append((f0, f1))
+    cat = lambda k, i: np.concatenate([to_np(o[i][k]) for o in out], 0)
+    r = np.random.default_rng(seed)
+    prev_imgs = [r.uniform(0, 2, (B, H, W, 3)).astype(np.float32), r.uniform(0, 2, (B, H, W, 3)).astype(np.float32),
+                 r.uniform(0, 2, (B, H, W, 1)).astype(np.float32)]
+    opac = np.ones((B, H, W, 1), np.float32)
+    popac = np.ones((B, H, W, 1), np.float32)
+    popac[:, H // 3, :] = 0.0  # a row of transparent history
+    opac[:, :, W // 5] = 0.0
+    return (cat("mvec", 1), prev_imgs, cat("linearZ", 1), cat("linearZ", 0), cat("normal", 1), cat("normal", 0),
+            cat("pnFwidth", 1), opac, popac)
+
+
+@pytest.mark.parametrize("B,H,W,pan", [(1, 64, 80, (1.3, -0.6)), (2, 45, 61, (-2.7, 2.2)), (1, 270, 480, (3.0, -3.0)),
+                                       (1, 33, 35, (0.0, 0.0)), (1, 40, 40, (-0.4, 0.45))])
+def test_reproject_bit_exact_mask(ops, dev, B, H, W, pan):
+    case = reproject_case(H, W, seed=H * 7 + W, pan=pan, B=B)
+    mvec, ins, *rest = case
+    s_ref, o_ref = O.reproject(mvec, ins, *rest)
+    s, outs = ops.Reproject(to_dev(mvec, dev), [to_dev(a, dev) for a in ins], *[to_dev(a, dev) for a in rest])
+    assert s.dtype == torch.bool
+    s = to_np(s)
+    n_diff = int((s != s_ref).sum())
+    print(f"  reproject mask {B}x{H}x{W} pan={pan}: {int(s_ref.sum())} valid, {n_diff} differ", flush=True)
+    assert n_diff == 0
+    for g, r in zip(outs, o_ref):
+        err = rel_l2(to_np(g), r)
+        _report(f"reproject values {B}x{H}x{W}", err)
+        assert err <= 1e-6
+        assert np.all(to_np(g)[~s[..., 0]] == 0)
+
+
+def test_reproject_many_buffers(ops, dev):
+    """More buffers than one launch carries: split launches, identical masks."""
+    H, W = 32, 48
+    mvec, ins, *rest = reproject_case(H, W, seed=5)
+    r = np.random.default_rng(1)
+    ins = [r.uniform(0, 1, (1, H, W, c)).astype(np.float32) for c in (3, 1, 2, 4, 3, 1)]
+    s_ref, o_ref = O.reproject(mvec, ins, *rest)
+    s, outs = ops.Reproject(to_dev(mvec, dev), [to_dev(a, dev) for a in ins], *[to_dev(a, dev) for a in rest])
+    assert np.array_equal(to_np(s), s_ref)
+    for g, rr in zip(outs, o_ref):
+        assert rel_l2(to_np(g), rr) <= 1e-6
+
+
+# ------------------------------------------------------------------------------------------------
+def screening_case(H, W, seed, B=1):
+    ia, ib = log_images(H, W, seed=seed, B=B)
+    r = np.random.default_rng(seed)
+    # history: mostly close to the current frame so both outcomes occur
+    p = ((ia + ib) * 0.5 + r.normal(0, 0.05, ia.shape)).astype(np.float32)
+    p1 = (ia + r.normal(0, 0.05, ia.shape)).astype(np.float32)
+    p2 = (ib + r.normal(0, 0.3, ia.shape)).astype(np.float32)
+    m = r.random((B, H, W, 1)) < 0.9
+    return ia, ib, p, p1, p2, m
+
+
+@pytest.mark.parametrize("B,H,W,wc,wp", [(1, 64, 64, 7, 1), (1, 37, 45, 7, 1), (2, 29, 70, 7, 1), (1, 40, 36, 5, 3),
+                                         (1, 270, 480, 7, 1)])
+def test_temporal_screening_bit_exact(ops, dev, B, H, W, wc, wp):
+    ia, ib, p, p1, p2, m = screening_case(H, W, seed=H + W, B=B)
+    rs = O.erosion(m)
+    ref = O.temporal_screening(ia, ib, p, p1, p2, rs, wc, wp)
+    got = ops.temporal_screening(*[to_dev(a, dev) for a in (ia, ib, p, p1, p2, rs)], win_curr=wc, win_prev=wp)
+    for name, g, r in zip(("output", "output1", "output2", "success"), got, ref):
+        nd = int((to_np(g) != r).sum())
+        print(f"  temporal_screening {name} {B}x{H}x{W}: {nd} elements differ", flush=True)
+        assert nd == 0, name
+    # fused erosion variant
+    got = ops.erode_temporal_screening(*[to_dev(a, dev) for a in (ia, ib, p, p1, p2)], to_dev(m, dev), wc, wp)
+    for g, r in zip(got, list(ref) + [rs]):
+        assert np.array_equal(to_np(g), r)
+
+
+# ------------------------------------------------------------------------------------------------
+def train_case(H, W, seed, B=1):
+    ia, ib = log_images(H, W, seed=seed, B=B)
+    r = np.random.default_rng(seed + 1)
+    alb, nrm = rand_img(H, W, seed=seed + 2, B=B), unit_normals(H, W, seed=seed + 3, B=B)
+    param = r.normal(0, 1.0, (B, H, W, 6)).astype(np.float32)
+    succ = (r.random((B, H, W, 1)) < 0.8).astype(np.float32)
+    sprev = ((r.random((B, H, W, 1)) < 0.8) * succ).astype(np.float32)
+    prev_out = (ia + r.normal(0, 0.1, ia.shape)).astype(np.float32)
+    opac = (r.random((B, H, W, 1)) < 0.95).astype(np.float32)
+    pols1 = (ia + r.normal(0, 0.1, ia.shape)).astype(np.float32)
+    pols2 = (ib + r.normal(0, 0.1, ia.shape)).astype(np.float32)
+    return ia, ib, alb, nrm, param, succ, sprev, prev_out, opac, pols1, pols2
+
+
+def train_oracle(ia, ib, alb, nrm, param, succ, sprev, prev_out, opac, pols1, pols2):
+    band, alpha = O.band_alpha(param)
+    band32 = band.astype(np.float32)
+    accA, accB, wA, wB = O.spatial_filter(ia, ib, alb, nrm, band32)
+    agg = (accA, wA, accB, wB, alpha, succ, sprev, prev_out, opac, ib, ia, pols2, pols1)
+    out_log, out, loss, _ = O.aggregation_loss(*agg)
+    B = ia.shape[0]
+    avg = np.array([O.aggregation_loss(*[a[b:b + 1] for a in agg])[3] for b in range(B)])
+    # per-image mean: the gradient of each image's own mean loss
+    grads = [O.aggregation_loss_grad(*[a[b:b + 1] for a in agg]) for b in range(B)]
+    gsum1, gsumw1, gsum2, gsumw2, galpha = [np.concatenate([g[k] for g in grads], 0) for k in range(5)]
+    gband = O.spatial_filter_grad(gsum1, gsum2, gsumw1, gsumw2, ia, ib, alb, nrm, band32)
+    dparam = O.band_alpha_grad(param, gband, galpha)
+    return out_log, out, loss, avg, dparam
+
+
+@pytest.mark.parametrize("B,H,W", [(1, 64, 64), (1, 37, 51), (2, 48, 40)])
+def test_filter_train(dev, B, H, W):
+    from xdn import pipeline
+    case = train_case(H, W, seed=H + W, B=B)
+    ref = train_oracle(*case)
+    got = pipeline.filter_train(*[to_dev(a, dev) for a in case], want_loss=True)
+    names = ("out_log", "out", "loss", "avg_loss", "dparam")
+    for name, g, r in zip(names, got, ref):
+        err = rel_l2(to_np(g), r)
+        _report(f"filter_train {name} {B}x{H}x{W}", err)
+        assert err <= TOL, name
+
+
+def test_adam_keras(dev):
+    from xdn._lib import check, lib, ptr
+    r = np.random.default_rng(0)
+    n = 1000
+    p, g = r.normal(size=n).astype(np.float32), r.normal(size=n).astype(np.float32)
+    m, v = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    P, G, M, V = (to_dev(a, dev) for a in (p, g, m, v))
+    pr, mr, vr = p.astype(np.float64), m.astype(np.float64), v.astype(np.float64)
+    for step in (1, 2, 3):
+        check(lib.xdn_adam(ptr(P), ptr(G), ptr(M), ptr(V), n, step, 1e-3, 0.9, 0.999, 1e-7,
+                           torch.cuda.current_stream().cuda_stream), "adam")
+        alpha = 1e-3 * np.sqrt(1 - 0.999 ** step) / (1 - 0.9 ** step)
+        mr += (g - mr) * 0.1
+        vr += (g.astype(np.float64) ** 2 - vr) * 0.001
+        pr -= alpha * mr / (np.sqrt(vr) + 1e-7)
+    torch.cuda.synchronize()
+    assert rel_l2(to_np(P), pr) < 1e-6
