@@ -67,7 +67,6 @@ def stage_times(den, frames, reps, torch):
     outs = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in b.rp])
     dims = (ctypes.c_int * 3)(3, 3, 3)
     param = den.net.forward(ols1, ols2, b.prev_out, f["albedo"], f["normal"])
-    den.net._out = None
 
     def t_cr():
         check(lib.xdn_cross_regression_from_color(ptr(f["color"]), ptr(f["color2"]), ptr(f["albedo"]),
@@ -91,15 +90,15 @@ def stage_times(den, frames, reps, torch):
                                    ptr(b.prev_ols1), ptr(b.prev_ols2), ptr(out_log), ptr(b.out), ptr(b.dparam), None,
                                    ptr(b.avg_loss), ptr(b.ws), b.ws_bytes, B, H, W, 11, s), "ft")
 
-    def t_cnn():
+    def t_cnn_fwd():
         den.net.forward(ols1, ols2, b.prev_out, f["albedo"], f["normal"])
+
+    def t_cnn_bwd():
         den.net.backward(b.dparam)
-        for p in den.net.params:
-            p.grad = None
 
     out = {}
     for name, fn in (("cross_regression", t_cr), ("reproject", t_rp), ("screening", t_ts), ("filter_train", t_ft),
-                     ("cnn_fwd_bwd", t_cnn)):
+                     ("cnn_fwd", t_cnn_fwd), ("cnn_bwd", t_cnn_bwd)):
         fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

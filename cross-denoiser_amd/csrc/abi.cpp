@@ -12,6 +12,7 @@
 #include "xdn_common.h"
 #include "xdn_kernels.h"
 #include "xdn_misc.h"
+#include "xdn_cnn.h"
 
 namespace {
 
@@ -368,6 +369,202 @@ int xdn_adam(float *param, const float *grad, float *m, float *v, int count, int
     j.count = count;
     j.alpha = (float)((double)lr * sqrt(1.0 - pow((double)beta2, step)) / (1.0 - pow((double)beta1, step)));
     return hip_check(xdn::launch_adam(1, &j, beta1, beta2, eps, S(stream)), "adam");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// cnn5: the bandwidth network (5 conv3x3: 15 -> 32 -> 32 -> 32 -> 32 -> 6)
+// ---------------------------------------------------------------------------------------------
+namespace {
+struct LayerSpec {
+    int ci, co;
+};
+const LayerSpec kCnn5[5] = {{15, 32}, {32, 32}, {32, 32}, {32, 32}, {32, 6}};
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Cnn5Layout {
+    size_t w_off[5], b_off[5], count;          // FP32 parameter offsets (floats)
+    size_t fwd_off[5], dg_off[5], bias_off[5], packed;  // packed byte offsets
+    Cnn5Layout()
+    {
+        size_t o = 0, q = 0;
+        for (int l = 0; l < 5; ++l)
+        {
+            w_off[l] = o;
+            o += 9 * (size_t)kCnn5[l].ci * kCnn5[l].co;
+            b_off[l] = o;
+            o += kCnn5[l].co;
+            const int kcf = (kCnn5[l].ci + 15) / 16, kcd = (kCnn5[l].co + 15) / 16;
+            fwd_off[l] = q;
+            q += al256((size_t)9 * kcf * 64 * 16);
+            dg_off[l] = q;
+            q += l > 0 ? al256((size_t)9 * kcd * 64 * 16) : 0;
+            bias_off[l] = q;
+            q += 256;
+        }
+        count = o;
+        packed = q;
+    }
+};
+const Cnn5Layout kLay;
+
+struct Cnn5Ws {
+    uint16_t *X[4], *dY[2];
+    float *part, *stage;
+    int nblk_total;
+    int nblk[XDN_MAX_FRAMES];
+    size_t bytes;
+    Cnn5Ws(void *base, int B, int H, int W)
+    {
+        const size_t act = al256((size_t)B * H * W * 32 * 2);
+        char *p = static_cast<char *>(base);
+        size_t o = 0;
+        for (int i = 0; i < 4; ++i) X[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
+        for (int i = 0; i < 2; ++i) dY[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
+        nblk_total = B * xdn::wgrad_blocks(H, W);
+        part = reinterpret_cast<float *>(p + o);
+        o += al256((size_t)nblk_total * xdn::WG_PART * 4);
+        stage = reinterpret_cast<float *>(p + o);
+        o += al256((size_t)xdn::WG_SPLITS * xdn::WG_PART * 4);
+        bytes = o;
+    }
+};
+}  // namespace
+
+extern "C" {
+
+size_t xdn_cnn5_param_count(void) { return kLay.count; }
+
+size_t xdn_cnn5_packed_bytes(void) { return kLay.packed; }
+
+size_t xdn_cnn5_workspace_size(int batch, int height, int width)
+{
+    if (batch <= 0 || height <= 0 || width <= 0) return 0;
+    return Cnn5Ws(nullptr, batch, height, width).bytes;
+}
+
+int xdn_cnn5_pack(const float *params, void *packed, void *stream)
+{
+    xdn::PackJob j[5];
+    char *pk = static_cast<char *>(packed);
+    for (int l = 0; l < 5; ++l)
+    {
+        j[l].w = params + kLay.w_off[l];
+        j[l].b = params + kLay.b_off[l];
+        j[l].frag_fwd = pk + kLay.fwd_off[l];
+        j[l].frag_dgrad = l > 0 ? pk + kLay.dg_off[l] : nullptr;
+        j[l].bias32 = reinterpret_cast<float *>(pk + kLay.bias_off[l]);
+        j[l].ci = kCnn5[l].ci;
+        j[l].co = kCnn5[l].co;
+    }
+    return hip_check(xdn::launch_pack(5, j, S(stream)), "cnn5 pack");
+}
+
+static int cnn5_check(int B, int H, int W, size_t ws_bytes)
+{
+    if (int e = check_shape(B, H, W)) return e;
+    if (B > XDN_MAX_FRAMES) return fail(XDN_EINVAL, "cnn5: batch must be <= %d", XDN_MAX_FRAMES);
+    if (ws_bytes < xdn_cnn5_workspace_size(B, H, W)) return fail(XDN_EINVAL, "cnn5: workspace too small");
+    return XDN_OK;
+}
+
+int xdn_cnn5_forward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,
+                     const float *normal, const void *packed, float *param, void *workspace, size_t workspace_bytes,
+                     int batch, int height, int width, void *stream)
+{
+    if (int e = cnn5_check(batch, height, width, workspace_bytes)) return e;
+    Cnn5Ws ws(workspace, batch, height, width);
+    const size_t N = px(height, width);
+    const char *pk = static_cast<const char *>(packed);
+    std::vector<xdn::ConvFrame> fr(batch);
+    for (int l = 0; l < 5; ++l)
+    {
+        for (int b = 0; b < batch; ++b)
+        {
+            xdn::ConvFrame &f = fr[b];
+            memset(&f, 0, sizeof(f));
+            const size_t o3 = (size_t)b * N * 3, o32 = (size_t)b * N * 32;
+            if (l == 0)
+            {
+                f.in[0] = ols1 + o3; f.in[1] = ols2 + o3; f.in[2] = prev_out + o3; f.in[3] = albedo + o3;
+                f.in[4] = normal + o3;
+            }
+            else
+                f.in[0] = ws.X[l - 1] + o32;
+            f.out = l < 4 ? static_cast<void *>(ws.X[l] + o32) : static_cast<void *>(param + (size_t)b * N * 6);
+            f.wfrag = pk + kLay.fwd_off[l];
+            f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[l]);
+            f.H = height;
+            f.W = width;
+        }
+        const int kind = l == 0 ? xdn::CONV_L1_FWD : (l < 4 ? xdn::CONV_MID_FWD : xdn::CONV_L5_FWD);
+        if (int e = hip_check(xdn::launch_conv(kind, batch, fr.data(), S(stream)), "cnn5 forward")) return e;
+    }
+    return XDN_OK;
+}
+
+int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,
+                      const float *normal, const float *dparam, const void *packed, float *grad, void *workspace,
+                      size_t workspace_bytes, int batch, int height, int width, void *stream)
+{
+    if (int e = cnn5_check(batch, height, width, workspace_bytes)) return e;
+    Cnn5Ws ws(workspace, batch, height, width);
+    const size_t N = px(height, width);
+    const char *pk = static_cast<const char *>(packed);
+    const int nb = xdn::wgrad_blocks(height, width);
+    std::vector<xdn::ConvFrame> cf(batch);
+    std::vector<xdn::WgFrame> wf(batch);
+    xdn::WgFrame all{};
+    all.part = ws.part;
+    all.nblk = ws.nblk_total;
+    int cur = 0;  // dY ping-pong index holding the gradient of the current layer's output
+    for (int l = 4; l >= 0; --l)
+    {
+        // weight gradient of layer l: x = input of layer l, dy = gradient of its pre-activation
+        for (int b = 0; b < batch; ++b)
+        {
+            xdn::WgFrame &f = wf[b];
+            memset(&f, 0, sizeof(f));
+            const size_t o3 = (size_t)b * N * 3, o32 = (size_t)b * N * 32;
+            if (l == 0)
+            {
+                f.x[0] = ols1 + o3; f.x[1] = ols2 + o3; f.x[2] = prev_out + o3; f.x[3] = albedo + o3; f.x[4] = normal + o3;
+            }
+            else
+                f.x[0] = ws.X[l - 1] + o32;
+            f.dy = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6) : static_cast<const void *>(ws.dY[cur] + o32);
+            f.part = ws.part + (size_t)b * nb * xdn::WG_PART;
+            f.nblk = nb;
+            f.H = height;
+            f.W = width;
+        }
+        const int wk = l == 0 ? xdn::CONV_L1_WGRAD : (l == 4 ? xdn::CONV_L5_WGRAD : xdn::CONV_MID_WGRAD);
+        if (int e = hip_check(xdn::launch_wgrad(wk, batch, wf.data(), S(stream)), "cnn5 wgrad")) return e;
+        float *g = grad + kLay.w_off[l];
+        if (int e = hip_check(xdn::launch_wgrad_reduce(1, &all, &ws.stage, &g, kCnn5[l].ci, kCnn5[l].co, S(stream)),
+                              "cnn5 wgrad reduce"))
+            return e;
+        if (l == 0) break;
+        // data gradient into the output of layer l-1, masked by its ReLU
+        for (int b = 0; b < batch; ++b)
+        {
+            xdn::ConvFrame &f = cf[b];
+            memset(&f, 0, sizeof(f));
+            const size_t o32 = (size_t)b * N * 32;
+            f.in[0] = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6) : static_cast<const void *>(ws.dY[cur] + o32);
+            f.out = ws.dY[1 - cur] + o32;
+            f.mask = ws.X[l - 1] + o32;
+            f.wfrag = pk + kLay.dg_off[l];
+            f.H = height;
+            f.W = width;
+        }
+        const int dk = l == 4 ? xdn::CONV_L5_DGRAD : xdn::CONV_MID_DGRAD;
+        if (int e = hip_check(xdn::launch_conv(dk, batch, cf.data(), S(stream)), "cnn5 dgrad")) return e;
+        cur = 1 - cur;
+    }
+    return XDN_OK;
 }
 
 }  // extern "C"

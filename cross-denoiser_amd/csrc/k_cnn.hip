@@ -1,0 +1,579 @@
+// Bandwidth CNN of the online training step on gfx950 MFMA: five 3x3 'same' convolutions
+// 15 -> 32 -> 32 -> 32 -> 32 -> 6 with ReLU between them, BF16 operands, FP32 accumulation.
+//
+// All convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16 (M = 32 pixels of one row,
+// N = 32 output channels, K = 16 input channels):
+//   conv3x3<IN, CIN, EPI>  forward and data-gradient (the data gradient is a convolution of dY
+//     with the flipped, transposed kernel). Tile: 16 rows x 32 px per 256-lane workgroup, the
+//     tile plus a 1-px halo staged in LDS as NHWC bf16 with a padded pixel stride (conflict-free
+//     ds_read_b128). Each wave owns 4 output rows; every A fragment loaded from LDS feeds the
+//     MFMAs of up to three output rows (the three kernel rows), so LDS traffic is ~0.5 fragments
+//     per MFMA. Kernel fragments live in VGPRs for the whole tile.
+//     Inputs: bf16 activations, or the five FP32 3-channel images of the network input (the
+//     concatenation and the bf16 rounding are fused into the staging), or the FP32 6-channel
+//     gradient of the network output. Epilogues: bias + ReLU -> bf16, ReLU-mask of the layer
+//     input (backward) -> bf16, or bias -> FP32 6 channels (network output).
+//   conv3x3_wgrad<X, CI, DY>  weight gradient: M = input channel, N = output channel, K = 16
+//     pixels (2 rows x 8 px). Both operands are staged channel-major in LDS; the +-1 px shifts of
+//     the kernel columns are formed in registers from aligned 8-px chunks with v_alignbit
+//     (no misaligned LDS reads, no shifted copies). A wave keeps all 9 taps' 32x32 FP32
+//     accumulators and loops over many 8x32-px strips; per-block partial sums are reduced by
+//     wgrad_reduce in two deterministic stages.
+//   pack_weights  FP32 master weights -> bf16 MFMA B fragments for the forward and the data
+//     gradient (flipped/transposed) convolutions.
+#include "xdn_common.h"
+#include "xdn_cnn.h"
+
+namespace xdn {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+__device__ __forceinline__ uint32_t f2bf(float f)
+{
+    // round to nearest even (inputs are finite)
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint32_t pack2(float a, float b) { return f2bf(a) | (f2bf(b) << 16); }
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
+__device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Forward / data-gradient convolution
+// ---------------------------------------------------------------------------------------------
+struct ConvBatch {
+    TileMap map;
+    ConvFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(ConvBatch) <= 4096, "kernel argument too large");
+
+constexpr int CT_W = 32, CT_H = 16;
+
+template <int IN, int CIN, int EPI>
+__global__ __launch_bounds__(256) void conv3x3(ConvBatch b)
+{
+    constexpr int PS = CIN + 8;  // LDS pixel stride in bf16 (20 or 12 dwords: conflict-free b128 reads)
+    constexpr int LW = CT_W + 2, LH = CT_H + 2;
+    constexpr int KC = CIN / 16;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[LH * LW * PS];
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const ConvFrame &F = b.f[L.f];
+    const int H = F.H, W = F.W;
+    const int x0 = L.tx * CT_W, y0 = L.ty * CT_H;
+    const int tid = threadIdx.x;
+
+    // ---- stage the input tile (+1 px halo, zero outside the image) ----
+    if constexpr (IN == CONV_IN_BF16)
+    {
+        const uint16_t *src = static_cast<const uint16_t *>(F.in[0]);
+        for (int i = tid; i < LH * LW * (CIN / 8); i += 256)
+        {
+            const int pix = i / (CIN / 8), q = i - pix * (CIN / 8);
+            const int ly = pix / LW, lx = pix - ly * LW;
+            const int gy = y0 + ly - 1, gx = x0 + lx - 1;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * CIN + q * 8);
+            *reinterpret_cast<u32x4 *>(&lds[pix * PS + q * 8]) = v;
+        }
+    }
+    else
+    {
+        for (int pix = tid; pix < LH * LW; pix += 256)
+        {
+            const int ly = pix / LW, lx = pix - ly * LW;
+            const int gy = y0 + ly - 1, gx = x0 + lx - 1;
+            float v[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = 0.f;
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+            {
+                const size_t p = (size_t)gy * W + gx;
+                if constexpr (IN == CONV_IN_IMAGES)
+                {
+#pragma unroll
+                    for (int im = 0; im < 5; ++im)
+                    {
+                        const float *s = static_cast<const float *>(F.in[im]) + 3 * p;
+                        v[3 * im] = s[0]; v[3 * im + 1] = s[1]; v[3 * im + 2] = s[2];
+                    }
+                }
+                else
+                {
+                    const float *s = static_cast<const float *>(F.in[0]) + 6 * p;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) v[k] = s[k];
+                }
+            }
+            u32x4 a, c;
+            a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = pack2(v[6], v[7]);
+            c.x = pack2(v[8], v[9]); c.y = pack2(v[10], v[11]); c.z = pack2(v[12], v[13]); c.w = pack2(v[14], v[15]);
+            *reinterpret_cast<u32x4 *>(&lds[pix * PS]) = a;
+            *reinterpret_cast<u32x4 *>(&lds[pix * PS + 8]) = c;
+        }
+    }
+    // ---- kernel fragments (B operands) and bias ----
+    const int lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
+    u32x4 wf[9][KC];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc) wf[t][kc] = static_cast<const u32x4 *>(F.wfrag)[(t * KC + kc) * 64 + lane];
+    const float bias = (EPI == CONV_EPI_MASK) ? 0.f : F.bias[px];
+    __syncthreads();
+
+    f32x16 acc[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[r][k] = 0.f;
+#pragma unroll
+    for (int ir = 0; ir < 6; ++ir)  // LDS row 4*wv + ir feeds output rows ir-2 .. ir (kernel row ir - orow)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+            {
+                const u32x4 a = *reinterpret_cast<const u32x4 *>(&lds[((4 * wv + ir) * LW + px + dx) * PS + kc * 16 + 8 * h]);
+#pragma unroll
+                for (int orow = 0; orow < 4; ++orow)
+                {
+                    const int dy = ir - orow;
+                    if (dy < 0 || dy > 2) continue;
+                    acc[orow] = mfma(a, wf[dy * 3 + dx][kc], acc[orow]);
+                }
+            }
+
+    // ---- epilogue: lane owns output channel px; 16 pixels i = 8(k/4) + 4h + k%4 per row ----
+#pragma unroll
+    for (int orow = 0; orow < 4; ++orow)
+    {
+        const int gy = y0 + 4 * wv + orow;
+        if (gy >= H) continue;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+        {
+            const int gx = x0 + 8 * (k >> 2) + 4 * h + (k & 3);
+            if (gx >= W) continue;
+            const size_t p = (size_t)gy * W + gx;
+            const float v = acc[orow][k];
+            if constexpr (EPI == CONV_EPI_RELU)
+                static_cast<uint16_t *>(F.out)[p * 32 + px] = (uint16_t)f2bf(fmaxf(v + bias, 0.f));
+            else if constexpr (EPI == CONV_EPI_MASK)
+            {
+                const uint16_t m = F.mask[p * 32 + px];
+                const bool pos = (m & 0x8000u) == 0 && (m & 0x7FFFu) != 0;
+                static_cast<uint16_t *>(F.out)[p * 32 + px] = pos ? (uint16_t)f2bf(v) : (uint16_t)0;
+            }
+            else
+            {
+                if (px < 6) static_cast<float *>(F.out)[p * 6 + px] = v + bias;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Weight gradient
+// ---------------------------------------------------------------------------------------------
+struct WgBatch {
+    int n;
+    int blk_begin[XDN_MAX_FRAMES + 1];
+    WgFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(WgBatch) <= 4096, "kernel argument too large");
+
+constexpr int WS_W = 32, WS_H = 8;       // strip: 8 rows x 32 px
+constexpr int XT_COLS = WS_W + 16;       // 8-px halo chunk on each side
+constexpr int XT_CS = (WS_H + 2) * XT_COLS + 8;  // per-channel stride: 488 bf16 = 244 dwords
+constexpr int DT_CS = WS_H * WS_W + 8;           // 264 bf16 = 132 dwords
+
+// shift the 8-px chunk window by one pixel (bf16 pairs in dwords)
+__device__ __forceinline__ u32x4 shift_p1(u32x4 c, u32x4 n)
+{
+    u32x4 r;
+    r.x = __builtin_amdgcn_alignbit(c.y, c.x, 16);
+    r.y = __builtin_amdgcn_alignbit(c.z, c.y, 16);
+    r.z = __builtin_amdgcn_alignbit(c.w, c.z, 16);
+    r.w = __builtin_amdgcn_alignbit(n.x, c.w, 16);
+    return r;
+}
+__device__ __forceinline__ u32x4 shift_m1(u32x4 p, u32x4 c)
+{
+    u32x4 r;
+    r.x = __builtin_amdgcn_alignbit(c.x, p.w, 16);
+    r.y = __builtin_amdgcn_alignbit(c.y, c.x, 16);
+    r.z = __builtin_amdgcn_alignbit(c.z, c.y, 16);
+    r.w = __builtin_amdgcn_alignbit(c.w, c.z, 16);
+    return r;
+}
+
+template <int XK, int CI, int DYK>
+__global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t xt[32 * XT_CS];
+    __shared__ __attribute__((aligned(16))) uint16_t dt[32 * DT_CS];
+    __shared__ float dbs[4][32];
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const WgFrame &F = b.f[f];
+    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
+    const int H = F.H, W = F.W;
+    const int sx = (W + WS_W - 1) / WS_W, sy = (H + WS_H - 1) / WS_H;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, h = lane >> 5, ch = lane & 31;
+    const int rr = 2 * wv + h;  // strip row of this lane group
+
+    f32x16 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
+    float db = 0.f;
+
+    for (int s = li; s < sx * sy; s += nblk)
+    {
+        const int y0 = (s / sx) * WS_H, x0 = (s % sx) * WS_W;
+        __syncthreads();  // previous strip's reads are done
+        // X tile, channel-major: xt[c][r][col], r <-> y0-1+r, col <-> x0-8+col
+        if constexpr (XK == CONV_IN_BF16)
+        {
+            const uint16_t *src = static_cast<const uint16_t *>(F.x[0]);
+            for (int i = tid; i < (WS_H + 2) * XT_COLS * 4; i += 256)
+            {
+                const int pix = i >> 2, q = i & 3;
+                const int r = pix / XT_COLS, col = pix - r * XT_COLS;
+                const int gy = y0 - 1 + r, gx = x0 - 8 + col;
+                u32x4 v = {0u, 0u, 0u, 0u};
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                    v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * 32 + q * 8);
+                uint16_t *d = &xt[(q * 8) * XT_CS + r * XT_COLS + col];
+                d[0 * XT_CS] = (uint16_t)v.x; d[1 * XT_CS] = (uint16_t)(v.x >> 16);
+                d[2 * XT_CS] = (uint16_t)v.y; d[3 * XT_CS] = (uint16_t)(v.y >> 16);
+                d[4 * XT_CS] = (uint16_t)v.z; d[5 * XT_CS] = (uint16_t)(v.z >> 16);
+                d[6 * XT_CS] = (uint16_t)v.w; d[7 * XT_CS] = (uint16_t)(v.w >> 16);
+            }
+        }
+        else
+        {
+            for (int pix = tid; pix < (WS_H + 2) * XT_COLS; pix += 256)
+            {
+                const int r = pix / XT_COLS, col = pix - r * XT_COLS;
+                const int gy = y0 - 1 + r, gx = x0 - 8 + col;
+                float v[15];
+#pragma unroll
+                for (int k = 0; k < 15; ++k) v[k] = 0.f;
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                {
+                    const size_t p = (size_t)gy * W + gx;
+#pragma unroll
+                    for (int im = 0; im < 5; ++im)
+                    {
+                        const float *sp = static_cast<const float *>(F.x[im]) + 3 * p;
+                        v[3 * im] = sp[0]; v[3 * im + 1] = sp[1]; v[3 * im + 2] = sp[2];
+                    }
+                }
+                uint16_t *d = &xt[r * XT_COLS + col];
+#pragma unroll
+                for (int k = 0; k < 15; ++k) d[k * XT_CS] = (uint16_t)f2bf(v[k]);
+                d[15 * XT_CS] = 0;
+            }
+        }
+        // dY tile, channel-major: dt[co][r][col], r <-> y0+r, col <-> x0+col
+        if constexpr (DYK == CONV_IN_BF16)
+        {
+            const uint16_t *src = static_cast<const uint16_t *>(F.dy);
+            for (int i = tid; i < WS_H * WS_W * 4; i += 256)
+            {
+                const int pix = i >> 2, q = i & 3;
+                const int r = pix / WS_W, col = pix - r * WS_W;
+                const int gy = y0 + r, gx = x0 + col;
+                u32x4 v = {0u, 0u, 0u, 0u};
+                if (gy < H && gx < W) v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * 32 + q * 8);
+                uint16_t *d = &dt[(q * 8) * DT_CS + r * WS_W + col];
+                d[0 * DT_CS] = (uint16_t)v.x; d[1 * DT_CS] = (uint16_t)(v.x >> 16);
+                d[2 * DT_CS] = (uint16_t)v.y; d[3 * DT_CS] = (uint16_t)(v.y >> 16);
+                d[4 * DT_CS] = (uint16_t)v.z; d[5 * DT_CS] = (uint16_t)(v.z >> 16);
+                d[6 * DT_CS] = (uint16_t)v.w; d[7 * DT_CS] = (uint16_t)(v.w >> 16);
+            }
+        }
+        else
+        {
+            for (int pix = tid; pix < WS_H * WS_W; pix += 256)
+            {
+                const int r = pix / WS_W, col = pix - r * WS_W;
+                const int gy = y0 + r, gx = x0 + col;
+                float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                if (gy < H && gx < W)
+                {
+                    const float *sp = static_cast<const float *>(F.dy) + 6 * ((size_t)gy * W + gx);
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) v[k] = sp[k];
+                }
+                uint16_t *d = &dt[r * WS_W + col];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) d[k * DT_CS] = (uint16_t)f2bf(v[k]);
+#pragma unroll
+                for (int k = 6; k < 32; ++k) d[k * DT_CS] = 0;
+            }
+        }
+        __syncthreads();
+        // B fragments of the 4 chunks of this lane group's row; bias gradient on the side
+        u32x4 B[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            B[c] = *reinterpret_cast<const u32x4 *>(&dt[ch * DT_CS + rr * WS_W + 8 * c]);
+            db += (bf_lo(B[c].x) + bf_hi(B[c].x)) + (bf_lo(B[c].y) + bf_hi(B[c].y)) +
+                  (bf_lo(B[c].z) + bf_hi(B[c].z)) + (bf_lo(B[c].w) + bf_hi(B[c].w));
+        }
+        const bool live_ci = (CI == 32) || ch < CI;
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+        {
+            // 6 aligned chunks of X row rr + dy (xt row index rr + dy <-> image row y0 - 1 + rr + dy)
+            u32x4 X[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+            {
+                X[c] = live_ci ? *reinterpret_cast<const u32x4 *>(&xt[ch * XT_CS + (rr + dy) * XT_COLS + 8 * c])
+                               : u32x4{0u, 0u, 0u, 0u};
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                acc[dy * 3 + 0] = mfma(shift_m1(X[c], X[c + 1]), B[c], acc[dy * 3 + 0]);
+                acc[dy * 3 + 1] = mfma(X[c + 1], B[c], acc[dy * 3 + 1]);
+                acc[dy * 3 + 2] = mfma(shift_p1(X[c + 1], X[c + 2]), B[c], acc[dy * 3 + 2]);
+            }
+        }
+    }
+    // partials: [t][ci][co] then [co]
+    float *out = F.part + (size_t)li * WG_PART;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+        {
+            const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
+            out[t * 1024 + ci * 32 + ch] = acc[t][k];
+        }
+    db += __shfl_xor(db, 32);
+    if (h == 0) dbs[wv][ch] = db;
+    __syncthreads();
+    if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
+}
+
+// Stage 1: stage[f][split][e] = sum of partials split*P/S .. ; stage 2 writes the flat gradient
+// of the layer ([9][CI][CO] then [CO], unpadded).
+struct RedBatch {
+    int n, splits;
+    const float *part[XDN_MAX_FRAMES];
+    int nblk[XDN_MAX_FRAMES];
+    float *stage[XDN_MAX_FRAMES];
+    float *grad[XDN_MAX_FRAMES];
+    int ci, co;
+};
+
+__global__ __launch_bounds__(256) void wgrad_reduce1(RedBatch b)
+{
+    const int f = blockIdx.z, sp = blockIdx.y;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= WG_PART) return;
+    const int nb = b.nblk[f];
+    const int lo = (int)((long)nb * sp / b.splits), hi = (int)((long)nb * (sp + 1) / b.splits);
+    float s = 0.f;
+    const float *p = b.part[f] + e;
+    for (int i = lo; i < hi; ++i) s += p[(size_t)i * WG_PART];
+    b.stage[f][sp * WG_PART + e] = s;
+}
+
+__global__ __launch_bounds__(256) void wgrad_reduce2(RedBatch b)
+{
+    const int f = blockIdx.y;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int nw = 9 * b.ci * b.co;
+    if (e >= nw + b.co) return;
+    int src;
+    if (e < nw)
+    {
+        const int t = e / (b.ci * b.co), r = e - t * b.ci * b.co, ci = r / b.co, co = r - ci * b.co;
+        src = t * 1024 + ci * 32 + co;
+    }
+    else
+        src = 9 * 1024 + (e - nw);
+    float s = 0.f;
+    for (int sp = 0; sp < b.splits; ++sp) s += b.stage[f][sp * WG_PART + src];
+    b.grad[f][e] = s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP32 master [9][ci][co] + [co]  ->  bf16 B fragments:
+//   fwd  frag[t][kc][lane] = W[t][kc*16 + 8h + j][lane%32]        (KC = ceil(ci/16))
+//   dgrad frag[t][kc][lane] = W[8-t][lane%32][kc*16 + 8h + j]     (KC = ceil(co/16))
+// zero-padded outside ci x co; bias32[c] = b[c] (0 padded).
+// ---------------------------------------------------------------------------------------------
+struct PackBatch {
+    int n;
+    PackJob j[XDN_MAX_FRAMES * 5];
+};
+
+__global__ __launch_bounds__(64) void pack_weights(PackBatch b)
+{
+    const PackJob &J = b.j[blockIdx.y];
+    const int lane = threadIdx.x, h = lane >> 5, n = lane & 31;
+    const int kcf = (J.ci + 15) / 16, kcd = (J.co + 15) / 16;
+    const int item = blockIdx.x;  // t * 2 + kc for fwd (first 18), then 18 + t * 2 + kc for dgrad
+    const float *Wm = J.w;
+    if (item < 18)
+    {
+        const int t = item >> 1, kc = item & 1;
+        if (kc >= kcf) return;
+        uint32_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            float e[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+            {
+                const int ci = kc * 16 + 8 * h + 2 * q + s;
+                e[s] = (ci < J.ci && n < J.co) ? Wm[(t * J.ci + ci) * J.co + n] : 0.f;
+            }
+            v[q] = pack2(e[0], e[1]);
+        }
+        reinterpret_cast<u32x4 *>(J.frag_fwd)[(t * kcf + kc) * 64 + lane] = u32x4{v[0], v[1], v[2], v[3]};
+        if (t == 0 && kc == 0 && h == 0) J.bias32[n] = n < J.co ? J.b[n] : 0.f;
+    }
+    else if (J.frag_dgrad)
+    {
+        const int it = item - 18, t = it >> 1, kc = it & 1;
+        if (kc >= kcd) return;
+        uint32_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            float e[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+            {
+                const int co = kc * 16 + 8 * h + 2 * q + s;
+                e[s] = (co < J.co && n < J.ci) ? Wm[((8 - t) * J.ci + n) * J.co + co] : 0.f;
+            }
+            v[q] = pack2(e[0], e[1]);
+        }
+        reinterpret_cast<u32x4 *>(J.frag_dgrad)[(t * kcd + kc) * 64 + lane] = u32x4{v[0], v[1], v[2], v[3]};
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+static int div_up(int a, int b) { return (a + b - 1) / b; }
+
+template <int IN, int CIN, int EPI>
+static hipError_t launch_conv_t(int n, const ConvFrame *fr, hipStream_t st)
+{
+    ConvBatch b{};
+    int tx[XDN_MAX_FRAMES], ty[XDN_MAX_FRAMES];
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        tx[i] = div_up(fr[i].W, CT_W);
+        ty[i] = div_up(fr[i].H, CT_H);
+    }
+    b.map = make_tilemap(n, tx, ty);
+    if (b.map.total == 0) return hipSuccess;
+    hipLaunchKernelGGL((conv3x3<IN, CIN, EPI>), dim3(b.map.total), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st)
+{
+    switch (kind)
+    {
+    case CONV_L1_FWD: return launch_conv_t<CONV_IN_IMAGES, 16, CONV_EPI_RELU>(n, fr, st);
+    case CONV_MID_FWD: return launch_conv_t<CONV_IN_BF16, 32, CONV_EPI_RELU>(n, fr, st);
+    case CONV_L5_FWD: return launch_conv_t<CONV_IN_BF16, 32, CONV_EPI_F32_6>(n, fr, st);
+    case CONV_MID_DGRAD: return launch_conv_t<CONV_IN_BF16, 32, CONV_EPI_MASK>(n, fr, st);
+    case CONV_L5_DGRAD: return launch_conv_t<CONV_IN_F32_6, 16, CONV_EPI_MASK>(n, fr, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int wgrad_blocks(int H, int W)
+{
+    const int strips = div_up(W, WS_W) * div_up(H, WS_H);
+    // ~10 strips per block, at least enough blocks to cover the chip
+    int nb = div_up(strips, 10);
+    if (nb > strips) nb = strips;
+    return nb < 1 ? 1 : nb;
+}
+
+template <int XK, int CI, int DYK>
+static hipError_t launch_wgrad_t(int n, const WgFrame *fr, hipStream_t st)
+{
+    WgBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.blk_begin[i] = acc;
+        acc += fr[i].nblk;
+    }
+    b.blk_begin[n] = acc;
+    if (acc == 0) return hipSuccess;
+    hipLaunchKernelGGL((conv3x3_wgrad<XK, CI, DYK>), dim3(acc), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st)
+{
+    switch (kind)
+    {
+    case CONV_L1_WGRAD: return launch_wgrad_t<CONV_IN_IMAGES, 16, CONV_IN_BF16>(n, fr, st);
+    case CONV_MID_WGRAD: return launch_wgrad_t<CONV_IN_BF16, 32, CONV_IN_BF16>(n, fr, st);
+    case CONV_L5_WGRAD: return launch_wgrad_t<CONV_IN_BF16, 32, CONV_IN_F32_6>(n, fr, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_wgrad_reduce(int n, const WgFrame *fr, float *const *stage, float *const *grad, int ci, int co,
+                               hipStream_t st)
+{
+    RedBatch b{};
+    b.n = n;
+    b.splits = WG_SPLITS;
+    b.ci = ci;
+    b.co = co;
+    for (int i = 0; i < n; ++i)
+    {
+        b.part[i] = fr[i].part;
+        b.nblk[i] = fr[i].nblk;
+        b.stage[i] = stage[i];
+        b.grad[i] = grad[i];
+    }
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(wgrad_reduce1, dim3(div_up(WG_PART, 256), WG_SPLITS, n), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(wgrad_reduce2, dim3(div_up(9 * ci * co + co, 256), n), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st)
+{
+    PackBatch b{};
+    b.n = n;
+    for (int i = 0; i < n; ++i) b.j[i] = jobs[i];
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_weights, dim3(36, n), dim3(64), 0, st, b);
+    return hipGetLastError();
+}
+
+}  // namespace xdn

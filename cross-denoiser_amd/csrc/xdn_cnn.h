@@ -1,0 +1,52 @@
+// Launchers of the MFMA convolution kernels of the bandwidth CNN (internal header).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xdn {
+
+enum { CONV_IN_BF16 = 0, CONV_IN_IMAGES = 1, CONV_IN_F32_6 = 2 };
+enum { CONV_EPI_RELU = 0, CONV_EPI_MASK = 1, CONV_EPI_F32_6 = 2 };
+enum {
+    CONV_L1_FWD = 0,    // 5 FP32 3-ch images (15 ch) -> bf16 32 ch, bias + ReLU
+    CONV_MID_FWD = 1,   // bf16 32 -> bf16 32, bias + ReLU
+    CONV_L5_FWD = 2,    // bf16 32 -> FP32 6, bias
+    CONV_MID_DGRAD = 3, // bf16 32 (dY) -> bf16 32, ReLU mask of the layer input
+    CONV_L5_DGRAD = 4,  // FP32 6 (dY) -> bf16 32, ReLU mask
+};
+enum { CONV_L1_WGRAD = 0, CONV_MID_WGRAD = 1, CONV_L5_WGRAD = 2 };
+
+constexpr int WG_PART = 9 * 1024 + 32;  // one block's partial: [9][32][32] weights + [32] bias
+constexpr int WG_SPLITS = 16;
+
+struct ConvFrame {
+    const void *in[5];       // see the kernel kinds
+    void *out;
+    const uint16_t *mask;    // bf16 [H][W][32] (mask epilogue)
+    const void *wfrag;       // 16-byte B fragments [9][KC][64]
+    const float *bias;       // [32]
+    int H, W;
+};
+
+struct WgFrame {
+    const void *x[5];        // bf16 [H][W][32] activation, or the 5 FP32 input images
+    const void *dy;          // bf16 [H][W][32] or FP32 [H][W][6]
+    float *part;             // [nblk][WG_PART]
+    int H, W, nblk;
+};
+
+struct PackJob {
+    const float *w, *b;      // FP32 master [9][ci][co], [co]
+    void *frag_fwd, *frag_dgrad;  // dgrad may be null
+    float *bias32;
+    int ci, co;
+};
+
+hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st);
+hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st);
+hipError_t launch_wgrad_reduce(int n, const WgFrame *fr, float *const *stage, float *const *grad, int ci, int co,
+                               hipStream_t st);
+hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st);
+int wgrad_blocks(int H, int W);
+
+}  // namespace xdn

@@ -1,9 +1,10 @@
 """Bandwidth network of the online training step.
 
 ``cnn5`` is the network of the benchmark configuration: five 3x3 convolutions
-15 -> 32 -> 32 -> 32 -> 32 -> 6 with ReLU between them, BF16 operands with FP32 accumulation and
-FP32 master weights, Keras glorot-uniform initialisation (scripts/main.py:79-118 uses Keras layers
-with the default initialisers) and Keras Adam (libxdn.so xdn_adam).
+15 -> 32 -> 32 -> 32 -> 32 -> 6 with ReLU between them, BF16 MFMA operands with FP32 accumulation,
+FP32 master weights, Keras glorot-uniform initialisation (the reference's network uses Keras
+defaults, scripts/main.py:79-118) and Keras Adam (xdn_adam). Forward, data and weight gradients
+are the hand-written gfx950 kernels of libxdn.so (xdn_cnn5_*).
 
 The head of the network (band = 1/(x^2 + 1e-4), alpha = sigmoid(x), main.py:110-113) is fused into
 the filter kernel, so the network returns the raw 6-channel output.
@@ -31,47 +32,79 @@ def init_params(channels=CNN5_CHANNELS, seed=1234):
     return [(glorot_uniform((3, 3, ci, co), g).float(), torch.zeros(co)) for ci, co in zip(channels[:-1], channels[1:])]
 
 
-class TorchCNN5:
-    """cnn5 through torch convolutions (MIOpen), channels-last BF16."""
+def flatten_params(layers) -> torch.Tensor:
+    return torch.cat([t.reshape(-1) for wb in layers for t in wb])
 
-    def __init__(self, device, seed=1234):
-        self.device = device
-        self.params = []
-        for w, b in init_params(seed=seed):
-            self.params.append(w.permute(3, 2, 0, 1).contiguous().to(device).requires_grad_(True))  # OIHW
-            self.params.append(b.to(device).requires_grad_(True))
-        self.m = [torch.zeros_like(p) for p in self.params]
-        self.v = [torch.zeros_like(p) for p in self.params]
+
+def unflatten_params(flat: torch.Tensor, channels=CNN5_CHANNELS):
+    out, o = [], 0
+    for ci, co in zip(channels[:-1], channels[1:]):
+        w = flat[o:o + 9 * ci * co].reshape(3, 3, ci, co)
+        o += 9 * ci * co
+        b = flat[o:o + co]
+        o += co
+        out.append((w, b))
+    return out
+
+
+class HipCNN5:
+    """cnn5 on libxdn.so: parameters, Adam state, packed bf16 fragments and activations resident."""
+
+    def __init__(self, device, seed=1234, shape=(1, 64, 64)):
+        self.device = torch.device(device)
+        n = int(lib.xdn_cnn5_param_count())
+        flat = flatten_params(init_params(seed=seed))
+        assert flat.numel() == n
+        self.params = flat.to(self.device)
+        self.grad = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.packed = torch.empty(int(lib.xdn_cnn5_packed_bytes()), device=self.device, dtype=torch.uint8)
         self.step_count = 0
-        self._out = None
+        self.resize(shape)
+        self.pack()
+
+    def resize(self, shape):
+        B, H, W = shape
+        self.shape = (B, H, W)
+        self.ws_bytes = int(lib.xdn_cnn5_workspace_size(B, H, W))
+        self.ws = torch.empty(self.ws_bytes, device=self.device, dtype=torch.uint8)
+        self.out = torch.empty((B, H, W, 6), device=self.device, dtype=torch.float32)
+        self._inputs = None
+
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def pack(self):
+        check(lib.xdn_cnn5_pack(ptr(self.params), ptr(self.packed), self._stream()), "cnn5 pack")
 
     def forward(self, ols1, ols2, prev_out, albedo, normal):
-        x = torch.cat([ols1, ols2, prev_out, albedo, normal], -1)  # [B,H,W,15]
-        h = x.permute(0, 3, 1, 2).to(torch.bfloat16)              # channels-last NCHW view
-        n = len(self.params) // 2
-        for i in range(n):
-            w, b = self.params[2 * i], self.params[2 * i + 1]
-            h = torch.nn.functional.conv2d(h, w.to(torch.bfloat16), b.to(torch.bfloat16), padding=1)
-            if i < n - 1:
-                h = torch.relu(h)
-        out = h.float().permute(0, 2, 3, 1)
-        self._out = out
-        return out.detach().contiguous()
+        B, H, W = self.shape
+        if tuple(ols1.shape[:3]) != (B, H, W):
+            raise ValueError(f"cnn5 built for {self.shape}, got {tuple(ols1.shape)}")
+        self._inputs = (ols1, ols2, prev_out, albedo, normal)
+        check(lib.xdn_cnn5_forward(*[ptr(t) for t in self._inputs], ptr(self.packed), ptr(self.out), ptr(self.ws),
+                                   self.ws_bytes, B, H, W, self._stream()), "cnn5 forward")
+        return self.out
 
     def backward(self, dparam):
-        self._out.backward(dparam)
-        self._out = None
+        B, H, W = self.shape
+        if self._inputs is None:
+            raise RuntimeError("cnn5 backward without a forward")
+        check(lib.xdn_cnn5_backward(*[ptr(t) for t in self._inputs], ptr(dparam), ptr(self.packed), ptr(self.grad),
+                                    ptr(self.ws), self.ws_bytes, B, H, W, self._stream()), "cnn5 backward")
 
     def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-7):
         self.step_count += 1
-        st = torch.cuda.current_stream(self.device).cuda_stream
-        for p, m, v in zip(self.params, self.m, self.v):
-            check(lib.xdn_adam(ptr(p.data), ptr(p.grad), ptr(m), ptr(v), p.numel(), self.step_count, lr, beta1,
-                               beta2, eps, st), "adam")
-            p.grad = None
+        check(lib.xdn_adam(ptr(self.params), ptr(self.grad), ptr(self.m), ptr(self.v), self.params.numel(),
+                           self.step_count, lr, beta1, beta2, eps, self._stream()), "adam")
+        self.pack()
+
+    def layers(self):
+        return unflatten_params(self.params)
 
 
-def make(kind: str, device, seed=1234, shape=None):
+def make(kind: str, device, seed=1234, shape=(1, 64, 64)):
     if kind == "cnn5":
-        return TorchCNN5(device, seed=seed)
+        return HipCNN5(device, seed=seed, shape=shape)
     raise ValueError(f"unknown network {kind!r}")

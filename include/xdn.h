@@ -145,6 +145,31 @@ int xdn_filter_train(const float *ols1, const float *ols2, const float *albedo, 
 int xdn_adam(float *param, const float *grad, float *m, float *v, int count, int step, float lr, float beta1,
              float beta2, float eps, void *stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * cnn5: the bandwidth network of the benchmark configuration (the reference's online network is
+ * the Keras U-Net of scripts/main.py:83-118, trained per frame at :289-303): five 3x3 'same'
+ * convolutions 15 -> 32 -> 32 -> 32 -> 32 -> 6, ReLU between them, BF16 MFMA operands, FP32
+ * accumulation. The network input is the concatenation [ols1, ols2, prev_out, albedo, normal]
+ * (main.py:291), fused into the first layer; its raw 6-channel output feeds xdn_filter_train.
+ *   params   FP32 [xdn_cnn5_param_count()]: per layer W [3][3][ci][co] (Keras HWIO) then b [co]
+ *   packed   xdn_cnn5_packed_bytes() bytes of bf16 MFMA fragments, rebuilt by xdn_cnn5_pack after
+ *            every parameter update
+ *   workspace xdn_cnn5_workspace_size(B,H,W) bytes: the four hidden activations kept for the
+ *            backward pass, two gradient buffers and the weight-gradient partial sums
+ * A batch of B images shares one set of weights; xdn_cnn5_backward writes the gradient summed over
+ * the batch into grad [param_count] (dparam = d loss / d network output, [B,H,W,6]).
+ * ------------------------------------------------------------------------------------------- */
+size_t xdn_cnn5_param_count(void);
+size_t xdn_cnn5_packed_bytes(void);
+size_t xdn_cnn5_workspace_size(int batch, int height, int width);
+int xdn_cnn5_pack(const float *params, void *packed, void *stream);
+int xdn_cnn5_forward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,
+                     const float *normal, const void *packed, float *param, void *workspace, size_t workspace_bytes,
+                     int batch, int height, int width, void *stream);
+int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,
+                      const float *normal, const float *dparam, const void *packed, float *grad, void *workspace,
+                      size_t workspace_bytes, int batch, int height, int width, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,72 @@
+"""The per-frame online step (xdn.pipeline.Denoiser) wires the stages as train_step does
+(scripts/main.py:250-313): each stage output of a real step is checked against the FP64 oracle run on
+that stage's own inputs, read back from the step's buffers, so the mask/selection stages stay bit-exact.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import rel_l2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_denoiser_stages_match_oracle(dev):
+    from test_gpu_parity import train_oracle
+    from xdn import pipeline, synth
+    H, W = 48, 64
+    spec = synth.StreamSpec(H, W, seed=11, pan=(1.25, -0.75), disocclusion=0.1)
+    frames = [synth.make_frame(spec, t) for t in range(3)]
+    g = lambda fr: {k: v.to(dev) for k, v in fr.items()}
+    den = pipeline.Denoiser(H, W, device=dev)
+    den.step(g(frames[0]))
+    den.step(g(frames[1]))
+    hist = [h.cpu().numpy().copy() for h in den.history()]
+    plinz, pnrm = den.prev_linz.cpu().numpy().copy(), den.prev_nrm.cpu().numpy().copy()
+    f = {k: v.numpy() for k, v in frames[2].items()}
+    out = den.step(g(frames[2]))
+    torch.cuda.synchronize()
+    b = den.buf
+    n = lambda t: t.cpu().numpy()
+    # 1. cross-regression on log1p of the colours
+    y1 = O.prepare_input(f["color"]).astype(np.float32)
+    y2 = O.prepare_input(f["color2"]).astype(np.float32)
+    r1, r2, _, _ = O.cross_regression(y1, y2, f["albedo"], f["normal"])
+    assert rel_l2(n(out["ols1"]), r1) <= 1e-4 and rel_l2(n(out["ols2"]), r2) <= 1e-4
+    # 2. reprojection of the history with the previous frame's depth/normal, opacities one
+    ones = np.ones((1, H, W, 1), np.float32)
+    s, rp = O.reproject(f["mvec"], hist, f["linearZ"], plinz, f["normal"], pnrm, f["pnFwidth"], ones, ones)
+    assert np.array_equal(n(b.success), s)
+    assert 0 < s.sum() < s.size
+    for got, ref in zip(b.rp, rp):
+        assert rel_l2(n(got), ref) <= 1e-6
+    # 3. erosion + screening on the step's own ols / reprojected history
+    ols1, ols2 = n(out["ols1"]), n(out["ols2"])
+    se = O.erosion(n(b.success))
+    ref = O.temporal_screening(ols1, ols2, *[n(t) for t in b.rp], se, 7, 1)
+    for got, r in zip((b.prev_out, b.prev_ols1, b.prev_ols2, b.success_prev), ref):
+        assert np.array_equal(n(got), r)
+    assert np.array_equal(n(b.succ_e), se)
+    # 4. fused filter + aggregation + backward with the network output of this step
+    param = n(den.net.out)
+    ref = train_oracle(ols1, ols2, f["albedo"], f["normal"], param, se, n(b.success_prev), n(b.prev_out),
+                       f["opacity"], n(b.prev_ols1), n(b.prev_ols2))
+    assert rel_l2(n(out["out_log"]), ref[0]) <= 1e-4
+    assert rel_l2(n(out["out"]), ref[1]) <= 1e-4
+    assert rel_l2(n(b.dparam), ref[4]) <= 1e-4
+    assert abs(float(n(out["loss"])[0]) - float(ref[3][0])) <= 1e-4 * abs(float(ref[3][0]))
+
+
+def test_denoiser_trains(dev):
+    """The online loss goes down over a static sequence (the network learns the bandwidths)."""
+    from xdn import pipeline, synth
+    H, W = 64, 64
+    spec = synth.StreamSpec(H, W, seed=5, pan=(0.0, 0.0))
+    frame = {k: v.to(dev) for k, v in synth.make_frame(spec, 0).items()}
+    den = pipeline.Denoiser(H, W, device=dev, lr=1e-2)
+    losses = []
+    for _ in range(30):
+        losses.append(float(den.step(frame)["loss"][0]))
+    assert np.all(np.isfinite(losses))
+    assert np.mean(losses[-5:]) < np.mean(losses[1:6])
