@@ -51,54 +51,69 @@ def parse():
 
 
 def stage_times(den, frames, reps, torch):
-    """Average duration of each stage, HIP events on the stream the kernels run on."""
+    """Average duration of each stage, HIP events on the stream the kernels run on.
+
+    Each stage is re-run through its op-level C-ABI entry point on the stream slot's own buffers
+    (the same kernels and shapes the batched step launches)."""
     import ctypes
 
-    from xdn import pipeline
     from xdn._lib import check, lib, ptr
-    b = den.buf
-    B, H, W = b.shape
+    H, W = den.H, den.W
     f = frames[1]
     st = torch.cuda.current_stream()
     s = st.cuda_stream
-    prv = b.hist[0]
-    out_log, ols1, ols2 = b.hist[1]
+    buf = den.buffer
+    prv = den.history()
+    out_log, ols1, ols2 = den.hist[den.t % 2]
+    y1, y2, var1, var2 = buf("y1", 3), buf("y2", 3), buf("var1", 1), buf("var2", 1)
+    beta_bytes = lib.xdn_cross_regression_workspace_size(1, H, W, 4)
+    beta = buf("beta", 1)
+    success, succ_e, sprev = buf("success", 1, torch.uint8), buf("succ_e", 1), buf("success_prev", 1)
+    rp = [buf("rp_out", 3), buf("rp_ols1", 3), buf("rp_ols2", 3)]
+    pout, pols1, pols2 = buf("prev_out", 3), buf("prev_ols1", 3), buf("prev_ols2", 3)
+    param, dparam, ones = buf("param", 6), buf("dparam", 6), buf("ones", 1)
+    cnn_ws = buf("cnn_ws", 1, torch.uint8)
+    cnn_ws_bytes = int(lib.xdn_cnn5_workspace_size(1, H, W))
+    ft_bytes = lib.xdn_filter_train_workspace_size(1, H, W)
+    ft_ws = torch.empty(max(ft_bytes, 4) // 4, device=den.device)
     ins = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in prv])
-    outs = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in b.rp])
+    outs = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in rp])
     dims = (ctypes.c_int * 3)(3, 3, 3)
-    param = den.net.forward(ols1, ols2, b.prev_out, f["albedo"], f["normal"])
+    pf = frames[0]
 
     def t_cr():
         check(lib.xdn_cross_regression_from_color(ptr(f["color"]), ptr(f["color2"]), ptr(f["albedo"]),
-                                                  ptr(f["normal"]), ptr(b.y1), ptr(b.y2), ptr(ols1), ptr(ols2),
-                                                  ptr(b.var1), ptr(b.var2), ptr(b.beta), b.beta_bytes, B, H, W, 17, 4,
-                                                  s), "cr")
+                                                  ptr(f["normal"]), ptr(y1), ptr(y2), ptr(ols1), ptr(ols2),
+                                                  ptr(var1), ptr(var2), ptr(beta), beta_bytes, 1, H, W, 17, 4, s), "cr")
 
     def t_rp():
-        check(lib.xdn_reproject(ptr(f["mvec"]), 3, ins, dims, ptr(f["linearZ"]), ptr(den.prev_linz), ptr(f["normal"]),
-                                ptr(den.prev_nrm), ptr(f["pnFwidth"]), ptr(b.ones), ptr(b.ones), ptr(b.success), outs,
-                                B, H, W, s), "rp")
+        check(lib.xdn_reproject(ptr(f["mvec"]), 3, ins, dims, ptr(f["linearZ"]), ptr(pf["linearZ"]), ptr(f["normal"]),
+                                ptr(pf["normal"]), ptr(f["pnFwidth"]), ptr(ones), ptr(ones), ptr(success), outs,
+                                1, H, W, s), "rp")
 
     def t_ts():
-        check(lib.xdn_erode_temporal_screening(ptr(ols1), ptr(ols2), ptr(b.rp[0]), ptr(b.rp[1]), ptr(b.rp[2]),
-                                               ptr(b.success), ptr(b.succ_e), ptr(b.prev_out), ptr(b.prev_ols1),
-                                               ptr(b.prev_ols2), ptr(b.success_prev), B, H, W, 7, 1, s), "ts")
+        check(lib.xdn_erode_temporal_screening(ptr(ols1), ptr(ols2), ptr(rp[0]), ptr(rp[1]), ptr(rp[2]),
+                                               ptr(success), ptr(succ_e), ptr(pout), ptr(pols1), ptr(pols2),
+                                               ptr(sprev), 1, H, W, 7, 1, s), "ts")
+
+    def t_cnn_fwd():
+        check(lib.xdn_cnn5_forward(ptr(ols1), ptr(ols2), ptr(pout), ptr(f["albedo"]), ptr(f["normal"]),
+                                   ptr(den.packed), ptr(param), ptr(cnn_ws), cnn_ws_bytes, 1, H, W, s), "cnn fwd")
 
     def t_ft():
         check(lib.xdn_filter_train(ptr(ols1), ptr(ols2), ptr(f["albedo"]), ptr(f["normal"]), ptr(param),
-                                   ptr(b.succ_e), ptr(b.success_prev), ptr(b.prev_out), ptr(f["opacity"]),
-                                   ptr(b.prev_ols1), ptr(b.prev_ols2), ptr(out_log), ptr(b.out), ptr(b.dparam), None,
-                                   ptr(b.avg_loss), ptr(b.ws), b.ws_bytes, B, H, W, 11, s), "ft")
-
-    def t_cnn_fwd():
-        den.net.forward(ols1, ols2, b.prev_out, f["albedo"], f["normal"])
+                                   ptr(succ_e), ptr(sprev), ptr(pout), ptr(f["opacity"]), ptr(pols1), ptr(pols2),
+                                   ptr(out_log), ptr(den.out), ptr(dparam), None, ptr(den.avg_loss), ptr(ft_ws),
+                                   ft_bytes, 1, H, W, 11, s), "ft")
 
     def t_cnn_bwd():
-        den.net.backward(b.dparam)
+        check(lib.xdn_cnn5_backward(ptr(ols1), ptr(ols2), ptr(pout), ptr(f["albedo"]), ptr(f["normal"]),
+                                    ptr(dparam), ptr(den.packed), ptr(den.grad), ptr(cnn_ws), cnn_ws_bytes, 1, H, W,
+                                    s), "cnn bwd")
 
     out = {}
-    for name, fn in (("cross_regression", t_cr), ("reproject", t_rp), ("screening", t_ts), ("filter_train", t_ft),
-                     ("cnn_fwd", t_cnn_fwd), ("cnn_bwd", t_cnn_bwd)):
+    for name, fn in (("cross_regression", t_cr), ("reproject", t_rp), ("screening", t_ts), ("cnn_fwd", t_cnn_fwd),
+                     ("filter_train", t_ft), ("cnn_bwd", t_cnn_bwd)):
         fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

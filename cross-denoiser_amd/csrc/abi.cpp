@@ -8,13 +8,13 @@
 
 #include <vector>
 
-#include "../../include/xdn.h"
-#include "xdn_common.h"
+#include "abi_internal.h"
 #include "xdn_kernels.h"
 #include "xdn_misc.h"
-#include "xdn_cnn.h"
 
-namespace {
+using namespace xdn_abi;
+
+namespace xdn_abi {
 
 thread_local char g_err[512] = "";
 
@@ -27,21 +27,9 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-int hip_check(hipError_t e, const char *what)
-{
-    if (e != hipSuccess) return fail(XDN_EHIP, "%s: %s", what, hipGetErrorString(e));
-    return XDN_OK;
-}
+}  // namespace xdn_abi
 
-inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
-
-inline size_t px(int H, int W) { return (size_t)H * (size_t)W; }
-
-int check_shape(int B, int H, int W)
-{
-    if (B < 0 || H <= 0 || W <= 0) return fail(XDN_EINVAL, "invalid shape [%d, %d, %d]", B, H, W);
-    return XDN_OK;
-}
+namespace {
 
 template <class Fill, class Launch>
 int sf_run(int B, int H, int W, int win, Fill fill, Launch launch, const char *what)
@@ -68,7 +56,7 @@ extern "C" {
 
 const char *xdn_version(void) { return "xdn 0.1 gfx950"; }
 
-const char *xdn_last_error(void) { return g_err; }
+const char *xdn_last_error(void) { return xdn_abi::g_err; }
 
 size_t xdn_cross_regression_workspace_size(int batch, int height, int width, int sparsity)
 {
@@ -376,62 +364,6 @@ int xdn_adam(float *param, const float *grad, float *m, float *v, int count, int
 // ---------------------------------------------------------------------------------------------
 // cnn5: the bandwidth network (5 conv3x3: 15 -> 32 -> 32 -> 32 -> 32 -> 6)
 // ---------------------------------------------------------------------------------------------
-namespace {
-struct LayerSpec {
-    int ci, co;
-};
-const LayerSpec kCnn5[5] = {{15, 32}, {32, 32}, {32, 32}, {32, 32}, {32, 6}};
-
-inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-struct Cnn5Layout {
-    size_t w_off[5], b_off[5], count;          // FP32 parameter offsets (floats)
-    size_t fwd_off[5], dg_off[5], bias_off[5], packed;  // packed byte offsets
-    Cnn5Layout()
-    {
-        size_t o = 0, q = 0;
-        for (int l = 0; l < 5; ++l)
-        {
-            w_off[l] = o;
-            o += 9 * (size_t)kCnn5[l].ci * kCnn5[l].co;
-            b_off[l] = o;
-            o += kCnn5[l].co;
-            const int kcf = (kCnn5[l].ci + 15) / 16, kcd = (kCnn5[l].co + 15) / 16;
-            fwd_off[l] = q;
-            q += al256((size_t)9 * kcf * 64 * 16);
-            dg_off[l] = q;
-            q += l > 0 ? al256((size_t)9 * kcd * 64 * 16) : 0;
-            bias_off[l] = q;
-            q += 256;
-        }
-        count = o;
-        packed = q;
-    }
-};
-const Cnn5Layout kLay;
-
-struct Cnn5Ws {
-    uint16_t *X[4], *dY[2];
-    float *part, *stage;
-    int nblk_total;
-    int nblk[XDN_MAX_FRAMES];
-    size_t bytes;
-    Cnn5Ws(void *base, int B, int H, int W)
-    {
-        const size_t act = al256((size_t)B * H * W * 32 * 2);
-        char *p = static_cast<char *>(base);
-        size_t o = 0;
-        for (int i = 0; i < 4; ++i) X[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
-        for (int i = 0; i < 2; ++i) dY[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
-        nblk_total = B * xdn::wgrad_blocks(H, W);
-        part = reinterpret_cast<float *>(p + o);
-        o += al256((size_t)nblk_total * xdn::WG_PART * 4);
-        stage = reinterpret_cast<float *>(p + o);
-        o += al256((size_t)xdn::WG_SPLITS * xdn::WG_PART * 4);
-        bytes = o;
-    }
-};
-}  // namespace
 
 extern "C" {
 

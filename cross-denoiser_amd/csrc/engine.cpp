@@ -1,0 +1,297 @@
+// Batched per-frame online denoising step over many render streams (xdn_denoise_step).
+//
+// One call advances every given slot (stream) by one frame. Each stage of train_step
+// (scripts/main.py:250-313) is ONE launch over all slots -- the frame-batch tile maps of the
+// kernels handle different resolutions in the same launch -- so n streams cost as many launches
+// as one: cross-regression (3), reprojection, erosion + screening, CNN forward (5), fused filter +
+// loss + backward, loss sums, CNN backward (5 wgrad + 5x2 reductions + 4 dgrad), Adam, pack.
+// Every slot owns its history, network, Adam moments and scratch; nothing is shared.
+#include <math.h>
+
+#include <vector>
+
+#include "abi_internal.h"
+#include "xdn_kernels.h"
+#include "xdn_misc.h"
+
+using namespace xdn_abi;
+
+namespace {
+
+constexpr int kRegWin = 17, kSparsity = 4, kFilterWin = 11;  // scripts/main.py:71-73
+
+// Scratch layout of one slot (all offsets 256-byte aligned).
+struct SlotScratch {
+    float *y1, *y2, *var1, *var2, *beta, *rp[3], *prev_out, *prev_ols1, *prev_ols2, *success_prev, *succ_e;
+    float *dparam, *param, *ones, *loss_parts;
+    uint8_t *success;
+    void *cnn_ws;
+    size_t cnn_ws_bytes, beta_bytes, bytes;
+    SlotScratch(void *base, int H, int W)
+    {
+        char *p = static_cast<char *>(base);
+        size_t o = 0;
+        const size_t n = px(H, W);
+        auto take = [&](size_t bytes) {
+            char *r = p + o;
+            o += al256(bytes);
+            return r;
+        };
+        y1 = (float *)take(n * 12); y2 = (float *)take(n * 12);
+        var1 = (float *)take(n * 4); var2 = (float *)take(n * 4);
+        beta_bytes = xdn_cross_regression_workspace_size(1, H, W, kSparsity);
+        beta = (float *)take(beta_bytes);
+        success = (uint8_t *)take(n);
+        for (int i = 0; i < 3; ++i) rp[i] = (float *)take(n * 12);
+        prev_out = (float *)take(n * 12); prev_ols1 = (float *)take(n * 12); prev_ols2 = (float *)take(n * 12);
+        success_prev = (float *)take(n * 4); succ_e = (float *)take(n * 4);
+        dparam = (float *)take(n * 24); param = (float *)take(n * 24);
+        ones = (float *)take(n * 4);
+        loss_parts = (float *)take((size_t)xdn::filter_train_tiles(H, W) * 4);
+        cnn_ws_bytes = Cnn5Ws(nullptr, 1, H, W).bytes;
+        cnn_ws = take(cnn_ws_bytes);
+        bytes = o;
+    }
+};
+
+__global__ void fill_kernel(float *p, size_t n, float v)
+{
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
+}
+
+int check_slot(const xdn_slot &s)
+{
+    if (s.height <= kRegWin / 2 || s.width <= kRegWin / 2)
+        return fail(XDN_EINVAL, "slot size %dx%d too small", s.height, s.width);
+    if (s.height % kSparsity || s.width % kSparsity)
+        return fail(XDN_EINVAL, "Image height and width must be divisible by sparsity");
+    return XDN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t xdn_slot_scratch_bytes(int height, int width)
+{
+    if (height <= 0 || width <= 0) return 0;
+    return SlotScratch(nullptr, height, width).bytes;
+}
+
+int xdn_slot_scratch_offsets(int height, int width, size_t *offsets)
+{
+    if (height <= 0 || width <= 0) return fail(XDN_EINVAL, "invalid slot size");
+    char *base = reinterpret_cast<char *>((size_t)1 << 40);  // any aligned base: offsets are relative
+    SlotScratch sc(base, height, width);
+    const void *p[XDN_SLOT_NBUF] = {sc.y1, sc.y2, sc.var1, sc.var2, sc.beta, sc.success, sc.rp[0], sc.rp[1],
+                                    sc.rp[2], sc.prev_out, sc.prev_ols1, sc.prev_ols2, sc.success_prev, sc.succ_e,
+                                    sc.dparam, sc.param, sc.ones, sc.loss_parts, sc.cnn_ws};
+    for (int i = 0; i < XDN_SLOT_NBUF; ++i) offsets[i] = (size_t)(static_cast<const char *>(p[i]) - base);
+    return XDN_OK;
+}
+
+int xdn_slot_init(const xdn_slot *slot, void *stream)
+{
+    if (int e = check_slot(*slot)) return e;
+    SlotScratch sc(slot->scratch, slot->height, slot->width);
+    const size_t n = px(slot->height, slot->width);
+    hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(256), 0, S(stream), sc.ones, n, 1.0f);
+    if (int e = hip_check(hipGetLastError(), "slot init")) return e;
+    return xdn_cnn5_pack(slot->params, slot->packed, stream);
+}
+
+int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
+{
+    if (n_slots < 0 || n_slots > XDN_MAX_FRAMES) return fail(XDN_EINVAL, "n_slots must be in [0, %d]", XDN_MAX_FRAMES);
+    for (int i = 0; i < n_slots; ++i)
+        if (int e = check_slot(slots[i])) return e;
+    if (n_slots == 0) return XDN_OK;
+    hipStream_t st = S(stream);
+    const int n = n_slots;
+    std::vector<SlotScratch> sc;
+    std::vector<Cnn5Ws> cw;
+    sc.reserve(n);
+    cw.reserve(n);
+    for (int i = 0; i < n; ++i)
+    {
+        sc.emplace_back(slots[i].scratch, slots[i].height, slots[i].width);
+        cw.emplace_back(sc[i].cnn_ws, 1, slots[i].height, slots[i].width);
+    }
+    // 1. cross-regression with the log1p prologue (variance launches hold 2 frames per slot)
+    {
+        std::vector<xdn::CrFrame> fr(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const xdn_slot &s = slots[i];
+            xdn::CrFrame &f = fr[i];
+            f.color = s.color; f.color2 = s.color2; f.y1 = sc[i].y1; f.y2 = sc[i].y2;
+            f.var1 = sc[i].var1; f.var2 = sc[i].var2; f.alb = s.albedo; f.nrm = s.normal; f.beta = sc[i].beta;
+            f.outa = s.ols1; f.outb = s.ols2; f.H = s.height; f.W = s.width;
+        }
+        for (int b0 = 0; b0 < n; b0 += XDN_MAX_FRAMES / 2)
+        {
+            const int m = n - b0 < XDN_MAX_FRAMES / 2 ? n - b0 : XDN_MAX_FRAMES / 2;
+            if (int e = hip_check(xdn::launch_variance(m, fr.data() + b0, 3, true, st), "variance")) return e;
+        }
+        if (int e = hip_check(xdn::launch_regression(n, fr.data(), kRegWin, kSparsity, st), "regression")) return e;
+        if (int e = hip_check(xdn::launch_recon(n, fr.data(), kRegWin, kSparsity, 1, st), "recon")) return e;
+    }
+    // 2. reprojection of [out_log, ols1, ols2] of the previous frame (main.py:266, opacities one)
+    {
+        std::vector<xdn::RpFrame> fr(n);
+        const int dims[3] = {3, 3, 3};
+        for (int i = 0; i < n; ++i)
+        {
+            const xdn_slot &s = slots[i];
+            xdn::RpFrame &f = fr[i];
+            memset(&f, 0, sizeof(f));
+            f.mvec = s.mvec; f.linZ = s.linear_z; f.prevLinZ = s.prev_linear_z; f.nrm = s.normal;
+            f.prevNrm = s.prev_normal; f.pnFw = s.pn_fwidth; f.opac = sc[i].ones; f.prevOpac = sc[i].ones;
+            f.success = sc[i].success;
+            f.in[0] = s.prev_out_log; f.in[1] = s.prev_ols1; f.in[2] = s.prev_ols2;
+            f.out[0] = sc[i].rp[0]; f.out[1] = sc[i].rp[1]; f.out[2] = sc[i].rp[2];
+            f.H = s.height; f.W = s.width;
+        }
+        if (int e = hip_check(xdn::launch_reproject(n, fr.data(), 3, dims, st), "reproject")) return e;
+    }
+    // 3. 3x3 erosion + temporal screening (win_curr 7, win_prev 1, main.py:270-280)
+    {
+        std::vector<xdn::TsFrame> fr(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const xdn_slot &s = slots[i];
+            xdn::TsFrame &f = fr[i];
+            f.c1 = s.ols1; f.c2 = s.ols2; f.prev = sc[i].rp[0]; f.prev1 = sc[i].rp[1]; f.prev2 = sc[i].rp[2];
+            f.rsucc = nullptr; f.succ_mask = sc[i].success; f.succ_e = sc[i].succ_e;
+            f.o = sc[i].prev_out; f.o1 = sc[i].prev_ols1; f.o2 = sc[i].prev_ols2; f.osucc = sc[i].success_prev;
+            f.H = s.height; f.W = s.width;
+        }
+        if (int e = hip_check(xdn::launch_screening(n, fr.data(), 7, 1, true, st), "screening")) return e;
+    }
+    // 4. network forward on [ols1, ols2, prev_out, albedo, normal] (main.py:291)
+    {
+        std::vector<xdn::ConvFrame> fr(n);
+        for (int l = 0; l < 5; ++l)
+        {
+            for (int i = 0; i < n; ++i)
+            {
+                const xdn_slot &s = slots[i];
+                const char *pk = static_cast<const char *>(s.packed);
+                xdn::ConvFrame &f = fr[i];
+                memset(&f, 0, sizeof(f));
+                if (l == 0)
+                {
+                    f.in[0] = s.ols1; f.in[1] = s.ols2; f.in[2] = sc[i].prev_out; f.in[3] = s.albedo; f.in[4] = s.normal;
+                }
+                else
+                    f.in[0] = cw[i].X[l - 1];
+                f.out = l < 4 ? static_cast<void *>(cw[i].X[l]) : static_cast<void *>(sc[i].param);
+                f.wfrag = pk + kLay.fwd_off[l];
+                f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[l]);
+                f.H = s.height; f.W = s.width;
+            }
+            const int kind = l == 0 ? xdn::CONV_L1_FWD : (l < 4 ? xdn::CONV_MID_FWD : xdn::CONV_L5_FWD);
+            if (int e = hip_check(xdn::launch_conv(kind, n, fr.data(), st), "cnn forward")) return e;
+        }
+    }
+    // 5. fused spatial filter + temporal accumulation + loss + backward to the network output
+    {
+        std::vector<xdn::TrainFrame> fr(n);
+        std::vector<const float *> parts(n);
+        std::vector<int> cnt(n);
+        std::vector<float> scale(n);
+        std::vector<float *> outp(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const xdn_slot &s = slots[i];
+            xdn::TrainFrame &f = fr[i];
+            f.ols1 = s.ols1; f.ols2 = s.ols2; f.alb = s.albedo; f.nrm = s.normal; f.param = sc[i].param;
+            f.succ_e = sc[i].succ_e; f.succ_prev = sc[i].success_prev; f.prev_out = sc[i].prev_out;
+            f.opacity = s.opacity; f.prev_ols1 = sc[i].prev_ols1; f.prev_ols2 = sc[i].prev_ols2;
+            f.out_log = s.out_log; f.out = s.out; f.dparam = sc[i].dparam; f.loss = nullptr;
+            f.loss_part = sc[i].loss_parts; f.H = s.height; f.W = s.width;
+            parts[i] = sc[i].loss_parts;
+            cnt[i] = xdn::filter_train_tiles(s.height, s.width);
+            scale[i] = (float)(1.0 / (3.0 * (double)px(s.height, s.width)));
+            outp[i] = s.avg_loss;
+        }
+        if (int e = hip_check(xdn::launch_filter_train(n, fr.data(), kFilterWin, st), "filter_train")) return e;
+        if (int e = hip_check(xdn::launch_sums(n, parts.data(), cnt.data(), scale.data(), outp.data(), st), "loss")) return e;
+    }
+    // 6. network backward: per layer one weight-gradient launch, per-slot reductions, one data-gradient launch
+    {
+        std::vector<xdn::WgFrame> wf(n);
+        std::vector<xdn::ConvFrame> cf(n);
+        std::vector<float *> stage(n), grad(n);
+        int cur = 0;
+        for (int l = 4; l >= 0; --l)
+        {
+            for (int i = 0; i < n; ++i)
+            {
+                const xdn_slot &s = slots[i];
+                xdn::WgFrame &f = wf[i];
+                memset(&f, 0, sizeof(f));
+                if (l == 0)
+                {
+                    f.x[0] = s.ols1; f.x[1] = s.ols2; f.x[2] = sc[i].prev_out; f.x[3] = s.albedo; f.x[4] = s.normal;
+                }
+                else
+                    f.x[0] = cw[i].X[l - 1];
+                f.dy = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
+                f.part = cw[i].part;
+                f.nblk = cw[i].nblk_total;
+                f.H = s.height; f.W = s.width;
+                stage[i] = cw[i].stage;
+                grad[i] = s.grad + kLay.w_off[l];
+            }
+            const int wk = l == 0 ? xdn::CONV_L1_WGRAD : (l == 4 ? xdn::CONV_L5_WGRAD : xdn::CONV_MID_WGRAD);
+            if (int e = hip_check(xdn::launch_wgrad(wk, n, wf.data(), st), "cnn wgrad")) return e;
+            if (int e = hip_check(xdn::launch_wgrad_reduce(n, wf.data(), stage.data(), grad.data(), kCnn5[l].ci,
+                                                           kCnn5[l].co, st), "cnn wgrad reduce"))
+                return e;
+            if (l == 0) break;
+            for (int i = 0; i < n; ++i)
+            {
+                const xdn_slot &s = slots[i];
+                xdn::ConvFrame &f = cf[i];
+                memset(&f, 0, sizeof(f));
+                f.in[0] = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
+                f.out = cw[i].dY[1 - cur];
+                f.mask = cw[i].X[l - 1];
+                f.wfrag = static_cast<const char *>(s.packed) + kLay.dg_off[l];
+                f.H = s.height; f.W = s.width;
+            }
+            const int dk = l == 4 ? xdn::CONV_L5_DGRAD : xdn::CONV_MID_DGRAD;
+            if (int e = hip_check(xdn::launch_conv(dk, n, cf.data(), st), "cnn dgrad")) return e;
+            cur = 1 - cur;
+        }
+    }
+    // 7. Keras Adam on every slot's parameters, then rebuild the bf16 fragments
+    {
+        std::vector<xdn::AdamJob> jobs(n);
+        std::vector<xdn::PackJob> pj(5 * n);
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-7f;
+        for (int i = 0; i < n; ++i)
+        {
+            const xdn_slot &s = slots[i];
+            const int t = s.adam_step < 1 ? 1 : s.adam_step;
+            jobs[i].param = s.params; jobs[i].grad = s.grad; jobs[i].m = s.adam_m; jobs[i].v = s.adam_v;
+            jobs[i].count = (int)kLay.count;
+            jobs[i].alpha = (float)((double)lr * sqrt(1.0 - pow((double)b2, t)) / (1.0 - pow((double)b1, t)));
+            char *pk = static_cast<char *>(s.packed);
+            for (int l = 0; l < 5; ++l)
+            {
+                xdn::PackJob &j = pj[5 * i + l];
+                j.w = s.params + kLay.w_off[l]; j.b = s.params + kLay.b_off[l];
+                j.frag_fwd = pk + kLay.fwd_off[l]; j.frag_dgrad = l > 0 ? pk + kLay.dg_off[l] : nullptr;
+                j.bias32 = reinterpret_cast<float *>(pk + kLay.bias_off[l]);
+                j.ci = kCnn5[l].ci; j.co = kCnn5[l].co;
+            }
+        }
+        if (int e = hip_check(xdn::launch_adam(n, jobs.data(), b1, b2, eps, st), "adam")) return e;
+        if (int e = hip_check(xdn::launch_pack(5 * n, pj.data(), st), "pack")) return e;
+    }
+    return XDN_OK;
+}
+
+}  // extern "C"

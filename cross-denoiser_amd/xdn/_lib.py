@@ -49,6 +49,10 @@ PROTOTYPES = {
     "xdn_cnn5_pack": (_I, [_P, _P, _P]),
     "xdn_cnn5_forward": (_I, [_P] * 7 + [_P, _Z] + [_I] * 3 + [_P]),
     "xdn_cnn5_backward": (_I, [_P] * 8 + [_P, _Z] + [_I] * 3 + [_P]),
+    "xdn_slot_scratch_bytes": (_Z, [_I, _I]),
+    "xdn_slot_scratch_offsets": (_I, [_I, _I, _P]),
+    "xdn_slot_init": (_I, [_P, _P]),
+    "xdn_denoise_step": (_I, [_P, _I, _F, _P]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():

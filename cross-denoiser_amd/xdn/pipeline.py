@@ -1,29 +1,46 @@
 """Per-frame online denoising step (train_step, scripts/main.py:250-313) on libxdn.so.
 
-Stage order per frame, all on one HIP stream, all buffers preallocated per stream slot:
+A render stream is a *slot* (xdn_slot in include/xdn.h): its history, bandwidth network, Adam
+moments and scratch live in HBM for the stream's lifetime. ``xdn_denoise_step`` advances any number
+of slots (up to 16, mixed resolutions) by one frame with one launch per stage:
 
-  1. cross_regression_from_color  log1p + variance + regression + block recon -> ols1, ols2
-  2. Reproject                    history [out_log, ols1, ols2] with depth/normal validity
-  3. erode_temporal_screening     3x3 erosion of the mask fused into screening -> prev_out, ...
-  4. bandwidth CNN forward        [ols1, ols2, prev_out, albedo, normal] -> raw param (6 ch)
-  5. filter_train                 spatial filter + temporal accumulation + loss + backward to param
-  6. CNN backward + Adam          (Keras Adam semantics)
-  7. history handoff              out_log, ols1, ols2 (+ this frame's linearZ, normal), ping-pong
+  1. cross-regression with the log1p prologue   -> ols1, ols2
+  2. reprojection of [out_log, ols1, ols2]      (opacities one, main.py:266)
+  3. 3x3 erosion + temporal screening           (win_curr 7, win_prev 1)
+  4. bandwidth CNN forward on [ols1, ols2, prev_out, albedo, normal]
+  5. fused spatial filter + temporal accumulation + loss + backward to the network output
+  6. CNN backward, Keras Adam, bf16 fragment repack
+  7. history handoff (main.py:338-341): out_log/ols1/ols2 ping-pong, the frame's linearZ/normal
 
 Labels follow main.py:283-286; frame-0 history is zeros (main.py:323-330).
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from . import cnn as cnn_mod
-from . import ops
 from ._lib import check, lib, ptr
 
 REGRESSION_WIDTH = 17   # scripts/main.py:71-74
 FILTER_WIDTH = 11
 SPARSITY = 4
 LEARNING_RATE = 1e-3
+
+SLOT_BUFFERS = ("y1", "y2", "var1", "var2", "beta", "success", "rp_out", "rp_ols1", "rp_ols2", "prev_out",
+                "prev_ols1", "prev_ols2", "success_prev", "succ_e", "dparam", "param", "ones", "loss_parts",
+                "cnn_ws")
+
+
+class XdnSlot(ctypes.Structure):
+    """Mirror of xdn_slot (include/xdn.h)."""
+    _fields_ = ([("height", ctypes.c_int), ("width", ctypes.c_int)] +
+                [(n, ctypes.c_void_p) for n in (
+                    "color", "color2", "albedo", "normal", "mvec", "linear_z", "pn_fwidth", "opacity",
+                    "prev_out_log", "prev_ols1", "prev_ols2", "prev_linear_z", "prev_normal",
+                    "out_log", "ols1", "ols2", "out", "scratch", "params", "grad", "adam_m", "adam_v", "packed")] +
+                [("adam_step", ctypes.c_int), ("avg_loss", ctypes.c_void_p)])
 
 
 def filter_train(ols1, ols2, albedo, normal, param, success, success_prev, prev_out, opacity, prev_ols1, prev_ols2,
@@ -48,89 +65,116 @@ def filter_train(ols1, ols2, albedo, normal, param, success, success_prev, prev_
     return out_log, out, loss, avg, dparam
 
 
-class FrameBuffers:
-    """All per-frame intermediates of one stream slot at one resolution (preallocated)."""
-
-    def __init__(self, B: int, H: int, W: int, device):
-        e3 = lambda: torch.empty((B, H, W, 3), device=device, dtype=torch.float32)
-        e1 = lambda: torch.empty((B, H, W, 1), device=device, dtype=torch.float32)
-        self.shape = (B, H, W)
-        self.y1, self.y2, self.var1, self.var2 = e3(), e3(), e1(), e1()
-        self.beta_bytes = lib.xdn_cross_regression_workspace_size(B, H, W, SPARSITY)
-        self.beta = torch.empty(self.beta_bytes // 4, device=device, dtype=torch.float32)
-        self.success = torch.empty((B, H, W, 1), device=device, dtype=torch.bool)
-        self.rp = [e3(), e3(), e3()]
-        self.prev_out, self.prev_ols1, self.prev_ols2 = e3(), e3(), e3()
-        self.success_prev, self.succ_e = e1(), e1()
-        self.net_in = torch.empty((B, H, W, 15), device=device, dtype=torch.float32)
-        self.out = e3()
-        self.dparam = torch.empty((B, H, W, 6), device=device, dtype=torch.float32)
-        self.avg_loss = torch.empty(B, device=device, dtype=torch.float32)
-        self.ws_bytes = lib.xdn_filter_train_workspace_size(B, H, W)
-        self.ws = torch.empty(max(self.ws_bytes, 4) // 4, device=device, dtype=torch.float32)
-        self.ones = torch.ones((B, H, W, 1), device=device, dtype=torch.float32)
-        # history ping-pong: [out_log, ols1, ols2] x 2
-        self.hist = [[e3(), e3(), e3()], [e3(), e3(), e3()]]
+def slot_bytes(H: int, W: int) -> int:
+    """HBM resident per stream slot: scratch, history, outputs and network state."""
+    hist = 7 * H * W * 3 * 4  # 2 x (out_log, ols1, ols2) + out
+    net = 4 * int(lib.xdn_cnn5_param_count()) * 4 + int(lib.xdn_cnn5_packed_bytes()) + 4
+    return int(lib.xdn_slot_scratch_bytes(H, W)) + hist + net
 
 
 class Denoiser:
-    """One render stream: resident history, bandwidth CNN and its Adam state."""
+    """One render stream: resident history, bandwidth network (cnn5) and its Adam state."""
 
-    def __init__(self, H: int, W: int, device="cuda", B: int = 1, seed: int = 1234, lr: float = LEARNING_RATE,
-                 cnn: str = "cnn5"):
+    def __init__(self, H: int, W: int, device="cuda", seed: int = 1234, lr: float = LEARNING_RATE):
+        if H % SPARSITY or W % SPARSITY or H <= REGRESSION_WIDTH // 2 or W <= REGRESSION_WIDTH // 2:
+            raise ValueError(f"stream size {H}x{W}: height and width must be divisible by {SPARSITY} and > 8")
         self.device = torch.device(device)
-        self.buf = FrameBuffers(B, H, W, self.device)
-        self.net = cnn_mod.make(cnn, self.device, seed=seed, shape=(B, H, W))
+        self.H, self.W = H, W
         self.lr = lr
+        dev = self.device
+        e3 = lambda: torch.zeros((1, H, W, 3), device=dev, dtype=torch.float32)
+        self.scratch = torch.empty(int(lib.xdn_slot_scratch_bytes(H, W)), device=dev, dtype=torch.uint8)
+        self.hist = [[e3(), e3(), e3()], [e3(), e3(), e3()]]  # [out_log, ols1, ols2] ping-pong
+        self.out = e3()
+        flat = cnn_mod.flatten_params(cnn_mod.init_params(seed=seed))
+        self.params = flat.to(dev)
+        self.grad = torch.zeros_like(self.params)
+        self.adam_m = torch.zeros_like(self.params)
+        self.adam_v = torch.zeros_like(self.params)
+        self.packed = torch.empty(int(lib.xdn_cnn5_packed_bytes()), device=dev, dtype=torch.uint8)
+        self.avg_loss = torch.zeros(1, device=dev, dtype=torch.float32)
+        self.zeros3 = e3()
+        self.prev_frame = None  # G-buffers of the previous frame (linearZ, normal)
         self.t = 0
-        self.prev_linz = torch.zeros((B, H, W, 3), device=self.device)
-        self.prev_nrm = torch.zeros((B, H, W, 3), device=self.device)
-        for h in self.buf.hist[1]:
-            h.zero_()
+        offs = (ctypes.c_size_t * len(SLOT_BUFFERS))()
+        check(lib.xdn_slot_scratch_offsets(H, W, ctypes.addressof(offs)), "slot offsets")
+        self._offsets = dict(zip(SLOT_BUFFERS, [int(o) for o in offs]))
+        slot = self.make_slot(None)
+        check(lib.xdn_slot_init(ctypes.addressof(slot), self._stream()), "slot init")
+
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def buffer(self, name: str, channels: int, dtype=torch.float32) -> torch.Tensor:
+        """View of an intermediate of the last step inside the slot scratch (see SLOT_BUFFERS)."""
+        o = self._offsets[name]
+        n = self.H * self.W * channels * torch.empty(0, dtype=dtype).element_size()
+        return self.scratch[o:o + n].view(dtype).view(1, self.H, self.W, channels)
 
     def history(self):
         """(out_log, ols1, ols2) of the previous frame."""
-        return self.buf.hist[(self.t + 1) % 2]
+        return self.hist[(self.t + 1) % 2]
+
+    def make_slot(self, frame) -> XdnSlot:
+        s = XdnSlot()
+        s.height, s.width = self.H, self.W
+        if frame is not None:
+            for k, name in (("color", "color"), ("color2", "color2"), ("albedo", "albedo"), ("normal", "normal"),
+                            ("mvec", "mvec"), ("linearZ", "linear_z"), ("pnFwidth", "pn_fwidth"),
+                            ("opacity", "opacity")):
+                t = frame[k]
+                if t.device != self.device or t.dtype != torch.float32 or not t.is_contiguous():
+                    raise ValueError(f"frame[{k!r}] must be a contiguous float32 tensor on {self.device}")
+                if tuple(t.shape[:3]) != (1, self.H, self.W):
+                    raise ValueError(f"frame[{k!r}] has shape {tuple(t.shape)}, stream is {self.H}x{self.W}")
+                setattr(s, name, t.data_ptr())
+        prv, cur = self.hist[(self.t + 1) % 2], self.hist[self.t % 2]
+        s.prev_out_log, s.prev_ols1, s.prev_ols2 = (t.data_ptr() for t in prv)
+        pf = self.prev_frame
+        s.prev_linear_z = (pf["linearZ"] if pf is not None else self.zeros3).data_ptr()
+        s.prev_normal = (pf["normal"] if pf is not None else self.zeros3).data_ptr()
+        s.out_log, s.ols1, s.ols2 = (t.data_ptr() for t in cur)
+        s.out = self.out.data_ptr()
+        s.scratch = self.scratch.data_ptr()
+        s.params, s.grad, s.adam_m, s.adam_v = (t.data_ptr() for t in (self.params, self.grad, self.adam_m,
+                                                                        self.adam_v))
+        s.packed = self.packed.data_ptr()
+        s.adam_step = self.t + 1
+        s.avg_loss = self.avg_loss.data_ptr()
+        return s
+
+    def advance(self, frame):
+        """Host bookkeeping after the step of `frame` was enqueued."""
+        self.prev_frame = frame
+        self.t += 1
+
+    def outputs(self) -> dict:
+        cur = self.hist[(self.t + 1) % 2]  # written by the last step
+        return {"out": self.out, "out_log": cur[0], "ols1": cur[1], "ols2": cur[2], "loss": self.avg_loss}
 
     def step(self, frame: dict) -> dict:
-        """One train_step on frame (dict of [B,H,W,C] float32 tensors, see xdn.synth)."""
-        b = self.buf
-        B, H, W = b.shape
-        st = torch.cuda.current_stream(self.device).cuda_stream
-        cur = b.hist[self.t % 2]           # written this frame
-        prv = b.hist[(self.t + 1) % 2]     # previous frame's
-        out_log, ols1, ols2 = cur
-        # 1. cross-regression with the log1p prologue
-        check(lib.xdn_cross_regression_from_color(
-            ptr(frame["color"]), ptr(frame["color2"]), ptr(frame["albedo"]), ptr(frame["normal"]), ptr(b.y1),
-            ptr(b.y2), ptr(ols1), ptr(ols2), ptr(b.var1), ptr(b.var2), ptr(b.beta), b.beta_bytes, B, H, W,
-            REGRESSION_WIDTH, SPARSITY, st), "cross_regression")
-        # 2. reprojection of the history (main.py:266: opacity = prev_opacity = ones)
-        import ctypes
-        ins = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in prv])
-        outs = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in b.rp])
-        dims = (ctypes.c_int * 3)(3, 3, 3)
-        check(lib.xdn_reproject(ptr(frame["mvec"]), 3, ins, dims, ptr(frame["linearZ"]), ptr(self.prev_linz),
-                                ptr(frame["normal"]), ptr(self.prev_nrm), ptr(frame["pnFwidth"]), ptr(b.ones),
-                                ptr(b.ones), ptr(b.success), outs, B, H, W, st), "reproject")
-        # 3. erosion + temporal screening (win_curr 7, win_prev 1, main.py:280)
-        check(lib.xdn_erode_temporal_screening(
-            ptr(ols1), ptr(ols2), ptr(b.rp[0]), ptr(b.rp[1]), ptr(b.rp[2]), ptr(b.success), ptr(b.succ_e),
-            ptr(b.prev_out), ptr(b.prev_ols1), ptr(b.prev_ols2), ptr(b.success_prev), B, H, W, 7, 1, st),
-            "erode_temporal_screening")
-        # 4. network forward on [ols1, ols2, prev_out, albedo, normal] (main.py:291)
-        param = self.net.forward(ols1, ols2, b.prev_out, frame["albedo"], frame["normal"])
-        # 5. fused filter + aggregation + loss + backward to param
-        check(lib.xdn_filter_train(
-            ptr(ols1), ptr(ols2), ptr(frame["albedo"]), ptr(frame["normal"]), ptr(param), ptr(b.succ_e),
-            ptr(b.success_prev), ptr(b.prev_out), ptr(frame["opacity"]), ptr(b.prev_ols1), ptr(b.prev_ols2),
-            ptr(out_log), ptr(b.out), ptr(b.dparam), None, ptr(b.avg_loss), ptr(b.ws), b.ws_bytes, B, H, W,
-            FILTER_WIDTH, st), "filter_train")
-        # 6. network backward + Adam
-        self.net.backward(b.dparam)
-        self.net.adam_step(self.lr)
-        # 7. history handoff (main.py:338-341)
-        self.prev_linz.copy_(frame["linearZ"])
-        self.prev_nrm.copy_(frame["normal"])
-        self.t += 1
-        return {"out": b.out, "out_log": out_log, "ols1": ols1, "ols2": ols2, "loss": b.avg_loss}
+        """One train_step on `frame` (dict of [1,H,W,C] float32 tensors, see xdn.synth).
+
+        The frame's linearZ and normal are kept (not copied) as the next frame's history: do not
+        modify them in place before the next step has run.
+        """
+        slot = self.make_slot(frame)
+        check(lib.xdn_denoise_step(ctypes.addressof(slot), 1, self.lr, self._stream()), "denoise_step")
+        self.advance(frame)
+        return self.outputs()
+
+
+def denoise_step_multi(denoisers, frames, stream=None):
+    """One frame for each (denoiser, frame) pair in ONE batched step (all stages batched)."""
+    if not denoisers:
+        return
+    if len(denoisers) > 16:
+        raise ValueError("at most 16 streams per batched step")
+    lr = denoisers[0].lr
+    if any(d.lr != lr for d in denoisers):
+        raise ValueError("streams of one batched step share the learning rate")
+    arr = (XdnSlot * len(denoisers))(*[d.make_slot(f) for d, f in zip(denoisers, frames)])
+    st = stream if stream is not None else torch.cuda.current_stream(denoisers[0].device).cuda_stream
+    check(lib.xdn_denoise_step(ctypes.addressof(arr), len(denoisers), lr, st), "denoise_step")
+    for d, f in zip(denoisers, frames):
+        d.advance(f)

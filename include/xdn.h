@@ -170,6 +170,47 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
                       const float *normal, const float *dparam, const void *packed, float *grad, void *workspace,
                       size_t workspace_bytes, int batch, int height, int width, void *stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Batched online denoising step: train_step (scripts/main.py:250-313) for many render streams.
+ * One call advances every slot by one frame, each stage being ONE launch over all slots (their
+ * resolutions may differ): cross-regression with the log1p prologue, reprojection of the history
+ * (opacities one, main.py:266), erosion + temporal screening (win 7/1), cnn5 forward, fused
+ * filter + temporal accumulation + loss + backward, cnn5 backward, Keras Adam, fragment repack.
+ * Labels and history handoff follow main.py:283-286 and :338-341: the caller passes this frame's
+ * out_log/ols1/ols2 buffers as the next frame's prev_* (ping-pong) together with this frame's
+ * linear_z/normal. Frame 0 of a stream uses zero history (main.py:323-330).
+ * No counterpart call exists in the reference (it runs one stream, op by op).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct xdn_slot {
+    int height, width; /* divisible by 4, > 8 */
+    /* this frame, NHWC float32: [H,W,3] except opacity [H,W,1] */
+    const float *color, *color2, *albedo, *normal, *mvec, *linear_z, *pn_fwidth, *opacity;
+    /* history from the previous frame: out_log, ols1, ols2, linear_z, normal [H,W,3] */
+    const float *prev_out_log, *prev_ols1, *prev_ols2, *prev_linear_z, *prev_normal;
+    /* outputs of this frame [H,W,3] (out = max(0, expm1(out_log))) */
+    float *out_log, *ols1, *ols2, *out;
+    /* slot-owned scratch of xdn_slot_scratch_bytes(H, W) bytes, prepared once by xdn_slot_init */
+    void *scratch;
+    /* network state: params, grad, Adam m and v [xdn_cnn5_param_count()], packed fragments */
+    float *params, *grad, *adam_m, *adam_v;
+    void *packed;
+    int adam_step; /* step number of this update, from 1 */
+    float *avg_loss; /* [1] mean loss of this frame */
+} xdn_slot;
+
+size_t xdn_slot_scratch_bytes(int height, int width);
+/* Byte offsets inside the scratch of its XDN_SLOT_NBUF intermediate buffers, in this order:
+ * y1, y2 [H,W,3]; var1, var2 [H,W,1]; regression coefficients; reprojection success [H,W,1] u8;
+ * reprojected out_log, ols1, ols2 [H,W,3]; screened prev_out, prev_ols1, prev_ols2 [H,W,3];
+ * success_prev, eroded success [H,W,1]; dparam, network output [H,W,6]; ones [H,W,1];
+ * loss partials; cnn5 workspace. For inspection and tests. */
+#define XDN_SLOT_NBUF 19
+int xdn_slot_scratch_offsets(int height, int width, size_t *offsets);
+/* Fills the slot's constant scratch and packs its network; call once when the slot is admitted. */
+int xdn_slot_init(const xdn_slot *slot, void *stream);
+/* One frame for each of n_slots (<= 16) slots, enqueued on `stream`. */
+int xdn_denoise_step(const xdn_slot *slots, int n_slots, float learning_rate, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
