@@ -14,11 +14,10 @@
 //     gradient of the network output. Epilogues: bias + ReLU -> bf16, ReLU-mask of the layer
 //     input (backward) -> bf16, or bias -> FP32 6 channels (network output).
 //   conv3x3_wgrad<X, CI, DY>  weight gradient: M = input channel, N = output channel, K = 16
-//     pixels (2 rows x 8 px). Both operands are staged channel-major in LDS; the +-1 px shifts of
-//     the kernel columns are formed in registers from aligned 8-px chunks with v_alignbit
-//     (no misaligned LDS reads, no shifted copies). A wave keeps all 9 taps' 32x32 FP32
-//     accumulators and loops over many 8x32-px strips; per-block partial sums are reduced by
-//     wgrad_reduce in two deterministic stages.
+//     pixels (2 rows x 8 px). Both operands are staged channel-major in LDS, X three times
+//     (shifted by -1, 0, +1 px) so every operand is one aligned ds_read_b128. A wave keeps all 9
+//     taps' 32x32 FP32 accumulators (AGPRs) and loops over many 8x32-px strips; per-block partial
+//     sums are reduced by wgrad_reduce in two deterministic stages.
 //   pack_weights  FP32 master weights -> bf16 MFMA B fragments for the forward and the data
 //     gradient (flipped/transposed) convolutions.
 #include "xdn_common.h"
@@ -192,35 +191,18 @@ struct WgBatch {
 };
 static_assert(sizeof(WgBatch) <= 4096, "kernel argument too large");
 
-constexpr int WS_W = 32, WS_H = 8;       // strip: 8 rows x 32 px
-constexpr int XT_COLS = WS_W + 16;       // 8-px halo chunk on each side
-constexpr int XT_CS = (WS_H + 2) * XT_COLS + 8;  // per-channel stride: 488 bf16 = 244 dwords
-constexpr int DT_CS = WS_H * WS_W + 8;           // 264 bf16 = 132 dwords
+constexpr int WS_W = 32, WS_H = 8;          // strip: 8 rows x 32 px
+constexpr int XS_CS = (WS_H + 2) * WS_W + 8;  // per channel and shift copy: 328 bf16 = 164 dwords
+constexpr int DT_CS = WS_H * WS_W + 8;        // 264 bf16 = 132 dwords
 
-// shift the 8-px chunk window by one pixel (bf16 pairs in dwords)
-__device__ __forceinline__ u32x4 shift_p1(u32x4 c, u32x4 n)
-{
-    u32x4 r;
-    r.x = __builtin_amdgcn_alignbit(c.y, c.x, 16);
-    r.y = __builtin_amdgcn_alignbit(c.z, c.y, 16);
-    r.z = __builtin_amdgcn_alignbit(c.w, c.z, 16);
-    r.w = __builtin_amdgcn_alignbit(n.x, c.w, 16);
-    return r;
-}
-__device__ __forceinline__ u32x4 shift_m1(u32x4 p, u32x4 c)
-{
-    u32x4 r;
-    r.x = __builtin_amdgcn_alignbit(c.x, p.w, 16);
-    r.y = __builtin_amdgcn_alignbit(c.y, c.x, 16);
-    r.z = __builtin_amdgcn_alignbit(c.z, c.y, 16);
-    r.w = __builtin_amdgcn_alignbit(c.w, c.z, 16);
-    return r;
-}
-
+// The X operand is staged three times, shifted by -1, 0, +1 px (copy sh holds X at x0 + c + sh - 1
+// in column c), so every MFMA operand is one aligned ds_read_b128. MFMA operands are never built
+// with VALU instructions: on gfx950 a VALU write to a VGPR that a just-issued MFMA reads as
+// SrcA/SrcB corrupted the product in this kernel (observed non-deterministic results).
 template <int XK, int CI, int DYK>
 __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t xt[32 * XT_CS];
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3 * 32 * XS_CS];
     __shared__ __attribute__((aligned(16))) uint16_t dt[32 * DT_CS];
     __shared__ float dbs[4][32];
     int f = 0;
@@ -231,6 +213,7 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
     const int sx = (W + WS_W - 1) / WS_W, sy = (H + WS_H - 1) / WS_H;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, h = lane >> 5, ch = lane & 31;
     const int rr = 2 * wv + h;  // strip row of this lane group
+    constexpr int XC = WS_W + 2;  // staged columns x0-1 .. x0+32
 
     f32x16 acc[9];
 #pragma unroll
@@ -242,52 +225,61 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
     for (int s = li; s < sx * sy; s += nblk)
     {
         const int y0 = (s / sx) * WS_H, x0 = (s % sx) * WS_W;
-        __syncthreads();  // previous strip's reads are done
-        // X tile, channel-major: xt[c][r][col], r <-> y0-1+r, col <-> x0-8+col
+        __syncthreads();  // the previous strip's reads are done
+        // X: column col <-> x0 - 1 + col, row r <-> y0 - 1 + r; scattered into the three copies
+        auto put = [&](int r, int col, int c0, const uint16_t (&e)[8], int ne) {
+#pragma unroll
+            for (int sh = 0; sh < 3; ++sh)
+            {
+                const int c = col - sh;
+                if (c < 0 || c >= WS_W) continue;
+                uint16_t *d = &xs[(sh * 32 + c0) * XS_CS + r * WS_W + c];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (j < ne) d[j * XS_CS] = e[j];
+            }
+        };
         if constexpr (XK == CONV_IN_BF16)
         {
             const uint16_t *src = static_cast<const uint16_t *>(F.x[0]);
-            for (int i = tid; i < (WS_H + 2) * XT_COLS * 4; i += 256)
+            for (int i = tid; i < (WS_H + 2) * XC * 4; i += 256)
             {
                 const int pix = i >> 2, q = i & 3;
-                const int r = pix / XT_COLS, col = pix - r * XT_COLS;
-                const int gy = y0 - 1 + r, gx = x0 - 8 + col;
+                const int r = pix / XC, col = pix - r * XC;
+                const int gy = y0 - 1 + r, gx = x0 - 1 + col;
                 u32x4 v = {0u, 0u, 0u, 0u};
                 if (gy >= 0 && gy < H && gx >= 0 && gx < W)
                     v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * 32 + q * 8);
-                uint16_t *d = &xt[(q * 8) * XT_CS + r * XT_COLS + col];
-                d[0 * XT_CS] = (uint16_t)v.x; d[1 * XT_CS] = (uint16_t)(v.x >> 16);
-                d[2 * XT_CS] = (uint16_t)v.y; d[3 * XT_CS] = (uint16_t)(v.y >> 16);
-                d[4 * XT_CS] = (uint16_t)v.z; d[5 * XT_CS] = (uint16_t)(v.z >> 16);
-                d[6 * XT_CS] = (uint16_t)v.w; d[7 * XT_CS] = (uint16_t)(v.w >> 16);
+                const uint16_t e[8] = {(uint16_t)v.x, (uint16_t)(v.x >> 16), (uint16_t)v.y, (uint16_t)(v.y >> 16),
+                                       (uint16_t)v.z, (uint16_t)(v.z >> 16), (uint16_t)v.w, (uint16_t)(v.w >> 16)};
+                put(r, col, q * 8, e, 8);
             }
         }
         else
         {
-            for (int pix = tid; pix < (WS_H + 2) * XT_COLS; pix += 256)
+            for (int i = tid; i < (WS_H + 2) * XC * 2; i += 256)
             {
-                const int r = pix / XT_COLS, col = pix - r * XT_COLS;
-                const int gy = y0 - 1 + r, gx = x0 - 8 + col;
-                float v[15];
-#pragma unroll
-                for (int k = 0; k < 15; ++k) v[k] = 0.f;
+                const int pix = i >> 1, q = i & 1;  // q: channels 0-7 or 8-15 of the 16 (15 real + 0)
+                const int r = pix / XC, col = pix - r * XC;
+                const int gy = y0 - 1 + r, gx = x0 - 1 + col;
+                float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (gy >= 0 && gy < H && gx >= 0 && gx < W)
                 {
                     const size_t p = (size_t)gy * W + gx;
 #pragma unroll
-                    for (int im = 0; im < 5; ++im)
+                    for (int j = 0; j < 8; ++j)
                     {
-                        const float *sp = static_cast<const float *>(F.x[im]) + 3 * p;
-                        v[3 * im] = sp[0]; v[3 * im + 1] = sp[1]; v[3 * im + 2] = sp[2];
+                        const int k = q * 8 + j;  // channel of the concatenated 15-channel input
+                        if (k < 15) v[j] = static_cast<const float *>(F.x[k / 3])[3 * p + k % 3];
                     }
                 }
-                uint16_t *d = &xt[r * XT_COLS + col];
+                uint16_t e[8];
 #pragma unroll
-                for (int k = 0; k < 15; ++k) d[k * XT_CS] = (uint16_t)f2bf(v[k]);
-                d[15 * XT_CS] = 0;
+                for (int j = 0; j < 8; ++j) e[j] = (uint16_t)f2bf(v[j]);
+                put(r, col, q * 8, e, 8);
             }
         }
-        // dY tile, channel-major: dt[co][r][col], r <-> y0+r, col <-> x0+col
+        // dY, channel-major: dt[co][r][col], r <-> y0 + r, col <-> x0 + col
         if constexpr (DYK == CONV_IN_BF16)
         {
             const uint16_t *src = static_cast<const uint16_t *>(F.dy);
@@ -326,34 +318,23 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
             }
         }
         __syncthreads();
-        // B fragments of the 4 chunks of this lane group's row; bias gradient on the side
-        u32x4 B[4];
+        const bool live_ci = (CI == 32) || ch < CI;
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {
-            B[c] = *reinterpret_cast<const u32x4 *>(&dt[ch * DT_CS + rr * WS_W + 8 * c]);
-            db += (bf_lo(B[c].x) + bf_hi(B[c].x)) + (bf_lo(B[c].y) + bf_hi(B[c].y)) +
-                  (bf_lo(B[c].z) + bf_hi(B[c].z)) + (bf_lo(B[c].w) + bf_hi(B[c].w));
-        }
-        const bool live_ci = (CI == 32) || ch < CI;
+            const u32x4 B = *reinterpret_cast<const u32x4 *>(&dt[ch * DT_CS + rr * WS_W + 8 * c]);
+            db += (bf_lo(B.x) + bf_hi(B.x)) + (bf_lo(B.y) + bf_hi(B.y)) + (bf_lo(B.z) + bf_hi(B.z)) +
+                  (bf_lo(B.w) + bf_hi(B.w));
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-        {
-            // 6 aligned chunks of X row rr + dy (xt row index rr + dy <-> image row y0 - 1 + rr + dy)
-            u32x4 X[6];
+            for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-            {
-                X[c] = live_ci ? *reinterpret_cast<const u32x4 *>(&xt[ch * XT_CS + (rr + dy) * XT_COLS + 8 * c])
-                               : u32x4{0u, 0u, 0u, 0u};
-            }
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-            {
-                acc[dy * 3 + 0] = mfma(shift_m1(X[c], X[c + 1]), B[c], acc[dy * 3 + 0]);
-                acc[dy * 3 + 1] = mfma(X[c + 1], B[c], acc[dy * 3 + 1]);
-                acc[dy * 3 + 2] = mfma(shift_p1(X[c + 1], X[c + 2]), B[c], acc[dy * 3 + 2]);
-            }
+                for (int sh = 0; sh < 3; ++sh)
+                {
+                    const u32x4 A = live_ci ? *reinterpret_cast<const u32x4 *>(
+                                                  &xs[(sh * 32 + ch) * XS_CS + (rr + dy) * WS_W + 8 * c])
+                                            : u32x4{0u, 0u, 0u, 0u};
+                    acc[dy * 3 + sh] = mfma(A, B, acc[dy * 3 + sh]);
+                }
         }
     }
     // partials: [t][ci][co] then [co]

@@ -143,6 +143,10 @@ class Denoiser:
         s.avg_loss = self.avg_loss.data_ptr()
         return s
 
+    def repack(self):
+        """Rebuild the packed bf16 fragments after the parameters were replaced."""
+        check(lib.xdn_cnn5_pack(ptr(self.params), ptr(self.packed), self._stream()), "cnn5 pack")
+
     def advance(self, frame):
         """Host bookkeeping after the step of `frame` was enqueued."""
         self.prev_frame = frame

@@ -1,0 +1,260 @@
+"""Multi-stream serving loop for the per-frame online denoiser.
+
+Render streams arrive (and may join mid-run) with frames at their own rates; each admitted stream
+owns a slot whose history, network, Adam state and scratch stay resident in HBM. The scheduler does
+continuous batching: whenever the GPU can take work, every stream that has an arrived frame and no
+frame in flight joins the next batch, which is ONE batched step (one launch per stage over all its
+streams, ``xdn_denoise_step``). Batches alternate between two HIP streams so a new batch overlaps the
+tail of the previous one on the device. Streams are admitted against an HBM budget; idle streams are
+evicted (their slot memory released) to admit waiting ones. Per-frame latency = completion (HIP
+event) - arrival; frames/s and p50/p99 are reported.
+
+The admission/eviction policy (``Admission``) is plain host logic, testable without a GPU.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+# ------------------------------------------------------------------------------------------------
+# Workload description
+# ------------------------------------------------------------------------------------------------
+@dataclass
+class StreamReq:
+    sid: int
+    height: int
+    width: int
+    n_frames: int
+    arrivals: list            # arrival time of each frame, seconds from the start
+    seed: int = 0
+    pan_max: float = 3.0
+    disocclusion: float = 0.1
+
+
+def poisson_arrivals(rate_hz: float, n: int, start: float = 0.0, rng=None) -> list:
+    rng = rng or np.random.default_rng(0)
+    gaps = rng.exponential(1.0 / rate_hz, size=n)
+    return list(start + np.cumsum(gaps))
+
+
+def onoff_arrivals(peak_hz: float, n: int, period_s: float = 0.2, start: float = 0.0, rng=None) -> list:
+    """Bursty arrivals: Poisson at peak_hz during 'on' periods, nothing during 'off' periods."""
+    rng = rng or np.random.default_rng(0)
+    out, t = [], start
+    while len(out) < n:
+        t += rng.exponential(1.0 / peak_hz)
+        phase = ((t - start) // period_s) % 2
+        if phase == 1:  # off: jump to the next on period
+            t = start + (math.floor((t - start) / period_s) + 1) * period_s
+            continue
+        out.append(t)
+    return out
+
+
+def config3(frames_per_stream=64, fps=60.0, seed=0) -> list:
+    """16 streams: 8x1280x720, 6x1920x1080, 2x2560x1440, Poisson at `fps`."""
+    rng = np.random.default_rng(seed)
+    sizes = [(720, 1280)] * 8 + [(1080, 1920)] * 6 + [(1440, 2560)] * 2
+    return [StreamReq(i, h, w, frames_per_stream, poisson_arrivals(fps, frames_per_stream, rng=rng), seed=seed + i)
+            for i, (h, w) in enumerate(sizes)]
+
+
+def config4(frames_per_stream=64, seed=0, join_at=0.25) -> list:
+    """4x3840x2160 + 12x1920x1080, bursty (on/off 200 ms at 120 fps), pans U[-8,8], 20% disocclusion;
+    half of the streams join mid-run."""
+    rng = np.random.default_rng(seed)
+    sizes = [(2160, 3840)] * 4 + [(1080, 1920)] * 12
+    out = []
+    for i, (h, w) in enumerate(sizes):
+        start = join_at if i % 2 else 0.0
+        out.append(StreamReq(i, h, w, frames_per_stream, onoff_arrivals(120.0, frames_per_stream, start=start, rng=rng),
+                             seed=seed + i, pan_max=8.0, disocclusion=0.2))
+    return out
+
+
+# ------------------------------------------------------------------------------------------------
+# Admission by memory budget with idle eviction (host logic)
+# ------------------------------------------------------------------------------------------------
+@dataclass
+class Admission:
+    budget: int
+    used: int = 0
+    resident: dict = field(default_factory=dict)   # sid -> bytes
+    last_active: dict = field(default_factory=dict)
+
+    def try_admit(self, sid: int, nbytes: int, now: float, busy: set, idle_after: float = 0.05):
+        """Admit `sid` if it fits, evicting idle resident streams (oldest activity first) if needed.
+
+        Returns (admitted, evicted_sids)."""
+        if sid in self.resident:
+            return True, []
+        if nbytes > self.budget:
+            return False, []
+        evicted = []
+        if self.used + nbytes > self.budget:
+            idle = sorted((t, s) for s, t in self.last_active.items()
+                          if s not in busy and now - t >= idle_after)
+            freed = 0
+            plan = []
+            for _, s in idle:
+                if self.used - freed + nbytes <= self.budget:
+                    break
+                plan.append(s)
+                freed += self.resident[s]
+            if self.used - freed + nbytes > self.budget:
+                return False, []
+            for s in plan:
+                self.release(s)
+                evicted.append(s)
+        self.resident[sid] = nbytes
+        self.used += nbytes
+        self.last_active[sid] = now
+        return True, evicted
+
+    def touch(self, sid: int, now: float):
+        self.last_active[sid] = now
+
+    def release(self, sid: int):
+        if sid in self.resident:
+            self.used -= self.resident.pop(sid)
+            self.last_active.pop(sid, None)
+
+
+# ------------------------------------------------------------------------------------------------
+# GPU serving loop
+# ------------------------------------------------------------------------------------------------
+class Server:
+    def __init__(self, device="cuda", hbm_budget=64 << 30, max_batch=16, lr=1e-3, pool_frames=4, batching=True,
+                 idle_after=0.05):
+        import torch
+        self.torch = torch
+        self.device = torch.device(device)
+        self.adm = Admission(hbm_budget)
+        self.max_batch = max_batch if batching else 1
+        self.lr = lr
+        self.pool_frames = pool_frames
+        self.idle_after = idle_after
+        self.hip_streams = [torch.cuda.Stream(self.device), torch.cuda.Stream(self.device)]
+        self.denoisers = {}
+        self.pools = {}
+        self.saved = {}    # evicted streams: (params, adam m, adam v, step) on the host
+
+    def _frame_pool(self, req: StreamReq):
+        """A few distinct frames of the stream, resident in HBM (the renderer's output buffers)."""
+        from . import synth
+        spec = synth.StreamSpec(req.height, req.width, seed=req.seed, stream=req.sid, pan=None, pan_max=req.pan_max,
+                                disocclusion=req.disocclusion)
+        return [synth.make_frame(spec, t, device=self.device) for t in range(self.pool_frames)]
+
+    def run(self, reqs: list, log=None) -> dict:
+        from . import pipeline
+        torch = self.torch
+        rejected = [r for r in reqs if pipeline.slot_bytes(r.height, r.width) > self.adm.budget]
+        reqs = [r for r in reqs if r not in rejected]
+        for r in reqs:
+            self.pools[r.sid] = self._frame_pool(r)
+        torch.cuda.synchronize()
+        events = []   # (arrival, sid, frame index)
+        for r in reqs:
+            for k, t in enumerate(r.arrivals):
+                events.append((t, r.sid, k))
+        events.sort()
+        by_sid = {r.sid: r for r in reqs}
+        pending = {r.sid: [] for r in reqs}   # arrived, not started: list of (arrival, k)
+        next_ev = 0
+        inflight = []     # (event, [(sid, k, arrival)])
+        busy = set()
+        latencies, done_frames, evictions, admit_wait = [], 0, 0, 0
+        total = len(events)
+        t0 = time.perf_counter()
+        bi = 0
+        while done_frames < total:
+            now = time.perf_counter() - t0
+            while next_ev < total and events[next_ev][0] <= now:
+                a, sid, k = events[next_ev]
+                pending[sid].append((a, k))
+                next_ev += 1
+            # completions
+            still = []
+            for ev, items in inflight:
+                if ev.query():
+                    tdone = time.perf_counter() - t0
+                    for sid, k, a in items:
+                        latencies.append(tdone - a)
+                        busy.discard(sid)
+                        self.adm.touch(sid, tdone)
+                    done_frames += len(items)
+                else:
+                    still.append((ev, items))
+            inflight = still
+            # admission of streams with pending frames
+            ready = []
+            for sid in sorted(pending, key=lambda s: pending[s][0][0] if pending[s] else 1e30):
+                if not pending[sid] or sid in busy:
+                    continue
+                if sid not in self.denoisers:
+                    r = by_sid[sid]
+                    ok, ev_s = self.adm.try_admit(sid, pipeline.slot_bytes(r.height, r.width), now, busy,
+                                                  self.idle_after)
+                    for s in ev_s:
+                        self._evict(s)
+                        evictions += 1
+                    if not ok:
+                        admit_wait += 1
+                        continue
+                    self.denoisers[sid] = self._admit(r)
+                ready.append(sid)
+                if len(ready) >= self.max_batch:
+                    break
+            if ready and len(inflight) < 2:
+                st = self.hip_streams[bi % 2]
+                bi += 1
+                items, dens, frs = [], [], []
+                for sid in ready:
+                    a, k = pending[sid].pop(0)
+                    d = self.denoisers[sid]
+                    dens.append(d)
+                    frs.append(self.pools[sid][k % self.pool_frames])
+                    items.append((sid, k, a))
+                    busy.add(sid)
+                # no cross-batch wait: a stream joins a batch only after the host has seen its
+                # previous frame complete, and streams share no state
+                with torch.cuda.stream(st):
+                    pipeline.denoise_step_multi(dens, frs, stream=st.cuda_stream)
+                    ev = torch.cuda.Event()
+                    ev.record(st)
+                inflight.append((ev, items))
+            elif not inflight and next_ev < total:
+                # idle until the next arrival
+                wait = events[next_ev][0] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.001))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = np.array(latencies) * 1e3
+        return {"frames": done_frames, "wall_s": wall, "fps": done_frames / wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "max_ms": float(lat.max()), "evictions": evictions, "admission_waits": admit_wait,
+                "batches": bi, "mean_batch": done_frames / max(bi, 1), "rejected_streams": [r.sid for r in rejected]}
+
+    def _admit(self, r: StreamReq):
+        """A slot for stream r; an evicted stream gets its network and Adam state back (its frame
+        history restarts from zero, as at frame 0)."""
+        from . import pipeline
+        d = pipeline.Denoiser(r.height, r.width, device=self.device, seed=1234 + r.sid, lr=self.lr)
+        saved = self.saved.pop(r.sid, None)
+        if saved is not None:
+            for t, src in zip((d.params, d.adam_m, d.adam_v), saved[:3]):
+                t.copy_(src)
+            d.t = saved[3]
+            d.repack()
+        return d
+
+    def _evict(self, sid: int):
+        d = self.denoisers.pop(sid, None)
+        if d is not None:
+            self.saved[sid] = (d.params.cpu(), d.adam_m.cpu(), d.adam_v.cpu(), d.t)

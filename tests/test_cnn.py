@@ -95,7 +95,7 @@ def test_cnn5_adam_and_pack_roundtrip(dev):
     assert not torch.equal(p0, net.params)
     # first Keras Adam step moves every parameter with a non-zero gradient by ~lr
     moved = (net.params - p0).abs()
-    nz = net.grad.abs() > 1e-4
+    nz = net.grad.abs() > 1e-3  # eps 1e-7 against sqrt(v)=0.0316|g| stays below 0.3%
     assert torch.allclose(moved[nz], torch.full_like(moved[nz], 1e-3), rtol=1e-2, atol=1e-6)
     out = net.forward(*imgs)
     ref_out, _ = ref_forward([t.cpu() for t in imgs], [(w.cpu(), b.cpu()) for w, b in net.layers()])
