@@ -128,6 +128,97 @@ int xdn_cross_regression_from_color(const float *color, const float *color2, con
                                  stream);
 }
 
+namespace {
+// Workspace of the cross-regression backward, per image (every buffer 256-B aligned).
+struct CrgLayout {
+    size_t var1, var2, beta, gq, u, gcen, evar, bytes;
+    CrgLayout(int H, int W, int s)
+    {
+        const size_t N = px(H, W), C = (size_t)(H / s) * (W / s);
+        size_t o = 0;
+        var1 = o; o += al256(N * 4);
+        var2 = o; o += al256(N * 4);
+        beta = o; o += al256(C * 36 * 4);
+        gq = o; o += al256(N * 2 * 16);
+        u = o; o += al256(C * 36 * 4);
+        gcen = o; o += al256(C * 24 * 4);
+        evar = o; o += al256(N * 6 * 4);
+        bytes = o;
+    }
+};
+}  // namespace
+
+size_t xdn_cross_regression_grad_workspace_size(int batch, int height, int width, int sparsity)
+{
+    if (batch <= 0 || height <= 0 || width <= 0 || sparsity <= 0) return 0;
+    return (size_t)batch * CrgLayout(height, width, sparsity).bytes;
+}
+
+int xdn_cross_regression_grad(const float *imga, const float *imgb, const float *albedo, const float *normal,
+                              const float *grad_outa, const float *grad_outb, const float *grad_vara,
+                              const float *grad_varb, float *grad_imga, float *grad_imgb, float *grad_albedo,
+                              float *grad_normal, void *workspace, size_t workspace_bytes, int B, int H, int W,
+                              int win, int s, void *stream)
+{
+    if (int e = check_shape(B, H, W)) return e;
+    if (win < 1 || s < 1) return fail(XDN_EINVAL, "win_size and sparsity must be >= 1");
+    if (H % s != 0 || W % s != 0) return fail(XDN_EINVAL, "Image height and width must be divisible by sparsity");
+    if (H <= win / 2 || W <= win / 2) return fail(XDN_EINVAL, "Image height and width must exceed win_size/2");
+    if (!imga || !imgb || !albedo || !normal || !grad_outa || !grad_outb || !grad_imga || !grad_imgb ||
+        !grad_albedo || !grad_normal)
+        return fail(XDN_EINVAL, "cross_regression_grad: null tensor");
+    const size_t need = xdn_cross_regression_grad_workspace_size(B, H, W, s);
+    if (!workspace || workspace_bytes < need)
+        return fail(XDN_EINVAL, "workspace too small: %zu < %zu", workspace_bytes, need);
+    const CrgLayout lay(H, W, s);
+    const size_t N = px(H, W);
+    for (int b0 = 0; b0 < B; b0 += XDN_MAX_FRAMES / 2)
+    {
+        const int n = B - b0 < XDN_MAX_FRAMES / 2 ? B - b0 : XDN_MAX_FRAMES / 2;
+        std::vector<xdn::CrFrame> fw(n);
+        std::vector<xdn::CrgFrame> fr(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const size_t b = (size_t)(b0 + i);
+            char *w = static_cast<char *>(workspace) + b * lay.bytes;
+            xdn::CrFrame &f = fw[i];
+            f = xdn::CrFrame{};
+            f.y1 = const_cast<float *>(imga + b * N * 3);
+            f.y2 = const_cast<float *>(imgb + b * N * 3);
+            f.var1 = reinterpret_cast<float *>(w + lay.var1);
+            f.var2 = reinterpret_cast<float *>(w + lay.var2);
+            f.alb = albedo + b * N * 3;
+            f.nrm = normal + b * N * 3;
+            f.beta = reinterpret_cast<float *>(w + lay.beta);
+            f.H = H;
+            f.W = W;
+            xdn::CrgFrame &g = fr[i];
+            g.y1 = f.y1; g.y2 = f.y2; g.alb = f.alb; g.nrm = f.nrm;
+            g.var1 = f.var1; g.var2 = f.var2; g.beta = f.beta;
+            g.gouta = grad_outa + b * N * 3;
+            g.goutb = grad_outb + b * N * 3;
+            g.gvara = grad_vara ? grad_vara + b * N : nullptr;
+            g.gvarb = grad_varb ? grad_varb + b * N : nullptr;
+            g.gq = reinterpret_cast<float4 *>(w + lay.gq);
+            g.u = reinterpret_cast<float *>(w + lay.u);
+            g.gcen = reinterpret_cast<float *>(w + lay.gcen);
+            g.evar = reinterpret_cast<float *>(w + lay.evar);
+            g.gya = grad_imga + b * N * 3;
+            g.gyb = grad_imgb + b * N * 3;
+            g.galb = grad_albedo + b * N * 3;
+            g.gnrm = grad_normal + b * N * 3;
+            g.H = H;
+            g.W = W;
+        }
+        if (int e = hip_check(xdn::launch_variance(n, fw.data(), 3, false, S(stream)), "variance")) return e;
+        if (int e = hip_check(xdn::launch_regression(n, fw.data(), win, s, S(stream)), "regression")) return e;
+        if (int e = hip_check(xdn::launch_cross_regression_grad(n, fr.data(), win, s, S(stream)),
+                              "cross_regression_grad"))
+            return e;
+    }
+    return XDN_OK;
+}
+
 static int sf_common_check(int B, int H, int W, int win)
 {
     if (int e = check_shape(B, H, W)) return e;

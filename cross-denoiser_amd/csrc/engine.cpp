@@ -23,7 +23,7 @@ constexpr int kRegWin = 17, kSparsity = 4, kFilterWin = 11;  // scripts/main.py:
 // Scratch layout of one slot (all offsets 256-byte aligned).
 struct SlotScratch {
     float *y1, *y2, *var1, *var2, *beta, *rp[3], *prev_out, *prev_ols1, *prev_ols2, *success_prev, *succ_e;
-    float *dparam, *param, *ones, *loss_parts;
+    float *dparam, *param, *ones, *loss_parts, *adam_alpha;
     uint8_t *success;
     void *cnn_ws;
     size_t cnn_ws_bytes, beta_bytes, bytes;
@@ -50,6 +50,7 @@ struct SlotScratch {
         loss_parts = (float *)take((size_t)xdn::filter_train_tiles(H, W) * 4);
         cnn_ws_bytes = Cnn5Ws(nullptr, 1, H, W).bytes;
         cnn_ws = take(cnn_ws_bytes);
+        adam_alpha = (float *)take(4);
         bytes = o;
     }
 };
@@ -85,7 +86,7 @@ int xdn_slot_scratch_offsets(int height, int width, size_t *offsets)
     SlotScratch sc(base, height, width);
     const void *p[XDN_SLOT_NBUF] = {sc.y1, sc.y2, sc.var1, sc.var2, sc.beta, sc.success, sc.rp[0], sc.rp[1],
                                     sc.rp[2], sc.prev_out, sc.prev_ols1, sc.prev_ols2, sc.success_prev, sc.succ_e,
-                                    sc.dparam, sc.param, sc.ones, sc.loss_parts, sc.cnn_ws};
+                                    sc.dparam, sc.param, sc.ones, sc.loss_parts, sc.cnn_ws, sc.adam_alpha};
     for (int i = 0; i < XDN_SLOT_NBUF; ++i) offsets[i] = (size_t)(static_cast<const char *>(p[i]) - base);
     return XDN_OK;
 }
@@ -270,6 +271,8 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
     {
         std::vector<xdn::AdamJob> jobs(n);
         std::vector<xdn::PackJob> pj(5 * n);
+        std::vector<int *> counters;
+        std::vector<float *> alphas;
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-7f;
         for (int i = 0; i < n; ++i)
         {
@@ -278,6 +281,12 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
             jobs[i].param = s.params; jobs[i].grad = s.grad; jobs[i].m = s.adam_m; jobs[i].v = s.adam_v;
             jobs[i].count = (int)kLay.count;
             jobs[i].alpha = (float)((double)lr * sqrt(1.0 - pow((double)b2, t)) / (1.0 - pow((double)b1, t)));
+            jobs[i].alpha_dev = s.adam_t ? sc[i].adam_alpha : nullptr;
+            if (s.adam_t)
+            {
+                counters.push_back(s.adam_t);
+                alphas.push_back(sc[i].adam_alpha);
+            }
             char *pk = static_cast<char *>(s.packed);
             for (int l = 0; l < 5; ++l)
             {
@@ -288,10 +297,53 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
                 j.ci = kCnn5[l].ci; j.co = kCnn5[l].co;
             }
         }
+        if (!counters.empty())
+            if (int e = hip_check(xdn::launch_adam_tick((int)counters.size(), counters.data(), alphas.data(), lr, b1,
+                                                        b2, st),
+                                  "adam step counter"))
+                return e;
         if (int e = hip_check(xdn::launch_adam(n, jobs.data(), b1, b2, eps, st), "adam")) return e;
         if (int e = hip_check(xdn::launch_pack(5 * n, pj.data(), st), "pack")) return e;
     }
     return XDN_OK;
+}
+
+int xdn_denoise_graph_create(const xdn_slot *slots, int n_slots, float lr, void *stream, void **graph)
+{
+    if (!graph) return fail(XDN_EINVAL, "graph output pointer is null");
+    *graph = nullptr;
+    if (!stream) return fail(XDN_EINVAL, "graph capture needs a non-null stream");
+    for (int i = 0; i < n_slots; ++i)
+        if (!slots[i].adam_t) return fail(XDN_EINVAL, "graph capture needs adam_t (device step counter) on every slot");
+    hipStream_t st = S(stream);
+    if (int e = hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture")) return e;
+    const int rc = xdn_denoise_step(slots, n_slots, lr, stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (rc != XDN_OK)
+    {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (int e = hip_check(ec, "end capture")) return e;
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (int e = hip_check(ei, "graph instantiate")) return e;
+    *graph = ex;
+    return XDN_OK;
+}
+
+int xdn_denoise_graph_launch(void *graph, void *stream)
+{
+    if (!graph) return fail(XDN_EINVAL, "null graph");
+    return hip_check(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), S(stream)), "graph launch");
+}
+
+int xdn_denoise_graph_destroy(void *graph)
+{
+    if (!graph) return XDN_OK;
+    return hip_check(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)), "graph destroy");
 }
 
 }  // extern "C"

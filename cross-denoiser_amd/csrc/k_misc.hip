@@ -17,6 +17,7 @@ struct AdamBatch {
 __global__ __launch_bounds__(256) void adam_kernel(AdamBatch b)
 {
     const AdamJob &J = b.j[blockIdx.y];
+    const float alpha = J.alpha_dev ? *J.alpha_dev : J.alpha;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < J.count; i += gridDim.x * 256)
     {
         const float g = J.grad[i];
@@ -25,8 +26,43 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamBatch b)
         v += (g * g - v) * (1.f - b.b2);
         J.m[i] = m;
         J.v[i] = v;
-        J.param[i] -= (m * J.alpha) / (sqrtf(v) + b.eps);
+        J.param[i] -= (m * alpha) / (sqrtf(v) + b.eps);
     }
+}
+
+struct TickBatch {
+    int n;
+    int *counter[XDN_MAX_FRAMES];
+    float *alpha[XDN_MAX_FRAMES];
+    float lr, b1, b2;
+};
+
+__global__ void adam_tick_kernel(TickBatch b)
+{
+    const int i = threadIdx.x;
+    if (i >= b.n) return;
+    const int t = *b.counter[i] + 1;
+    *b.counter[i] = t;
+    *b.alpha[i] = (float)((double)b.lr * sqrt(1.0 - pow((double)b.b2, (double)t)) / (1.0 - pow((double)b.b1, (double)t)));
+}
+
+hipError_t launch_adam_tick(int n, int *const *counter, float *const *alpha, float lr, float b1, float b2,
+                            hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    TickBatch b{};
+    b.n = n;
+    for (int i = 0; i < n; ++i)
+    {
+        b.counter[i] = counter[i];
+        b.alpha[i] = alpha[i];
+    }
+    b.lr = lr;
+    b.b1 = b1;
+    b.b2 = b2;
+    hipLaunchKernelGGL(adam_tick_kernel, dim3(1), dim3(64), 0, st, b);
+    return hipGetLastError();
 }
 
 hipError_t launch_adam(int n, const AdamJob *jobs, float b1, float b2, float eps, hipStream_t st)

@@ -26,6 +26,22 @@ hipError_t launch_variance(int n, const CrFrame *fr, int which, bool log_in, hip
 hipError_t launch_regression(int n, const CrFrame *fr, int win, int s, hipStream_t st);
 hipError_t launch_recon(int n, const CrFrame *fr, int win, int s, int rs, hipStream_t st);
 
+// Backward of cross_regression with respect to its inputs (recon_scale 1). var1/var2/beta hold
+// the recomputed forward quantities; gq, u, gcen, evar are scratch.
+struct CrgFrame {
+    const float *y1, *y2, *alb, *nrm;
+    const float *var1, *var2, *beta;     // [H][W], [H][W], [Hs][Ws][2][18]
+    const float *gouta, *goutb;          // [H][W][3]
+    const float *gvara, *gvarb;          // [H][W] or null
+    float4 *gq;                          // [H][W][2]
+    float *u;                            // [Hs][Ws][2][18]
+    float *gcen;                         // [Hs][Ws][2][12]
+    float *evar;                         // [H][W][6]
+    float *gya, *gyb, *galb, *gnrm;      // [H][W][3]
+    int H, W;
+};
+hipError_t launch_cross_regression_grad(int n, const CrgFrame *fr, int win, int s, hipStream_t st);
+
 // Spatial filter family.
 struct SfFrame {
     const float *ia, *ib, *alb, *nrm, *band;          // band: [H][W][5]

@@ -34,6 +34,8 @@ PROTOTYPES = {
     "xdn_cross_regression_workspace_size": (_Z, [_I, _I, _I, _I]),
     "xdn_cross_regression": (_I, [_P] * 8 + [_P, _Z] + [_I] * 6 + [_P]),
     "xdn_cross_regression_from_color": (_I, [_P] * 10 + [_P, _Z] + [_I] * 5 + [_P]),
+    "xdn_cross_regression_grad_workspace_size": (_Z, [_I, _I, _I, _I]),
+    "xdn_cross_regression_grad": (_I, [_P] * 12 + [_P, _Z] + [_I] * 5 + [_P]),
     "xdn_spatial_filter": (_I, [_P] * 9 + [_I] * 4 + [_P]),
     "xdn_spatial_filter_grad": (_I, [_P] * 10 + [_I] * 4 + [_P]),
     "xdn_spatial_filter_grad_inputs": (_I, [_P] * 13 + [_I] * 4 + [_P]),
@@ -53,6 +55,9 @@ PROTOTYPES = {
     "xdn_slot_scratch_offsets": (_I, [_I, _I, _P]),
     "xdn_slot_init": (_I, [_P, _P]),
     "xdn_denoise_step": (_I, [_P, _I, _F, _P]),
+    "xdn_denoise_graph_create": (_I, [_P, _I, _F, _P, _PP]),
+    "xdn_denoise_graph_launch": (_I, [_P, _P]),
+    "xdn_denoise_graph_destroy": (_I, [_P]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():

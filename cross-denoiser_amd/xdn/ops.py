@@ -57,17 +57,8 @@ def _same_hw(ref: torch.Tensor, *ts) -> None:
 
 
 # ---------------------------------------------------------------------------------------------
-def cross_regression(imga, imgb, albedo, normal, win_size, sparsity, recon_scale=1, name=None):
-    """Cross-regression of the two half-buffers (CrossRegressionOp, ops/regression.cc:125-196)."""
-    imga, imgb = _img(imga, "imga", 3), _img(imgb, "imgb", 3)
-    albedo, normal = _img(albedo, "albedo", 3), _img(normal, "normal", 3)
-    _same_hw(imga, ("imgb", imgb), ("albedo", albedo), ("normal", normal))
+def _cr_forward(imga, imgb, albedo, normal, win_size, sparsity, rs):
     B, H, W, C = imga.shape
-    if recon_scale > sparsity:
-        raise ValueError("recon_scale must be less than or equal to sparsity")
-    if H % recon_scale or W % recon_scale:
-        raise ValueError("Image height and width must be divisible by recon_scale")
-    rs = int(recon_scale)
     outa = torch.empty((B, H // rs, W // rs, C), device=imga.device, dtype=torch.float32)
     outb = torch.empty_like(outa)
     vara = torch.empty((B, H, W, 1), device=imga.device, dtype=torch.float32)
@@ -78,6 +69,65 @@ def cross_regression(imga, imgb, albedo, normal, win_size, sparsity, recon_scale
                                    ptr(vara), ptr(varb), ptr(ws), ws_bytes, B, H, W, int(win_size),
                                    int(sparsity), rs, stream_of(imga)), "cross_regression")
     return outa, outb, vara, varb
+
+
+class _CrossRegressionFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, imga, imgb, albedo, normal, win_size, sparsity, rs):
+        outs = _cr_forward(imga, imgb, albedo, normal, win_size, sparsity, rs)
+        ctx.save_for_backward(imga, imgb, albedo, normal)
+        ctx.args = (win_size, sparsity, rs)
+        return outs
+
+    @staticmethod
+    def backward(ctx, g_outa, g_outb, g_vara, g_varb):
+        imga, imgb, albedo, normal = ctx.saved_tensors
+        win_size, sparsity, rs = ctx.args
+        if rs != 1:
+            raise NotImplementedError("cross_regression backward supports recon_scale 1 only")
+        g = cross_regression_grad(imga, imgb, albedo, normal, g_outa, g_outb, g_vara, g_varb, win_size=win_size,
+                                  sparsity=sparsity)
+        return g[0], g[1], g[2], g[3], None, None, None
+
+
+def cross_regression(imga, imgb, albedo, normal, win_size, sparsity, recon_scale=1, name=None):
+    """Cross-regression of the two half-buffers (CrossRegressionOp, ops/regression.cc:125-196).
+
+    Differentiable with respect to the four images through torch.autograd (recon_scale 1)."""
+    imga, imgb = _img(imga, "imga", 3), _img(imgb, "imgb", 3)
+    albedo, normal = _img(albedo, "albedo", 3), _img(normal, "normal", 3)
+    _same_hw(imga, ("imgb", imgb), ("albedo", albedo), ("normal", normal))
+    B, H, W, C = imga.shape
+    if recon_scale > sparsity:
+        raise ValueError("recon_scale must be less than or equal to sparsity")
+    if H % recon_scale or W % recon_scale:
+        raise ValueError("Image height and width must be divisible by recon_scale")
+    return _CrossRegressionFn.apply(imga, imgb, albedo, normal, int(win_size), int(sparsity), int(recon_scale))
+
+
+def cross_regression_grad(imga, imgb, albedo, normal, grad_outa, grad_outb, grad_vara=None, grad_varb=None, *,
+                          win_size, sparsity):
+    """(d imga, d imgb, d albedo, d normal) of cross_regression (recon_scale 1), window weights
+    recomputed. ``grad_vara``/``grad_varb`` may be None (zero)."""
+    imga, imgb = _img(imga, "imga", 3), _img(imgb, "imgb", 3)
+    albedo, normal = _img(albedo, "albedo", 3), _img(normal, "normal", 3)
+    B, H, W, _ = imga.shape
+
+    def g(t, name, c):
+        if t is None:
+            return None if c == 1 else torch.zeros((B, H, W, c), device=imga.device, dtype=torch.float32)
+        return _img(t, name, c)
+
+    ga, gb = g(grad_outa, "grad_outa", 3), g(grad_outb, "grad_outb", 3)
+    gva, gvb = g(grad_vara, "grad_vara", 1), g(grad_varb, "grad_varb", 1)
+    _same_hw(imga, ("imgb", imgb), ("albedo", albedo), ("normal", normal), ("grad_outa", ga), ("grad_outb", gb))
+    outs = [torch.empty_like(imga) for _ in range(4)]
+    ws_bytes = lib.xdn_cross_regression_grad_workspace_size(B, H, W, int(sparsity))
+    ws = torch.empty(max(ws_bytes, 4) // 4, device=imga.device, dtype=torch.float32)
+    check(lib.xdn_cross_regression_grad(ptr(imga), ptr(imgb), ptr(albedo), ptr(normal), ptr(ga), ptr(gb), ptr(gva),
+                                        ptr(gvb), *[ptr(o) for o in outs], ptr(ws), ws_bytes, B, H, W, int(win_size),
+                                        int(sparsity), stream_of(imga)), "cross_regression_grad")
+    return tuple(outs)
 
 
 def cross_regression_from_color(color, color2, albedo, normal, win_size, sparsity):

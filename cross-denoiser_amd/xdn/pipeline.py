@@ -30,7 +30,7 @@ LEARNING_RATE = 1e-3
 
 SLOT_BUFFERS = ("y1", "y2", "var1", "var2", "beta", "success", "rp_out", "rp_ols1", "rp_ols2", "prev_out",
                 "prev_ols1", "prev_ols2", "success_prev", "succ_e", "dparam", "param", "ones", "loss_parts",
-                "cnn_ws")
+                "cnn_ws", "adam_alpha")
 
 
 class XdnSlot(ctypes.Structure):
@@ -40,7 +40,7 @@ class XdnSlot(ctypes.Structure):
                     "color", "color2", "albedo", "normal", "mvec", "linear_z", "pn_fwidth", "opacity",
                     "prev_out_log", "prev_ols1", "prev_ols2", "prev_linear_z", "prev_normal",
                     "out_log", "ols1", "ols2", "out", "scratch", "params", "grad", "adam_m", "adam_v", "packed")] +
-                [("adam_step", ctypes.c_int), ("avg_loss", ctypes.c_void_p)])
+                [("adam_step", ctypes.c_int), ("avg_loss", ctypes.c_void_p), ("adam_t", ctypes.c_void_p)])
 
 
 def filter_train(ols1, ols2, albedo, normal, param, success, success_prev, prev_out, opacity, prev_ols1, prev_ols2,
@@ -75,7 +75,8 @@ def slot_bytes(H: int, W: int) -> int:
 class Denoiser:
     """One render stream: resident history, bandwidth network (cnn5) and its Adam state."""
 
-    def __init__(self, H: int, W: int, device="cuda", seed: int = 1234, lr: float = LEARNING_RATE):
+    def __init__(self, H: int, W: int, device="cuda", seed: int = 1234, lr: float = LEARNING_RATE,
+                 use_graphs: bool = False):
         if H % SPARSITY or W % SPARSITY or H <= REGRESSION_WIDTH // 2 or W <= REGRESSION_WIDTH // 2:
             raise ValueError(f"stream size {H}x{W}: height and width must be divisible by {SPARSITY} and > 8")
         self.device = torch.device(device)
@@ -93,6 +94,10 @@ class Denoiser:
         self.adam_v = torch.zeros_like(self.params)
         self.packed = torch.empty(int(lib.xdn_cnn5_packed_bytes()), device=dev, dtype=torch.uint8)
         self.avg_loss = torch.zeros(1, device=dev, dtype=torch.float32)
+        self.adam_t = torch.zeros(1, device=dev, dtype=torch.int32)  # Adam updates done (device counter)
+        self.use_graphs = use_graphs
+        self._graphs = {}
+        self._cap_stream = None
         self.zeros3 = e3()
         self.prev_frame = None  # G-buffers of the previous frame (linearZ, normal)
         self.t = 0
@@ -141,11 +146,17 @@ class Denoiser:
         s.packed = self.packed.data_ptr()
         s.adam_step = self.t + 1
         s.avg_loss = self.avg_loss.data_ptr()
+        s.adam_t = self.adam_t.data_ptr()
         return s
 
     def repack(self):
         """Rebuild the packed bf16 fragments after the parameters were replaced."""
         check(lib.xdn_cnn5_pack(ptr(self.params), ptr(self.packed), self._stream()), "cnn5 pack")
+
+    def set_step(self, t: int):
+        """Resume at update t (after the parameters and Adam moments were restored)."""
+        self.t = int(t)
+        self.adam_t.fill_(self.t)
 
     def advance(self, frame):
         """Host bookkeeping after the step of `frame` was enqueued."""
@@ -162,10 +173,57 @@ class Denoiser:
         The frame's linearZ and normal are kept (not copied) as the next frame's history: do not
         modify them in place before the next step has run.
         """
-        slot = self.make_slot(frame)
-        check(lib.xdn_denoise_step(ctypes.addressof(slot), 1, self.lr, self._stream()), "denoise_step")
+        if self.use_graphs:
+            self._graph_for(frame).launch(self._stream())
+        else:
+            slot = self.make_slot(frame)
+            check(lib.xdn_denoise_step(ctypes.addressof(slot), 1, self.lr, self._stream()), "denoise_step")
         self.advance(frame)
         return self.outputs()
+
+    def _graph_for(self, frame) -> "StepGraph":
+        """The captured step for this frame's buffers and history parity (captured on first use)."""
+        names = ("color", "color2", "albedo", "normal", "mvec", "linearZ", "pnFwidth", "opacity")
+        pf = self.prev_frame
+        key = (self.t % 2, tuple(frame[k].data_ptr() for k in names),
+               None if pf is None else (pf["linearZ"].data_ptr(), pf["normal"].data_ptr()))
+        g = self._graphs.get(key)
+        if g is None:
+            if self._cap_stream is None:
+                self._cap_stream = torch.cuda.Stream(self.device)
+            g = StepGraph([self.make_slot(frame)], self.lr, self._cap_stream.cuda_stream)
+            self._graphs[key] = g
+        return g
+
+    def close(self):
+        for g in self._graphs.values():
+            g.destroy()
+        self._graphs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class StepGraph:
+    """xdn_denoise_step of fixed slots captured into one HIP graph (xdn_denoise_graph_create)."""
+
+    def __init__(self, slots, lr: float, capture_stream: int):
+        arr = (XdnSlot * len(slots))(*slots)
+        h = ctypes.c_void_p()
+        check(lib.xdn_denoise_graph_create(ctypes.addressof(arr), len(slots), lr, capture_stream, ctypes.byref(h)),
+              "graph capture")
+        self.handle = h
+
+    def launch(self, stream: int):
+        check(lib.xdn_denoise_graph_launch(self.handle, stream), "graph launch")
+
+    def destroy(self):
+        if self.handle:
+            lib.xdn_denoise_graph_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
 
 
 def denoise_step_multi(denoisers, frames, stream=None):

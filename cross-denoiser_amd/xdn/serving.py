@@ -129,12 +129,16 @@ class Admission:
 # ------------------------------------------------------------------------------------------------
 class Server:
     def __init__(self, device="cuda", hbm_budget=64 << 30, max_batch=16, lr=1e-3, pool_frames=4, batching=True,
-                 idle_after=0.05):
+                 idle_after=0.05, graphs=False):
+        """batching: same-stage kernels of all ready streams in one launch (xdn_denoise_step over
+        many slots); otherwise one frame at a time. graphs (with batching off): each frame is the
+        stream's captured step graph (one per frame buffers and history parity)."""
         import torch
         self.torch = torch
         self.device = torch.device(device)
         self.adm = Admission(hbm_budget)
         self.max_batch = max_batch if batching else 1
+        self.graphs = graphs and not batching
         self.lr = lr
         self.pool_frames = pool_frames
         self.idle_after = idle_after
@@ -224,7 +228,12 @@ class Server:
                 # no cross-batch wait: a stream joins a batch only after the host has seen its
                 # previous frame complete, and streams share no state
                 with torch.cuda.stream(st):
-                    pipeline.denoise_step_multi(dens, frs, stream=st.cuda_stream)
+                    if self.graphs:
+                        for d, f in zip(dens, frs):
+                            d._graph_for(f).launch(st.cuda_stream)
+                            d.advance(f)
+                    else:
+                        pipeline.denoise_step_multi(dens, frs, stream=st.cuda_stream)
                     ev = torch.cuda.Event()
                     ev.record(st)
                 inflight.append((ev, items))
@@ -245,12 +254,13 @@ class Server:
         """A slot for stream r; an evicted stream gets its network and Adam state back (its frame
         history restarts from zero, as at frame 0)."""
         from . import pipeline
-        d = pipeline.Denoiser(r.height, r.width, device=self.device, seed=1234 + r.sid, lr=self.lr)
+        d = pipeline.Denoiser(r.height, r.width, device=self.device, seed=1234 + r.sid, lr=self.lr,
+                              use_graphs=self.graphs)
         saved = self.saved.pop(r.sid, None)
         if saved is not None:
             for t, src in zip((d.params, d.adam_m, d.adam_v), saved[:3]):
                 t.copy_(src)
-            d.t = saved[3]
+            d.set_step(saved[3])
             d.repack()
         return d
 
@@ -258,3 +268,4 @@ class Server:
         d = self.denoisers.pop(sid, None)
         if d is not None:
             self.saved[sid] = (d.params.cpu(), d.adam_m.cpu(), d.adam_v.cpu(), d.t)
+            d.close()

@@ -59,6 +59,21 @@ int xdn_cross_regression_from_color(const float *color, const float *color2, con
                                     float *out_vara, float *out_varb, void *workspace, size_t workspace_bytes,
                                     int batch, int height, int width, int win_size, int sparsity, void *stream);
 
+/* Backward of the CrossRegression op with respect to imga, imgb, albedo and normal, for
+ * recon_scale 1. The reference registers no gradient for this op (ops/regression.cc:25-57 and
+ * scripts/ops.py:23-32 register one only for SpatialFilter); this is the adjoint of
+ * CrossRegressionFunc (ops/regression.cu.cc:244-500) for torch.autograd. Every window weight is
+ * recomputed; no forward state is passed in.
+ *   grad_outa, grad_outb [B,H,W,3];  grad_vara, grad_varb [B,H,W,1] (either may be NULL = zero)
+ *   grad_imga, grad_imgb, grad_albedo, grad_normal [B,H,W,3] (overwritten)
+ * The workspace is caller-owned, xdn_cross_regression_grad_workspace_size() bytes. */
+size_t xdn_cross_regression_grad_workspace_size(int batch, int height, int width, int sparsity);
+int xdn_cross_regression_grad(const float *imga, const float *imgb, const float *albedo, const float *normal,
+                              const float *grad_outa, const float *grad_outb, const float *grad_vara,
+                              const float *grad_varb, float *grad_imga, float *grad_imgb, float *grad_albedo,
+                              float *grad_normal, void *workspace, size_t workspace_bytes, int batch, int height,
+                              int width, int win_size, int sparsity, void *stream);
+
 /* ---------------------------------------------------------------------------------------------
  * SpatialFilter op: replaces REGISTER_OP("SpatialFilter") / SpatialFilterOp::Compute
  * (ops/regression.cc:59-86, 235-282) and SpatialFilterFunc (ops/regression.cu.cc:502-517).
@@ -194,8 +209,12 @@ typedef struct xdn_slot {
     /* network state: params, grad, Adam m and v [xdn_cnn5_param_count()], packed fragments */
     float *params, *grad, *adam_m, *adam_v;
     void *packed;
-    int adam_step; /* step number of this update, from 1 */
+    int adam_step; /* step number of this update, from 1 (ignored when adam_t is set) */
     float *avg_loss; /* [1] mean loss of this frame */
+    /* optional device int32 step counter (number of updates done so far). When set, each step
+     * increments it and forms the Adam bias correction on the device, so a captured step graph
+     * stays valid on replay; NULL uses adam_step from the host. */
+    int *adam_t;
 } xdn_slot;
 
 size_t xdn_slot_scratch_bytes(int height, int width);
@@ -203,13 +222,23 @@ size_t xdn_slot_scratch_bytes(int height, int width);
  * y1, y2 [H,W,3]; var1, var2 [H,W,1]; regression coefficients; reprojection success [H,W,1] u8;
  * reprojected out_log, ols1, ols2 [H,W,3]; screened prev_out, prev_ols1, prev_ols2 [H,W,3];
  * success_prev, eroded success [H,W,1]; dparam, network output [H,W,6]; ones [H,W,1];
- * loss partials; cnn5 workspace. For inspection and tests. */
-#define XDN_SLOT_NBUF 19
+ * loss partials; cnn5 workspace; Adam step size [1]. For inspection and tests. */
+#define XDN_SLOT_NBUF 20
 int xdn_slot_scratch_offsets(int height, int width, size_t *offsets);
 /* Fills the slot's constant scratch and packs its network; call once when the slot is admitted. */
 int xdn_slot_init(const xdn_slot *slot, void *stream);
 /* One frame for each of n_slots (<= 16) slots, enqueued on `stream`. */
 int xdn_denoise_step(const xdn_slot *slots, int n_slots, float learning_rate, void *stream);
+
+/* The same step captured once into a HIP graph (one graph per frame: the slots' buffer
+ * addresses are baked in, so a caller keeps one graph per distinct set of frame / history
+ * buffers, e.g. per history parity). Every slot must have adam_t set. `stream` must not be the
+ * null stream; the graph is launched with xdn_denoise_graph_launch and freed with
+ * xdn_denoise_graph_destroy. No counterpart in the reference (its steps are TF eager calls,
+ * scripts/main.py:250-313). */
+int xdn_denoise_graph_create(const xdn_slot *slots, int n_slots, float learning_rate, void *stream, void **graph);
+int xdn_denoise_graph_launch(void *graph, void *stream);
+int xdn_denoise_graph_destroy(void *graph);
 
 #ifdef __cplusplus
 }

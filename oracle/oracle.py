@@ -46,6 +46,8 @@ def lib():
         L.orc_regression.argtypes = [_f32p, _f32p, _f64p, _f32p, _f32p, _f64p, _i, _i, _i, _i]
         L.orc_recon.argtypes = [_f64p, _f32p, _f64p, _f32p, _f32p, _f64p, _i, _i, _i, _i, _i]
         L.orc_cross_regression.argtypes = [_f32p] * 4 + [_f64p] * 6 + [_i] * 5
+        L.orc_cross_regression_grad.argtypes = [_f32p] * 4 + [_f32p, _f32p, ctypes.c_void_p, ctypes.c_void_p] + \
+            [_f64p] * 4 + [_i] * 4
         L.orc_spatial_filter.argtypes = [_f32p] * 5 + [_f64p] * 4 + [_i] * 3
         L.orc_spatial_filter_grad.argtypes = [_f32p] * 9 + [_f64p] + [_i] * 3
         L.orc_spatial_filter_grad_inputs.argtypes = [_f32p] * 9 + [_f64p] * 4 + [_i] * 3
@@ -139,6 +141,24 @@ def cross_regression(imga, imgb, albedo, normal, win_size=17, sparsity=4, recon_
                                    oa, ob, va, vb, ba, bb, H, W, win_size, sparsity, rs)
         outa[b], outb[b], vara[b, ..., 0], varb[b, ..., 0] = oa, ob, va, vb
     return outa, outb, vara, varb
+
+
+def cross_regression_grad(imga, imgb, albedo, normal, grad_outa, grad_outb, grad_vara=None, grad_varb=None,
+                          win_size=17, sparsity=4):
+    """FP64 adjoint of cross_regression (recon_scale 1): (grad_imga, grad_imgb, grad_albedo, grad_normal)."""
+    imga, imgb, albedo, normal, grad_outa, grad_outb = map(_f32, (imga, imgb, albedo, normal, grad_outa, grad_outb))
+    B, H, W, _ = imga.shape
+    outs = [np.zeros((B, H, W, 3)) for _ in range(4)]
+    for b in range(B):
+        gv = [None if g is None else np.ascontiguousarray(_f32(g)[b].reshape(H, W)) for g in (grad_vara, grad_varb)]
+        o = [np.zeros((H, W, 3)) for _ in range(4)]
+        lib().orc_cross_regression_grad(imga[b].copy(), imgb[b].copy(), albedo[b].copy(), normal[b].copy(),
+                                        grad_outa[b].copy(), grad_outb[b].copy(),
+                                        None if gv[0] is None else gv[0].ctypes.data,
+                                        None if gv[1] is None else gv[1].ctypes.data, *o, H, W, win_size, sparsity)
+        for k in range(4):
+            outs[k][b] = o[k]
+    return tuple(outs)
 
 
 # --------------------------------------------------------------------------------------------

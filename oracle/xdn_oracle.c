@@ -246,6 +246,240 @@ void orc_cross_regression(const float *ya, const float *yb, const float *alb, co
 }
 
 /* ------------------------------------------------------------------------------------------- */
+/* Backward of cross_regression with respect to its four inputs (recon_scale 1).               */
+/* The reference registers no gradient for this op; this is the adjoint of the forward above,   */
+/* differentiated in FP64 (scatter form, serial). Conventions:                                  */
+/*   - beta = M^-1 b with M = sum w d d^T + 1e-3 I; the SQRT clamp of the Cholesky never binds   */
+/*     (every pivot of M is >= 1e-3 > 1e-5), so the solve is differentiated as a linear solve:  */
+/*     u = M^-1 g_beta, g_b = u, g_M = -u beta^T.                                               */
+/*   - max(0, .) of the reconstruction passes the gradient only where the value is > 0; the     */
+/*     max(sumW, 1e-7) guard passes it only where sumW > 1e-7.                                  */
+/*   - d sqrt(var)/d var is taken as 0 where var == 0.                                         */
+/* ------------------------------------------------------------------------------------------- */
+typedef struct
+{
+    const float *f, *a, *n; /* own half (features), albedo, normal */
+    const double *v;        /* variance of the own half */
+    double *gf, *ga, *gn, *gv;
+} CrgHalf;
+
+/* Adjoint of (w, delta) for the pair (P, Q), delta = x_P - x_Q per channel:
+ * w = exp(-|f_P - f_Q|^2 / (v_P + v_Q + 1e-4)) + 1e-7, delta_1 = (f_P - f_Q)/(sqrt v_P + sqrt v_Q + 1e-2),
+ * delta_2 = a_P - a_Q, delta_3..5 = n_P - n_Q. */
+static void crg_pair(const CrgHalf *h, size_t P, size_t Q, double gw, const double gd[3][P6])
+{
+    double fP[3], fQ[3];
+    load3(h->f, P, fP); load3(h->f, Q, fQ);
+    const double vP = h->v[P], vQ = h->v[Q];
+    double D = 0;
+    for (int k = 0; k < 3; ++k) D += sq(fP[k] - fQ[k]);
+    const double S = vP + vQ + 1e-4, E = exp(-D / S);
+    const double gD = -gw * E / S, gS = gw * E * D / (S * S);
+    const double sP = sqrt(vP), sQ = sqrt(vQ), den = sP + sQ + 1e-2;
+    double gden = 0;
+    for (int k = 0; k < 3; ++k)
+    {
+        const double df = fP[k] - fQ[k];
+        const double gfk = gD * 2.0 * df + gd[k][1] / den;
+        h->gf[3 * P + k] += gfk;
+        h->gf[3 * Q + k] -= gfk;
+        gden -= gd[k][1] * (df / den) / den;
+        h->ga[3 * P + k] += gd[k][2];
+        h->ga[3 * Q + k] -= gd[k][2];
+        for (int j = 0; j < 3; ++j)
+        {
+            h->gn[3 * P + j] += gd[k][3 + j];
+            h->gn[3 * Q + j] -= gd[k][3 + j];
+        }
+    }
+    h->gv[P] += gS + (sP > 0 ? gden * 0.5 / sP : 0.0);
+    h->gv[Q] += gS + (sQ > 0 ? gden * 0.5 / sQ : 0.0);
+}
+
+/* One half: features f (with its variance v), target t, upstream gradient gout of the
+ * reconstruction and gvar_ext of the variance output. Accumulates into gf (features), gt (target),
+ * ga, gn. */
+static void crg_half(const float *f, const float *t, const float *alb, const float *nrm, const float *gout,
+                     const float *gvar_ext, double *gf, double *gt, double *ga, double *gn, int H, int W,
+                     int win, int s)
+{
+    const int half = win / 2, Hs = H / s, Ws = W / s;
+    const size_t N = (size_t)H * W;
+    double *v = (double *)malloc(N * sizeof(double));
+    double *beta = (double *)malloc((size_t)Hs * Ws * P6 * 3 * sizeof(double));
+    double *gbeta = (double *)calloc((size_t)Hs * Ws * P6 * 3, sizeof(double));
+    double *gv = (double *)calloc(N, sizeof(double));
+    orc_variance(f, v, H, W);
+    orc_regression(t, f, v, alb, nrm, beta, H, W, win, s);
+    CrgHalf h = {f, alb, nrm, v, gf, ga, gn, gv};
+
+    /* reconstruction (orc_recon), adjoint per pixel */
+    for (int py = 0; py < H; ++py)
+        for (int px = 0; px < W; ++px)
+        {
+            const size_t p = (size_t)py * W + px;
+            int sx = idivup(px - half, s); if (sx < 0) sx = 0;
+            int ex = (px + half) / s; if (ex > (W - 1) / s) ex = (W - 1) / s;
+            int sy = idivup(py - half, s); if (sy < 0) sy = 0;
+            int ey = (py + half) / s; if (ey > (H - 1) / s) ey = (H - 1) / s;
+            double cw[3], ca[3], cn[3];
+            load3(f, p, cw); load3(alb, p, ca); load3(nrm, p, cn);
+            const double cstd = sqrt(v[p]);
+            double acc[3] = {0, 0, 0}, sumw = 0;
+            for (int pass = 0; pass < 2; ++pass)
+            {
+                double gacc[3] = {0, 0, 0}, gsumw = 0;
+                if (pass == 1)
+                {
+                    const double dd = sumw > 1e-7 ? sumw : 1e-7;
+                    for (int ch = 0; ch < 3; ++ch)
+                        gacc[ch] = acc[ch] / dd > 0 ? gout[3 * p + ch] / dd : 0.0;
+                    if (sumw > 1e-7)
+                        for (int ch = 0; ch < 3; ++ch) gsumw -= gacc[ch] * acc[ch] / dd;
+                }
+                for (int iy = sy; iy <= ey; ++iy)
+                    for (int ix = sx; ix <= ex; ++ix)
+                    {
+                        const double *b = beta + ((size_t)iy * Ws + ix) * P6 * 3;
+                        const size_t c = (size_t)(iy * s) * W + (size_t)(ix * s);
+                        double iw[3], ia[3], in[3];
+                        load3(f, c, iw); load3(alb, c, ia); load3(nrm, c, in);
+                        const double den = (cstd + sqrt(v[c])) + 1e-2;
+                        const double w = reg_weight(cw, iw, v[p], v[c]);
+                        double pred[3], d[3][P6];
+                        for (int ch = 0; ch < 3; ++ch)
+                        {
+                            d[ch][0] = 1.0; d[ch][1] = (cw[ch] - iw[ch]) / den; d[ch][2] = ca[ch] - ia[ch];
+                            d[ch][3] = cn[0] - in[0]; d[ch][4] = cn[1] - in[1]; d[ch][5] = cn[2] - in[2];
+                            pred[ch] = 0;
+                            for (int k = 0; k < P6; ++k) pred[ch] += b[k * 3 + ch] * d[ch][k];
+                        }
+                        if (pass == 0)
+                        {
+                            for (int ch = 0; ch < 3; ++ch) acc[ch] += w * pred[ch];
+                            sumw += w;
+                            continue;
+                        }
+                        double gw = gsumw, gd[3][P6];
+                        double *gb = gbeta + ((size_t)iy * Ws + ix) * P6 * 3;
+                        for (int ch = 0; ch < 3; ++ch)
+                        {
+                            gw += gacc[ch] * pred[ch];
+                            for (int k = 0; k < P6; ++k)
+                            {
+                                gb[k * 3 + ch] += w * gacc[ch] * d[ch][k];
+                                gd[ch][k] = w * gacc[ch] * b[k * 3 + ch];
+                            }
+                        }
+                        crg_pair(&h, p, c, gw, (const double(*)[P6])gd);
+                    }
+            }
+        }
+
+    /* regression (orc_regression), adjoint per center */
+    for (int by = 0; by < Hs; ++by)
+        for (int bx = 0; bx < Ws; ++bx)
+        {
+            const int cx = bx * s, cy = by * s;
+            const size_t c = (size_t)cy * W + cx;
+            const double *b = beta + ((size_t)by * Ws + bx) * P6 * 3;
+            const double *gb = gbeta + ((size_t)by * Ws + bx) * P6 * 3;
+            double cw[3], ca[3], cn[3];
+            load3(f, c, cw); load3(alb, c, ca); load3(nrm, c, cn);
+            const double cstd = sqrt(v[c]);
+            double A[3][P6][P6], G[3][P6], u[P6][3];
+            memset(A, 0, sizeof(A));
+            for (int pass = 0; pass < 2; ++pass)
+            {
+                if (pass == 1)
+                {
+                    for (int ch = 0; ch < 3; ++ch)
+                        for (int r = 0; r < P6; ++r) G[ch][r] = gb[r * 3 + ch];
+                    solve_center((const double(*)[P6][P6])A, (const double(*)[P6])G, u);
+                }
+                for (int iy = cy - half; iy <= cy + half; ++iy)
+                    for (int ix = cx - half; ix <= cx + half; ++ix)
+                    {
+                        const size_t i = (size_t)mirror_idx(iy, H) * W + mirror_idx(ix, W);
+                        double iw[3], ia[3], in[3], it[3];
+                        load3(f, i, iw); load3(alb, i, ia); load3(nrm, i, in); load3(t, i, it);
+                        const double w = reg_weight(cw, iw, v[c], v[i]);
+                        const double den = (cstd + sqrt(v[i])) + 1e-2;
+                        double d[3][P6];
+                        for (int ch = 0; ch < 3; ++ch)
+                        {
+                            d[ch][0] = 1.0; d[ch][1] = (iw[ch] - cw[ch]) / den; d[ch][2] = ia[ch] - ca[ch];
+                            d[ch][3] = in[0] - cn[0]; d[ch][4] = in[1] - cn[1]; d[ch][5] = in[2] - cn[2];
+                        }
+                        if (pass == 0)
+                        {
+                            for (int ch = 0; ch < 3; ++ch)
+                                for (int r = 0; r < P6; ++r)
+                                    for (int q = 0; q < P6; ++q) A[ch][r][q] += w * d[ch][r] * d[ch][q];
+                            continue;
+                        }
+                        double gw = 0, gd[3][P6];
+                        for (int ch = 0; ch < 3; ++ch)
+                        {
+                            double r = it[ch], sdot = 0;
+                            for (int k = 0; k < P6; ++k)
+                            {
+                                r -= d[ch][k] * b[k * 3 + ch];
+                                sdot += u[k][ch] * d[ch][k];
+                            }
+                            gw += sdot * r;
+                            for (int k = 0; k < P6; ++k) gd[ch][k] = w * (u[k][ch] * r - b[k * 3 + ch] * sdot);
+                            gt[3 * i + ch] += w * sdot;
+                        }
+                        crg_pair(&h, i, c, gw, (const double(*)[P6])gd);
+                    }
+            }
+        }
+
+    /* variance (orc_variance), adjoint */
+    if (gvar_ext)
+        for (size_t p = 0; p < N; ++p) gv[p] += gvar_ext[p];
+    for (int cy = 0; cy < H; ++cy)
+        for (int cx = 0; cx < W; ++cx)
+        {
+            int sx = cx - 1 < 0 ? 0 : cx - 1, ex = cx + 1 > W - 1 ? W - 1 : cx + 1;
+            int sy = cy - 1 < 0 ? 0 : cy - 1, ey = cy + 1 > H - 1 ? H - 1 : cy + 1;
+            const int n = (ex - sx + 1) * (ey - sy + 1) - 1;
+            const size_t p = (size_t)cy * W + cx;
+            double m[3] = {0, 0, 0};
+            for (int iy = sy; iy <= ey; ++iy)
+                for (int ix = sx; ix <= ex; ++ix)
+                    if (iy != cy || ix != cx)
+                        for (int k = 0; k < 3; ++k) m[k] += f[3 * ((size_t)iy * W + ix) + k];
+            for (int k = 0; k < 3; ++k)
+            {
+                const double e = gv[p] * 2.0 * (m[k] / n - f[3 * p + k]);
+                gf[3 * p + k] -= e;
+                for (int iy = sy; iy <= ey; ++iy)
+                    for (int ix = sx; ix <= ex; ++ix)
+                        if (iy != cy || ix != cx) gf[3 * ((size_t)iy * W + ix) + k] += e / n;
+            }
+        }
+    free(v); free(beta); free(gbeta); free(gv);
+}
+
+/* Gradients of L(outa, outb, vara, varb) with respect to (ya, yb, albedo, normal), given the
+ * upstream gradients of the four outputs of orc_cross_regression (g_vara/g_varb may be NULL).
+ * The four outputs are overwritten. */
+void orc_cross_regression_grad(const float *ya, const float *yb, const float *alb, const float *nrm,
+                               const float *g_outa, const float *g_outb, const float *g_vara,
+                               const float *g_varb, double *g_ya, double *g_yb, double *g_alb, double *g_nrm,
+                               int H, int W, int win, int s)
+{
+    const size_t n3 = (size_t)H * W * 3;
+    memset(g_ya, 0, n3 * sizeof(double)); memset(g_yb, 0, n3 * sizeof(double));
+    memset(g_alb, 0, n3 * sizeof(double)); memset(g_nrm, 0, n3 * sizeof(double));
+    /* half A: features ya, target yb; half B: features yb, target ya */
+    crg_half(ya, yb, alb, nrm, g_outa, g_vara, g_ya, g_yb, g_alb, g_nrm, H, W, win, s);
+    crg_half(yb, ya, alb, nrm, g_outb, g_varb, g_yb, g_ya, g_alb, g_nrm, H, W, win, s);
+}
+
+/* ------------------------------------------------------------------------------------------- */
 /* a6: bandwidth-driven bilateral filter, SpatialFilterKernel regression.cu.cc:84-195.          */
 /* w^X = exp(-b0|dX|^2 - b2|dalb|^2 - b3|dn|^2 - b4*((dx/h)^2+(dy/h)^2)), X in {A (b0), B (b1)}*/
 /* 11x11 reflect-101; outputs the unnormalized sum and the weight sum.                          */

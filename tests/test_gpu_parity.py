@@ -58,6 +58,57 @@ def test_cross_regression_generic_attrs(ops, dev, win, s, rs):
         assert err <= TOL, name
 
 
+def cr_grad_case(H, W, seed, B=1):
+    ya, yb = log_images(H, W, seed=seed, B=B)
+    return [ya, yb, rand_img(H, W, seed=seed + 1, B=B), unit_normals(H, W, seed=seed + 2, B=B)]
+
+
+# Gradients pass through the per-center 6x6 solves (u = M^-1 g_beta) in FP32; stated tolerance.
+CR_GRAD_TOL = 1e-4
+
+
+@pytest.mark.parametrize("B,H,W,win,s", [(1, 48, 64, 17, 4), (2, 36, 20, 17, 4), (1, 30, 27, 9, 3), (1, 24, 32, 13, 2)])
+def test_cross_regression_grad(ops, dev, B, H, W, win, s):
+    args = cr_grad_case(H, W, seed=H + W, B=B)
+    cots = _cots(B, H, W, 12)
+    ref = O.cross_regression_grad(*args, *cots, win_size=win, sparsity=s)
+    got = ops.cross_regression_grad(*[to_dev(a, dev) for a in args], *[to_dev(c, dev) for c in cots],
+                                    win_size=win, sparsity=s)
+    for name, g, r in zip(("imga", "imgb", "albedo", "normal"), got, ref):
+        err = rel_l2(to_np(g), r)
+        _report(f"cross_regression_grad {name} {B}x{H}x{W} win={win} s={s}", err)
+        assert err <= CR_GRAD_TOL, name
+
+
+def test_cross_regression_grad_512_sampled(ops, dev):
+    """512x512 inputs, gradient of L = sum r . outputs on 1024 random pixels against the FP64
+    adjoint (itself pinned by central differences of the FP64 forward)."""
+    H = W = 512
+    args = cr_grad_case(H, W, seed=512)
+    cots = _cots(1, H, W, 13)
+    ref = O.cross_regression_grad(*args, *cots)
+    got = ops.cross_regression_grad(*[to_dev(a, dev) for a in args], *[to_dev(c, dev) for c in cots],
+                                    win_size=17, sparsity=4)
+    r = np.random.default_rng(14)
+    ys, xs = r.integers(0, H, 1024), r.integers(0, W, 1024)
+    for name, g, rr in zip(("imga", "imgb", "albedo", "normal"), got, ref):
+        err = rel_l2(to_np(g)[0, ys, xs], rr[0, ys, xs])
+        _report(f"cross_regression_grad {name} 512x512 (1024 px)", err)
+        assert err <= CR_GRAD_TOL, name
+
+
+def test_cross_regression_autograd(ops, dev):
+    H, W = 32, 40
+    args = [to_dev(a, dev).requires_grad_(True) for a in cr_grad_case(H, W, seed=5)]
+    outs = ops.cross_regression(*args, win_size=17, sparsity=4)
+    cots = [to_dev(c, dev) for c in _cots(1, H, W, 15)]
+    L = sum((o * c).sum() for o, c in zip(outs, cots))
+    L.backward()
+    direct = ops.cross_regression_grad(*[a.detach() for a in args], *cots, win_size=17, sparsity=4)
+    for a, g in zip(args, direct):
+        assert torch.equal(a.grad, g)
+
+
 def test_cross_regression_errors(ops, dev):
     x = torch.zeros(1, 32, 32, 3, device=dev)
     with pytest.raises(ValueError, match="recon_scale must be less than or equal to sparsity"):

@@ -192,6 +192,56 @@ def test_cross_regression_composes_stages():
     np.testing.assert_array_equal(ob, O.recon(beta_b, yb, vb, alb, nrm))
 
 
+def _cr_inputs(H, W, seed):
+    r = np.random.default_rng(seed)
+    ref = 1.0 + r.random((1, H, W, 3))
+    ya = np.log1p(ref * r.lognormal(0, 1.0, (1, H, W, 3))).astype(np.float32)
+    yb = np.log1p(ref * r.lognormal(0, 1.0, (1, H, W, 3))).astype(np.float32)
+    return [ya, yb, rand_img(H, W, seed=seed + 1), unit_normals(H, W, seed=seed + 2)]
+
+
+@pytest.mark.parametrize("H,W,win,s", [(16, 20, 17, 4), (12, 9, 7, 3)])
+def test_cross_regression_grad_matches_central_differences(H, W, win, s):
+    """The FP64 adjoint of cross_regression against central differences of the forward, with a
+    Richardson step (h = 2^-10 and 2^-11, exactly representable perturbations of the FP32 inputs)
+    on random elements of all four inputs; L = sum of seeded cotangents times all four outputs."""
+    ins = _cr_inputs(H, W, 70 + H)
+    r = np.random.default_rng(71)
+    cots = [r.normal(size=(1, H, W, c)).astype(np.float32) for c in (3, 3, 1, 1)]
+    c64 = [c.astype(np.float64) for c in cots]
+    grads = O.cross_regression_grad(*ins, *cots, win_size=win, sparsity=s)
+
+    def L(xs):
+        return sum(float((o * c).sum()) for o, c in zip(O.cross_regression(*xs, win, s, 1), c64))
+
+    def d(k, ix, hh):
+        xp, xm = [x.copy() for x in ins], [x.copy() for x in ins]
+        xp[k][ix] = np.float32(ins[k][ix] + hh)
+        xm[k][ix] = np.float32(ins[k][ix] - hh)
+        return (L(xp) - L(xm)) / (float(xp[k][ix]) - float(xm[k][ix]))
+
+    h = 2.0 ** -10
+    for k in range(4):
+        idx = list(np.ndindex(ins[k].shape))
+        pick = [idx[i] for i in r.choice(len(idx), 40, replace=False)] + [(0, 0, 0, 0), (0, H - 1, W - 1, 2)]
+        fd = np.array([(4 * d(k, ix, h / 2) - d(k, ix, h)) / 3 for ix in pick])
+        an = np.array([grads[k][ix] for ix in pick])
+        assert np.linalg.norm(an - fd) <= 1e-6 * np.linalg.norm(fd), k
+
+
+def test_cross_regression_grad_linear_in_cotangents():
+    ins = _cr_inputs(12, 16, 80)
+    r = np.random.default_rng(81)
+    c1 = [r.normal(size=(1, 12, 16, c)).astype(np.float32) for c in (3, 3, 1, 1)]
+    c2 = [r.normal(size=(1, 12, 16, c)).astype(np.float32) for c in (3, 3, 1, 1)]
+    g1 = O.cross_regression_grad(*ins, *c1)
+    g2 = O.cross_regression_grad(*ins, *c2)
+    g12 = O.cross_regression_grad(*ins, *[a + b for a, b in zip(c1, c2)])
+    # the only difference is the FP32 rounding of the summed cotangents
+    for a, b, ab in zip(g1, g2, g12):
+        assert np.linalg.norm(ab - (a + b)) <= 1e-6 * np.linalg.norm(ab)
+
+
 # ------------------------------------------------------------------------------------------------
 # a6/a7 spatial filter
 # ------------------------------------------------------------------------------------------------
