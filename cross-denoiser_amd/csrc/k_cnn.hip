@@ -62,7 +62,8 @@ __global__ __launch_bounds__(256) void conv3x3(ConvBatch b)
     constexpr int PS = CIN + 8;  // LDS pixel stride in bf16 (20 or 12 dwords: conflict-free b128 reads)
     constexpr int LW = CT_W + 2, LH = CT_H + 2;
     constexpr int KC = CIN / 16;
-    __shared__ __attribute__((aligned(16))) uint16_t lds[LH * LW * PS];
+    constexpr int IN_ELEMS = LH * LW * PS, OUT_ELEMS = CT_H * CT_W * 40;  // input tile / epilogue tile
+    __shared__ __attribute__((aligned(16))) uint16_t lds[IN_ELEMS > OUT_ELEMS ? IN_ELEMS : OUT_ELEMS];
     const int tile = xcd_tile(blockIdx.x, b.map.total);
     const TileLoc L = locate(b.map, tile);
     const ConvFrame &F = b.f[L.f];
@@ -152,31 +153,79 @@ __global__ __launch_bounds__(256) void conv3x3(ConvBatch b)
                 }
             }
 
-    // ---- epilogue: lane owns output channel px; 16 pixels i = 8(k/4) + 4h + k%4 per row ----
-#pragma unroll
-    for (int orow = 0; orow < 4; ++orow)
+    // ---- epilogue: lane owns output channel px of 16 pixels x = 8(k/4) + 4h + k%4 per row. The
+    // tile is transposed through LDS (the input tile is dead after the barrier) so every global
+    // store -- and the ReLU-mask load of the backward -- moves 16 contiguous bytes.
+    __syncthreads();
+    if constexpr (EPI == CONV_EPI_F32_6)
     {
-        const int gy = y0 + 4 * wv + orow;
-        if (gy >= H) continue;
+        float *o = reinterpret_cast<float *>(lds);  // [16 rows][32 px][6]
+        static_assert(sizeof(lds) >= CT_H * CT_W * 6 * sizeof(float), "epilogue buffer");
+        if (px < 6)
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
+            for (int orow = 0; orow < 4; ++orow)
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                {
+                    const int x = 8 * (k >> 2) + 4 * h + (k & 3);
+                    o[((4 * wv + orow) * CT_W + x) * 6 + px] = acc[orow][k] + bias;
+                }
+        __syncthreads();
+        // each row of the tile is 32 px x 6 floats = 48 float4, contiguous in the image
+        for (int i = tid; i < CT_H * 48; i += 256)
         {
-            const int gx = x0 + 8 * (k >> 2) + 4 * h + (k & 3);
-            if (gx >= W) continue;
-            const size_t p = (size_t)gy * W + gx;
-            const float v = acc[orow][k];
-            if constexpr (EPI == CONV_EPI_RELU)
-                static_cast<uint16_t *>(F.out)[p * 32 + px] = (uint16_t)f2bf(fmaxf(v + bias, 0.f));
-            else if constexpr (EPI == CONV_EPI_MASK)
-            {
-                const uint16_t m = F.mask[p * 32 + px];
-                const bool pos = (m & 0x8000u) == 0 && (m & 0x7FFFu) != 0;
-                static_cast<uint16_t *>(F.out)[p * 32 + px] = pos ? (uint16_t)f2bf(v) : (uint16_t)0;
-            }
+            const int r = i / 48, q = i - r * 48;
+            const int gy = y0 + r, x = (q * 4) / 6;
+            if (gy >= H || x0 + x >= W) continue;
+            const float4 v = reinterpret_cast<const float4 *>(o)[r * 48 + q];
+            float *dst = static_cast<float *>(F.out) + ((size_t)gy * W + x0) * 6 + q * 4;
+            if (x0 + CT_W <= W && (W & 1) == 0)  // whole row of the tile, 16-B aligned
+                *reinterpret_cast<float4 *>(dst) = v;
             else
             {
-                if (px < 6) static_cast<float *>(F.out)[p * 6 + px] = v + bias;
+                // partial tile: element e of this float4 is pixel (4q + e) / 6
+                const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (x0 + (4 * q + e) / 6 < W) dst[e] = vv[e];
             }
+        }
+    }
+    else
+    {
+        constexpr int OS = 40;  // bf16 pixel stride of the output tile in LDS
+        static_assert(sizeof(lds) >= CT_H * CT_W * OS * 2, "epilogue buffer");
+#pragma unroll
+        for (int orow = 0; orow < 4; ++orow)
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+            {
+                const int x = 8 * (k >> 2) + 4 * h + (k & 3);
+                const float v = acc[orow][k];
+                lds[((4 * wv + orow) * CT_W + x) * OS + px] =
+                    (uint16_t)f2bf(EPI == CONV_EPI_RELU ? fmaxf(v + bias, 0.f) : v);
+            }
+        __syncthreads();
+        for (int i = tid; i < CT_H * CT_W * 4; i += 256)
+        {
+            const int pix = i >> 2, q = i & 3;
+            const int gy = y0 + pix / CT_W, gx = x0 + (pix & (CT_W - 1));
+            if (gy >= H || gx >= W) continue;
+            u32x4 v = *reinterpret_cast<const u32x4 *>(&lds[pix * OS + q * 8]);
+            const size_t o = ((size_t)gy * W + gx) * 32 + q * 8;
+            if constexpr (EPI == CONV_EPI_MASK)
+            {
+                // keep the gradient where the layer's bf16 activation is > 0 (ReLU derivative)
+                const u32x4 m = *reinterpret_cast<const u32x4 *>(F.mask + o);
+                auto keep = [](uint32_t mw, uint32_t vw) {
+                    const uint32_t lo = mw & 0xFFFFu, hi = mw >> 16;
+                    const uint32_t klo = ((lo & 0x8000u) == 0 && lo != 0) ? 0xFFFFu : 0u;
+                    const uint32_t khi = ((hi & 0x8000u) == 0 && hi != 0) ? 0xFFFF0000u : 0u;
+                    return vw & (klo | khi);
+                };
+                v.x = keep(m.x, v.x); v.y = keep(m.y, v.y); v.z = keep(m.z, v.z); v.w = keep(m.w, v.w);
+            }
+            *reinterpret_cast<u32x4 *>(static_cast<uint16_t *>(F.out) + o) = v;
         }
     }
 }
@@ -191,20 +240,37 @@ struct WgBatch {
 };
 static_assert(sizeof(WgBatch) <= 4096, "kernel argument too large");
 
-constexpr int WS_W = 32, WS_H = 8;          // strip: 8 rows x 32 px
-constexpr int XS_CS = (WS_H + 2) * WS_W + 8;  // per channel and shift copy: 328 bf16 = 164 dwords
-constexpr int DT_CS = WS_H * WS_W + 8;        // 264 bf16 = 132 dwords
+constexpr int WS_W = 32, WS_H = 8;              // strip: 8 rows x 32 px
+constexpr int WX_C = WS_W + 2, WX_R = WS_H + 2;  // staged X: 10 rows x 34 px (1-px halo)
 
-// The X operand is staged three times, shifted by -1, 0, +1 px (copy sh holds X at x0 + c + sh - 1
-// in column c), so every MFMA operand is one aligned ds_read_b128. MFMA operands are never built
-// with VALU instructions: on gfx950 a VALU write to a VGPR that a just-issued MFMA reads as
-// SrcA/SrcB corrupted the product in this kernel (observed non-deterministic results).
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s lds_v4s;
+
+// Transposing LDS read (ds_read_b64_tr_b16). In each group of 16 lanes, lane L receives element
+// L%4 of the 8-byte blocks addressed by the lanes 4j + L/4 (j = 0..3). So when lane s of a group
+// addresses pixel p0 + s/4, channels c0 + 4(s%4) .. +3 of a pixel-major tile, lane L receives
+// channel c0 + L of the pixels p0 .. p0+3: four consecutive K values of an MFMA operand.
+__device__ __forceinline__ uint2 ld_tr(const uint16_t *p)
+{
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)p));
+}
+
+// Weight gradient dW[t][ci][co] = sum_p X[p + shift_t][ci] dY[p][co] over 8x32-px strips.
+// X (with a 1-px halo) and dY are staged pixel-major exactly as they sit in HBM (16-B stores);
+// the channel-major MFMA operands (M = ci, N = co, K = 16 px: 8 px of strip row 2wv + h per lane
+// half h) come from transposing reads, the +-1-px shifts of the taps being address offsets.
+// Wave wv accumulates strip rows 2wv and 2wv+1; the four waves' accumulators are summed through
+// LDS before the block writes its partial.
 template <int XK, int CI, int DYK>
 __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t xs[3 * 32 * XS_CS];
-    __shared__ __attribute__((aligned(16))) uint16_t dt[32 * DT_CS];
+    constexpr int XS = CI > 16 ? 32 : 16;                // staged X channels (pixel stride)
+    constexpr int DS = DYK == CONV_IN_BF16 ? 32 : 16;    // staged dY channels
+    constexpr int X_ELEMS = WX_R * WX_C * XS, D_ELEMS = WS_H * WS_W * DS;
+    constexpr int LDS_ELEMS = (X_ELEMS + D_ELEMS) > 8192 ? (X_ELEMS + D_ELEMS) : 8192;  // >= 4 x 1024 floats
+    __shared__ __attribute__((aligned(16))) uint16_t lds[LDS_ELEMS];
     __shared__ float dbs[4][32];
+    uint16_t *xs = lds, *ds = lds + X_ELEMS;
     int f = 0;
     while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
     const WgFrame &F = b.f[f];
@@ -212,8 +278,10 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
     const int H = F.H, W = F.W;
     const int sx = (W + WS_W - 1) / WS_W, sy = (H + WS_H - 1) / WS_H;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, h = lane >> 5, ch = lane & 31;
-    const int rr = 2 * wv + h;  // strip row of this lane group
-    constexpr int XC = WS_W + 2;  // staged columns x0-1 .. x0+32
+    const int rr = 2 * wv + h;                           // strip row of this lane half
+    const int grp = (lane >> 4) & 1, s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3;
+    const int cbx = (XS > 16 ? 16 * grp : 0) + 4 * qq;   // channel offset addressed by this lane
+    const int cbd = (DS > 16 ? 16 * grp : 0) + 4 * qq;
 
     f32x16 acc[9];
 #pragma unroll
@@ -226,41 +294,27 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
     {
         const int y0 = (s / sx) * WS_H, x0 = (s % sx) * WS_W;
         __syncthreads();  // the previous strip's reads are done
-        // X: column col <-> x0 - 1 + col, row r <-> y0 - 1 + r; scattered into the three copies
-        auto put = [&](int r, int col, int c0, const uint16_t (&e)[8], int ne) {
-#pragma unroll
-            for (int sh = 0; sh < 3; ++sh)
-            {
-                const int c = col - sh;
-                if (c < 0 || c >= WS_W) continue;
-                uint16_t *d = &xs[(sh * 32 + c0) * XS_CS + r * WS_W + c];
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (j < ne) d[j * XS_CS] = e[j];
-            }
-        };
+        // X: LDS pixel (r, col) <-> (y0 - 1 + r, x0 - 1 + col), zero outside the image
         if constexpr (XK == CONV_IN_BF16)
         {
             const uint16_t *src = static_cast<const uint16_t *>(F.x[0]);
-            for (int i = tid; i < (WS_H + 2) * XC * 4; i += 256)
+            for (int i = tid; i < WX_R * WX_C * 4; i += 256)
             {
                 const int pix = i >> 2, q = i & 3;
-                const int r = pix / XC, col = pix - r * XC;
+                const int r = pix / WX_C, col = pix - r * WX_C;
                 const int gy = y0 - 1 + r, gx = x0 - 1 + col;
                 u32x4 v = {0u, 0u, 0u, 0u};
                 if (gy >= 0 && gy < H && gx >= 0 && gx < W)
                     v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * 32 + q * 8);
-                const uint16_t e[8] = {(uint16_t)v.x, (uint16_t)(v.x >> 16), (uint16_t)v.y, (uint16_t)(v.y >> 16),
-                                       (uint16_t)v.z, (uint16_t)(v.z >> 16), (uint16_t)v.w, (uint16_t)(v.w >> 16)};
-                put(r, col, q * 8, e, 8);
+                *reinterpret_cast<u32x4 *>(&xs[pix * XS + q * 8]) = v;
             }
         }
         else
         {
-            for (int i = tid; i < (WS_H + 2) * XC * 2; i += 256)
+            for (int i = tid; i < WX_R * WX_C * 2; i += 256)
             {
                 const int pix = i >> 1, q = i & 1;  // q: channels 0-7 or 8-15 of the 16 (15 real + 0)
-                const int r = pix / XC, col = pix - r * XC;
+                const int r = pix / WX_C, col = pix - r * WX_C;
                 const int gy = y0 - 1 + r, gx = x0 - 1 + col;
                 float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                 if (gy >= 0 && gy < H && gx >= 0 && gx < W)
@@ -273,13 +327,12 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
                         if (k < 15) v[j] = static_cast<const float *>(F.x[k / 3])[3 * p + k % 3];
                     }
                 }
-                uint16_t e[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) e[j] = (uint16_t)f2bf(v[j]);
-                put(r, col, q * 8, e, 8);
+                u32x4 a;
+                a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = pack2(v[6], v[7]);
+                *reinterpret_cast<u32x4 *>(&xs[pix * XS + q * 8]) = a;
             }
         }
-        // dY, channel-major: dt[co][r][col], r <-> y0 + r, col <-> x0 + col
+        // dY: LDS pixel (r, col) <-> (y0 + r, x0 + col)
         if constexpr (DYK == CONV_IN_BF16)
         {
             const uint16_t *src = static_cast<const uint16_t *>(F.dy);
@@ -290,11 +343,7 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
                 const int gy = y0 + r, gx = x0 + col;
                 u32x4 v = {0u, 0u, 0u, 0u};
                 if (gy < H && gx < W) v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * 32 + q * 8);
-                uint16_t *d = &dt[(q * 8) * DT_CS + r * WS_W + col];
-                d[0 * DT_CS] = (uint16_t)v.x; d[1 * DT_CS] = (uint16_t)(v.x >> 16);
-                d[2 * DT_CS] = (uint16_t)v.y; d[3 * DT_CS] = (uint16_t)(v.y >> 16);
-                d[4 * DT_CS] = (uint16_t)v.z; d[5 * DT_CS] = (uint16_t)(v.z >> 16);
-                d[6 * DT_CS] = (uint16_t)v.w; d[7 * DT_CS] = (uint16_t)(v.w >> 16);
+                *reinterpret_cast<u32x4 *>(&ds[pix * DS + q * 8]) = v;
             }
         }
         else
@@ -310,19 +359,19 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
 #pragma unroll
                     for (int k = 0; k < 6; ++k) v[k] = sp[k];
                 }
-                uint16_t *d = &dt[r * WS_W + col];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) d[k * DT_CS] = (uint16_t)f2bf(v[k]);
-#pragma unroll
-                for (int k = 6; k < 32; ++k) d[k * DT_CS] = 0;
+                u32x4 a, z = {0u, 0u, 0u, 0u};
+                a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = 0u;
+                *reinterpret_cast<u32x4 *>(&ds[pix * DS]) = a;
+                *reinterpret_cast<u32x4 *>(&ds[pix * DS + 8]) = z;
             }
         }
         __syncthreads();
-        const bool live_ci = (CI == 32) || ch < CI;
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {
-            const u32x4 B = *reinterpret_cast<const u32x4 *>(&dt[ch * DT_CS + rr * WS_W + 8 * c]);
+            const uint16_t *dp = &ds[(rr * WS_W + 8 * c + jq) * DS + cbd];
+            const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * DS);
+            const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
             db += (bf_lo(B.x) + bf_hi(B.x)) + (bf_lo(B.y) + bf_hi(B.y)) + (bf_lo(B.z) + bf_hi(B.z)) +
                   (bf_lo(B.w) + bf_hi(B.w));
 #pragma unroll
@@ -330,23 +379,29 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
 #pragma unroll
                 for (int sh = 0; sh < 3; ++sh)
                 {
-                    const u32x4 A = live_ci ? *reinterpret_cast<const u32x4 *>(
-                                                  &xs[(sh * 32 + ch) * XS_CS + (rr + dy) * WS_W + 8 * c])
-                                            : u32x4{0u, 0u, 0u, 0u};
+                    const uint16_t *xp = &xs[((rr + dy) * WX_C + 8 * c + sh + jq) * XS + cbx];
+                    const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
+                    const u32x4 A = {a0.x, a0.y, a1.x, a1.y};
                     acc[dy * 3 + sh] = mfma(A, B, acc[dy * 3 + sh]);
                 }
         }
     }
-    // partials: [t][ci][co] then [co]
+    // partials: [t][ci][co] then [co]; the four waves' sums are added in LDS one tap at a time
     float *out = F.part + (size_t)li * WG_PART;
+    float *red = reinterpret_cast<float *>(lds);  // [4 waves][1024]
 #pragma unroll
     for (int t = 0; t < 9; ++t)
+    {
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < 16; ++k)
         {
             const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
-            out[t * 1024 + ci * 32 + ch] = acc[t][k];
+            red[wv * 1024 + ci * 32 + ch] = acc[t][k];
         }
+        __syncthreads();
+        for (int e = tid; e < 1024; e += 256) out[t * 1024 + e] = (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]);
+    }
     db += __shfl_xor(db, 32);
     if (h == 0) dbs[wv][ch] = db;
     __syncthreads();

@@ -5,7 +5,8 @@ owns a slot whose history, network, Adam state and scratch stay resident in HBM.
 continuous batching: whenever the GPU can take work, every stream that has an arrived frame and no
 frame in flight joins the next batch, which is ONE batched step (one launch per stage over all its
 streams, ``xdn_denoise_step``). Batches alternate between two HIP streams so a new batch overlaps the
-tail of the previous one on the device. Streams are admitted against an HBM budget; idle streams are
+tail of the previous one on the device. The comparison modes are the plain loop (one frame at a time,
+eager launches) and per-stream captured step graphs (``pipeline.StepGraph``). Streams are admitted against an HBM budget; idle streams are
 evicted (their slot memory released) to admit waiting ones. Per-frame latency = completion (HIP
 event) - arrival; frames/s and p50/p99 are reported.
 
@@ -129,10 +130,11 @@ class Admission:
 # ------------------------------------------------------------------------------------------------
 class Server:
     def __init__(self, device="cuda", hbm_budget=64 << 30, max_batch=16, lr=1e-3, pool_frames=4, batching=True,
-                 idle_after=0.05, graphs=False):
+                 idle_after=0.05, graphs=False, max_inflight=2):
         """batching: same-stage kernels of all ready streams in one launch (xdn_denoise_step over
         many slots); otherwise one frame at a time. graphs (with batching off): each frame is the
-        stream's captured step graph (one per frame buffers and history parity)."""
+        stream's captured step graph (one per frame buffers and history parity). max_inflight:
+        batches on the device at once (each on its own HIP stream)."""
         import torch
         self.torch = torch
         self.device = torch.device(device)
@@ -142,7 +144,8 @@ class Server:
         self.lr = lr
         self.pool_frames = pool_frames
         self.idle_after = idle_after
-        self.hip_streams = [torch.cuda.Stream(self.device), torch.cuda.Stream(self.device)]
+        self.max_inflight = max(1, int(max_inflight))
+        self.hip_streams = [torch.cuda.Stream(self.device) for _ in range(self.max_inflight)]
         self.denoisers = {}
         self.pools = {}
         self.saved = {}    # evicted streams: (params, adam m, adam v, step) on the host
@@ -214,8 +217,8 @@ class Server:
                 ready.append(sid)
                 if len(ready) >= self.max_batch:
                     break
-            if ready and len(inflight) < 2:
-                st = self.hip_streams[bi % 2]
+            if ready and len(inflight) < self.max_inflight:
+                st = self.hip_streams[bi % self.max_inflight]
                 bi += 1
                 items, dens, frs = [], [], []
                 for sid in ready:
@@ -269,3 +272,63 @@ class Server:
         if d is not None:
             self.saved[sid] = (d.params.cpu(), d.adam_m.cpu(), d.adam_v.cpu(), d.t)
             d.close()
+
+
+# ------------------------------------------------------------------------------------------------
+# Command line: python -m xdn.serving --config 3 --mode all
+# ------------------------------------------------------------------------------------------------
+def warm_up(device, sizes, lr=1e-3):
+    """First launches of every kernel at every resolution (module load, code object upload)."""
+    import torch
+
+    from . import pipeline, synth
+    for (h, w) in sorted(set(sizes)):
+        spec = synth.StreamSpec(h, w, seed=99)
+        fr = [synth.make_frame(spec, t, device=device) for t in range(2)]
+        d = pipeline.Denoiser(h, w, device=device, lr=lr)
+        d.step(fr[0])
+        d.step(fr[1])
+        pipeline.denoise_step_multi([d], [fr[0]])
+        torch.cuda.synchronize()
+        d.close()
+        del d, fr
+    torch.cuda.empty_cache()
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+
+    import torch
+    ap = argparse.ArgumentParser(description="multi-stream serving loop of the online denoiser")
+    ap.add_argument("--config", type=int, choices=(3, 4), default=3)
+    ap.add_argument("--frames", type=int, default=64, help="frames per stream")
+    ap.add_argument("--mode", choices=("batched", "plain", "graphs", "all"), default="all")
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args(argv)
+    dev = torch.device("cuda:0")
+    reqs = config3(a.frames, seed=a.seed) if a.config == 3 else config4(a.frames, seed=a.seed)
+    warm_up(dev, [(r.height, r.width) for r in reqs])
+    modes = ("plain", "graphs", "batched") if a.mode == "all" else (a.mode,)
+    for m in modes:
+        print(f"[serving] config {a.config}, mode {m}, {len(reqs)} streams x {a.frames} frames", file=sys.stderr,
+              flush=True)
+        # plain: the one-frame-at-a-time loop (one frame on the device at a time, eager launches)
+        srv = Server(dev, hbm_budget=int(a.budget_gb * (1 << 30)), batching=(m == "batched"), graphs=(m == "graphs"),
+                     max_inflight=1 if m == "plain" else 2)
+        res = srv.run(reqs)
+        res.update({"config": a.config, "mode": m, "streams": len(reqs), "frames_per_stream": a.frames,
+                    "budget_gb": a.budget_gb,
+                    "stream_sizes": sorted({f"{r.width}x{r.height}" for r in reqs})})
+        print(json.dumps(res), flush=True)
+        for d in srv.denoisers.values():
+            d.close()
+        del srv
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+
+
+if __name__ == "__main__":
+    main()

@@ -3,8 +3,16 @@
 The reference applies the bf16 roundings at the same points as the kernels (network input,
 weights, each hidden activation, each backward activation gradient) and computes everything else
 in float64 with library convolutions (torch.nn.functional.conv2d and torch.nn.grad); the kernels
-accumulate in FP32. Tolerances (relative L2): network output <= 2e-3, weight/bias gradients
-<= 5e-3 (FP32 accumulation over all pixels and rare bf16 rounding flips of near-tie values).
+accumulate in FP32.
+
+Two bars:
+  * per kernel, fed with the kernels' own bf16 operands read back from the workspace (so no
+    rounding difference propagates): weight and bias gradients <= 1e-5 relative L2, data gradient
+    equal to the bf16-rounded float64 result except for 1-ulp rounding ties (<= 1e-3 of elements);
+  * end to end (the chain of 4 bf16-rounded data gradients from the float64 chain of its own):
+    network output <= 2e-3, gradients of layers 4-5 <= 5e-3, of layers 1-3 <= 2e-2 (a value that
+    rounds differently in one layer is carried through the following layers; the bias gradient
+    is a signed sum over all pixels, which magnifies those differences relative to its size).
 """
 import numpy as np
 import pytest
@@ -74,11 +82,49 @@ def test_cnn5_forward_backward(dev, B, H, W):
     ref = ref_backward(layers_cpu, acts, dparam)
     from xdn.cnn import unflatten_params
     got = unflatten_params(net.grad.cpu())
+    errs = []
     for l in range(5):
         ew = rel_l2(got[l][0].numpy(), ref[l][0].numpy())
         eb = rel_l2(got[l][1].numpy(), ref[l][1].numpy())
         print(f"  cnn5 layer {l + 1} dW {ew:.2e} db {eb:.2e}", flush=True)
-        assert ew <= 5e-3 and eb <= 5e-3, l
+        errs.append((ew, eb))
+    for l, (ew, eb) in enumerate(errs):
+        tol = 5e-3 if l >= 3 else 2e-2
+        assert ew <= tol and eb <= tol, l
+    check_kernels_on_own_operands(net, imgs, layers_cpu, B, H, W)
+
+
+def _ws_act(net, i, B, H, W):
+    """bf16 tensor i of the cnn5 workspace (0-3 hidden activations, 4-5 gradient ping-pong) as float64 NCHW."""
+    act = (B * H * W * 32 * 2 + 255) // 256 * 256
+    raw = net.ws[i * act:i * act + B * H * W * 32 * 2].view(torch.bfloat16).view(B, H, W, 32)
+    return raw.cpu().to(torch.float64).permute(0, 3, 1, 2)
+
+
+def check_kernels_on_own_operands(net, imgs, layers_cpu, B, H, W):
+    """After backward: ping-pong slot 0 holds dY of layer 1, slot 1 dY of layer 2 (pre-activation
+    gradients), slot X[0] the layer-1 activation. Checks wgrad (image input and bf16 input) and
+    the masked dgrad of layer 2 on exactly the operands the kernels read."""
+    from xdn.cnn import unflatten_params
+    got = unflatten_params(net.grad.cpu())
+    x0 = bf(torch.cat(imgs, -1).double()).permute(0, 3, 1, 2)
+    a1, dy1, dy2 = _ws_act(net, 0, B, H, W), _ws_act(net, 4, B, H, W), _ws_act(net, 5, B, H, W)
+    for l, (x, dy) in enumerate(((x0, dy1), (a1, dy2))):
+        w = layers_cpu[l][0]
+        dw = torch.nn.grad.conv2d_weight(x, bf(w.double()).permute(3, 2, 0, 1).shape, dy, padding=1)
+        ew = rel_l2(got[l][0].numpy(), dw.permute(2, 3, 1, 0).numpy())
+        eb = rel_l2(got[l][1].numpy(), dy.sum((0, 2, 3)).numpy())
+        print(f"  wgrad layer {l + 1} on its own operands: dW {ew:.2e} db {eb:.2e}", flush=True)
+        assert ew <= 1e-5 and eb <= 1e-5, l
+    w1 = bf(layers_cpu[1][0].double()).permute(3, 2, 0, 1)
+    dx = torch.nn.grad.conv2d_input(a1.shape, w1, dy2, padding=1) * (a1 > 0)
+    ref = dx.to(torch.bfloat16).to(torch.float64)
+    diff = (ref != dy1)
+    ulp = (ref.abs() * 2.0 ** -7).clamp_min(1e-30)
+    far = ((ref - dy1).abs() > 1.01 * ulp) & diff
+    print(f"  dgrad layer 2 on its own operands: {int(diff.sum())} of {diff.numel()} differ by 1 ulp, "
+          f"{int(far.sum())} by more", flush=True)
+    assert int(far.sum()) == 0 and int(diff.sum()) <= 1e-3 * diff.numel()
 
 
 def test_cnn5_adam_and_pack_roundtrip(dev):

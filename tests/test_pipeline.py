@@ -64,7 +64,8 @@ def test_denoiser_stages_match_oracle(dev):
 
 
 def test_denoiser_trains(dev):
-    """The online loss goes down over a static sequence (the network learns the bandwidths)."""
+    """Repeated steps on one frame with the history reset before each step (every step is a
+    first frame, so the loss depends on the network alone): the online training lowers the loss."""
     from xdn import pipeline, synth
     H, W = 64, 64
     spec = synth.StreamSpec(H, W, seed=5, pan=(0.0, 0.0))
@@ -72,9 +73,14 @@ def test_denoiser_trains(dev):
     den = pipeline.Denoiser(H, W, device=dev, lr=1e-2)
     losses = []
     for _ in range(30):
+        den.prev_frame = None
+        for bufs in den.hist:
+            for t in bufs:
+                t.zero_()
         losses.append(float(den.step(frame)["loss"][0]))
+    print(f"  loss {losses[0]:.5f} -> {losses[-1]:.5f}", flush=True)
     assert np.all(np.isfinite(losses))
-    assert np.mean(losses[-5:]) < np.mean(losses[1:6])
+    assert np.mean(losses[-5:]) < 0.98 * np.mean(losses[:5])
 
 
 def test_batched_step_equals_single_streams(dev):
