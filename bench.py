@@ -126,6 +126,19 @@ def stage_times(den, frames, reps, torch):
     return out
 
 
+def profiled_traffic(kernel: str, H: int, W: int):
+    """HBM bytes per launch of `kernel` (read + write MB in profiles/step_1080p_counters.md, from the
+    copy-calibrated FETCH_SIZE / WRITE_SIZE counter passes), for the 1920x1080 workload only."""
+    path = os.path.join(ROOT, "profiles", "step_1080p_counters.md")
+    if (H, W) != (1080, 1920) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 5 and cells[1] == f"`{kernel}`":
+            return round((float(cells[3]) + float(cells[4])) * 1e6)
+    return None
+
+
 def cpu_baseline(torch, H, W):
     """Oracle (FP64, all host threads) + torch-CPU CNN step on a 256x256 crop, scaled to 1080p."""
     import numpy as np
@@ -216,14 +229,18 @@ def main():
     npx = H * W
     if dom == "filter_train":
         flops = ALGO["filter_train"]["flop_per_px"] * npx
-        name = "sf_train_tile (fused spatial filter + aggregation + backward)"
+        name = "sf_train_tile<5> (fused spatial filter + aggregation + backward)"
     else:
         flops = (ALGO["regression"]["flop_per_px"] + ALGO["recon"]["flop_per_px"]) * npx
         name = "cross_regression (variance + regression_rows + recon_tile)"
     achieved = flops / (stages[dom] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": name,
-                "note": "VALU-bound FP32 kernel; peak is the published FP32 rate"}
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": profiled_traffic(name.split(" ")[0], H, W), "kernel": name,
+                "algorithmic_flop_per_px": ALGO[dom]["flop_per_px"] if dom == "filter_train" else None,
+                "note": "VALU-bound FP32 kernel (no MFMA); peak is the published FP32 rate (the non-packed VALU "
+                        "rate is half of it); traffic = FETCH_SIZE + WRITE_SIZE bytes per launch from the committed "
+                        "counter profile, copy-calibrated"}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(torch, H, W)

@@ -262,7 +262,7 @@ __device__ __forceinline__ uint2 ld_tr(const uint16_t *p)
 // Wave wv accumulates strip rows 2wv and 2wv+1; the four waves' accumulators are summed through
 // LDS before the block writes its partial.
 template <int XK, int CI, int DYK>
-__global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
+__global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
 {
     constexpr int XS = CI > 16 ? 32 : 16;                // staged X channels (pixel stride)
     constexpr int DS = DYK == CONV_IN_BF16 ? 32 : 16;    // staged dY channels
@@ -290,82 +290,130 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad(WgBatch b)
         for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
     float db = 0.f;
 
-    for (int s = li; s < sx * sy; s += nblk)
-    {
+    // Software pipeline: the global loads of strip s + nblk are issued into registers before the
+    // MFMAs of strip s and land in LDS after them.
+    constexpr int NX = XK == CONV_IN_BF16 ? (WX_R * WX_C * 4 + 255) / 256 : (WX_R * WX_C + 255) / 256;
+    constexpr int ND = DYK == CONV_IN_BF16 ? WS_H * WS_W * 4 / 256 : 1;
+    u32x4 rx[NX], rd[ND];
+    float fx[XK == CONV_IN_BF16 ? 1 : NX][15];
+    float fd[6];
+    auto fetch = [&](int s) {
         const int y0 = (s / sx) * WS_H, x0 = (s % sx) * WS_W;
-        __syncthreads();  // the previous strip's reads are done
         // X: LDS pixel (r, col) <-> (y0 - 1 + r, x0 - 1 + col), zero outside the image
-        if constexpr (XK == CONV_IN_BF16)
+#pragma unroll
+        for (int u = 0; u < NX; ++u)
         {
-            const uint16_t *src = static_cast<const uint16_t *>(F.x[0]);
-            for (int i = tid; i < WX_R * WX_C * 4; i += 256)
+            if constexpr (XK == CONV_IN_BF16)
             {
+                const int i = tid + 256 * u;
                 const int pix = i >> 2, q = i & 3;
                 const int r = pix / WX_C, col = pix - r * WX_C;
                 const int gy = y0 - 1 + r, gx = x0 - 1 + col;
-                u32x4 v = {0u, 0u, 0u, 0u};
-                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                    v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * 32 + q * 8);
-                *reinterpret_cast<u32x4 *>(&xs[pix * XS + q * 8]) = v;
+                rx[u] = u32x4{0u, 0u, 0u, 0u};
+                if (i < WX_R * WX_C * 4 && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                    rx[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.x[0]) +
+                                                             ((size_t)gy * W + gx) * 32 + q * 8);
             }
-        }
-        else
-        {
-            for (int i = tid; i < WX_R * WX_C * 2; i += 256)
+            else
             {
-                const int pix = i >> 1, q = i & 1;  // q: channels 0-7 or 8-15 of the 16 (15 real + 0)
+                // one pixel per item: the 15 channels of the five 3-channel images
+                const int pix = tid + 256 * u;
                 const int r = pix / WX_C, col = pix - r * WX_C;
                 const int gy = y0 - 1 + r, gx = x0 - 1 + col;
-                float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+#pragma unroll
+                for (int j = 0; j < 15; ++j) fx[u][j] = 0.f;
+                if (pix < WX_R * WX_C && gy >= 0 && gy < H && gx >= 0 && gx < W)
                 {
                     const size_t p = (size_t)gy * W + gx;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j)
+                    for (int im = 0; im < 5; ++im)
                     {
-                        const int k = q * 8 + j;  // channel of the concatenated 15-channel input
-                        if (k < 15) v[j] = static_cast<const float *>(F.x[k / 3])[3 * p + k % 3];
+                        const float *sp = static_cast<const float *>(F.x[im]) + 3 * p;
+                        fx[u][3 * im] = sp[0]; fx[u][3 * im + 1] = sp[1]; fx[u][3 * im + 2] = sp[2];
                     }
                 }
-                u32x4 a;
-                a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = pack2(v[6], v[7]);
-                *reinterpret_cast<u32x4 *>(&xs[pix * XS + q * 8]) = a;
             }
         }
         // dY: LDS pixel (r, col) <-> (y0 + r, x0 + col)
-        if constexpr (DYK == CONV_IN_BF16)
+#pragma unroll
+        for (int u = 0; u < ND; ++u)
         {
-            const uint16_t *src = static_cast<const uint16_t *>(F.dy);
-            for (int i = tid; i < WS_H * WS_W * 4; i += 256)
+            if constexpr (DYK == CONV_IN_BF16)
             {
+                const int i = tid + 256 * u;
                 const int pix = i >> 2, q = i & 3;
                 const int r = pix / WS_W, col = pix - r * WS_W;
                 const int gy = y0 + r, gx = x0 + col;
-                u32x4 v = {0u, 0u, 0u, 0u};
-                if (gy < H && gx < W) v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * 32 + q * 8);
-                *reinterpret_cast<u32x4 *>(&ds[pix * DS + q * 8]) = v;
+                rd[u] = u32x4{0u, 0u, 0u, 0u};
+                if (gy < H && gx < W)
+                    rd[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.dy) +
+                                                             ((size_t)gy * W + gx) * 32 + q * 8);
             }
-        }
-        else
-        {
-            for (int pix = tid; pix < WS_H * WS_W; pix += 256)
+            else
             {
-                const int r = pix / WS_W, col = pix - r * WS_W;
+                const int r = tid / WS_W, col = tid - r * WS_W;
                 const int gy = y0 + r, gx = x0 + col;
-                float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < 6; ++k) fd[k] = 0.f;
                 if (gy < H && gx < W)
                 {
                     const float *sp = static_cast<const float *>(F.dy) + 6 * ((size_t)gy * W + gx);
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) v[k] = sp[k];
+                    for (int k = 0; k < 6; ++k) fd[k] = sp[k];
                 }
-                u32x4 a, z = {0u, 0u, 0u, 0u};
-                a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = 0u;
-                *reinterpret_cast<u32x4 *>(&ds[pix * DS]) = a;
-                *reinterpret_cast<u32x4 *>(&ds[pix * DS + 8]) = z;
             }
         }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int u = 0; u < NX; ++u)
+        {
+            if constexpr (XK == CONV_IN_BF16)
+            {
+                const int i = tid + 256 * u;
+                if (i < WX_R * WX_C * 4) *reinterpret_cast<u32x4 *>(&xs[(i >> 2) * XS + (i & 3) * 8]) = rx[u];
+            }
+            else
+            {
+                const int pix = tid + 256 * u;
+                if (pix < WX_R * WX_C)
+                {
+                    u32x4 a, c;
+                    a.x = pack2(fx[u][0], fx[u][1]); a.y = pack2(fx[u][2], fx[u][3]);
+                    a.z = pack2(fx[u][4], fx[u][5]); a.w = pack2(fx[u][6], fx[u][7]);
+                    c.x = pack2(fx[u][8], fx[u][9]); c.y = pack2(fx[u][10], fx[u][11]);
+                    c.z = pack2(fx[u][12], fx[u][13]); c.w = pack2(fx[u][14], 0.f);
+                    *reinterpret_cast<u32x4 *>(&xs[pix * XS]) = a;
+                    *reinterpret_cast<u32x4 *>(&xs[pix * XS + 8]) = c;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < ND; ++u)
+        {
+            if constexpr (DYK == CONV_IN_BF16)
+            {
+                const int i = tid + 256 * u;
+                *reinterpret_cast<u32x4 *>(&ds[(i >> 2) * DS + (i & 3) * 8]) = rd[u];
+            }
+            else
+            {
+                u32x4 a, z = {0u, 0u, 0u, 0u};
+                a.x = pack2(fd[0], fd[1]); a.y = pack2(fd[2], fd[3]); a.z = pack2(fd[4], fd[5]); a.w = 0u;
+                *reinterpret_cast<u32x4 *>(&ds[tid * DS]) = a;
+                *reinterpret_cast<u32x4 *>(&ds[tid * DS + 8]) = z;
+            }
+        }
+    };
+
+    const int nstrip = sx * sy;
+    if (li < nstrip) fetch(li);
+    for (int s = li; s < nstrip; s += nblk)
+    {
+        __syncthreads();  // the previous strip's reads are done
+        commit();
         __syncthreads();
+        if (s + nblk < nstrip) fetch(s + nblk);
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {
@@ -546,8 +594,8 @@ hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st)
 int wgrad_blocks(int H, int W)
 {
     const int strips = div_up(W, WS_W) * div_up(H, WS_H);
-    // ~10 strips per block, at least enough blocks to cover the chip
-    int nb = div_up(strips, 10);
+    // ~16 strips per block (about two blocks per CU at 1080p): the strip loop is software-pipelined
+    int nb = div_up(strips, 16);
     if (nb > strips) nb = strips;
     return nb < 1 ? 1 : nb;
 }

@@ -37,6 +37,9 @@ __device__ __forceinline__ bool reproj_valid(int W, int H, int px, int py, float
     return true;
 }
 
+// FNL > 0: the buffer list is FNL buffers of FD channels, known at compile time (the training
+// step reprojects three 3-channel buffers); FNL == 0 reads the list shape from the batch.
+template <int FNL, int FD>
 __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
 {
     const int tile = xcd_tile(blockIdx.x, b.map.total);
@@ -46,7 +49,7 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
     const int cx = L.tx * RP_TX + (threadIdx.x & (RP_TX - 1)), cy = L.ty * RP_TY + threadIdx.x / RP_TX;
     if (cx >= W || cy >= H) return;
     const size_t c = (size_t)cy * W + cx;
-    const int nl = b.n_list;
+    const int nl = FNL > 0 ? FNL : b.n_list;
     float acc[XDN_MAX_LIST][4];
 #pragma unroll
     for (int l = 0; l < XDN_MAX_LIST; ++l)
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
                 for (int l = 0; l < XDN_MAX_LIST; ++l)
                 {
                     if (l >= nl) break;
-                    const int d = b.dims[l];
+                    const int d = FNL > 0 ? FD : b.dims[l];
                     const float *src = F.in[l] + li * d;
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
                     for (int l = 0; l < XDN_MAX_LIST; ++l)
                     {
                         if (l >= nl) break;
-                        const int d = b.dims[l];
+                        const int d = FNL > 0 ? FD : b.dims[l];
                         const float *src = F.in[l] + qi * d;
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
     for (int l = 0; l < XDN_MAX_LIST; ++l)
     {
         if (l >= nl) break;
-        const int d = b.dims[l];
+        const int d = FNL > 0 ? FD : b.dims[l];
         float *dst = F.out[l] + c * d;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -175,7 +178,12 @@ hipError_t launch_reproject(int n, const RpFrame *fr, int n_list, const int *dim
     }
     b.map = make_tilemap(n, tx, ty);
     if (b.map.total == 0) return hipSuccess;
-    hipLaunchKernelGGL(reproject_kernel, dim3(b.map.total), dim3(256), 0, st, b);
+    bool three3 = n_list == 3;
+    for (int l = 0; l < n_list; ++l) three3 = three3 && dims[l] == 3;
+    if (three3)
+        hipLaunchKernelGGL((reproject_kernel<3, 3>), dim3(b.map.total), dim3(256), 0, st, b);
+    else
+        hipLaunchKernelGGL((reproject_kernel<0, 0>), dim3(b.map.total), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 

@@ -17,7 +17,7 @@ enum {
 enum { CONV_L1_WGRAD = 0, CONV_MID_WGRAD = 1, CONV_L5_WGRAD = 2 };
 
 constexpr int WG_PART = 9 * 1024 + 32;  // one block's partial: [9][32][32] weights + [32] bias
-constexpr int WG_SPLITS = 16;
+constexpr int WG_SPLITS = 32;
 
 struct ConvFrame {
     const void *in[5];       // see the kernel kinds

@@ -8,11 +8,9 @@ accumulate in FP32.
 Two bars:
   * per kernel, fed with the kernels' own bf16 operands read back from the workspace (so no
     rounding difference propagates): weight and bias gradients <= 1e-5 relative L2, data gradient
-    equal to the bf16-rounded float64 result except for 1-ulp rounding ties (<= 1e-3 of elements);
-  * end to end (the chain of 4 bf16-rounded data gradients from the float64 chain of its own):
-    network output <= 2e-3, gradients of layers 4-5 <= 5e-3, of layers 1-3 <= 2e-2 (a value that
-    rounds differently in one layer is carried through the following layers; the bias gradient
-    is a signed sum over all pixels, which magnifies those differences relative to its size).
+    equal to the bf16-rounded float64 result except for rare rounding-boundary cases;
+  * end to end: network output <= 2e-3 against the float64 forward; the five weight and bias
+    gradients <= 5e-3 against the float64 backward chain run on the kernels' own activations.
 """
 import numpy as np
 import pytest
@@ -79,7 +77,11 @@ def test_cnn5_forward_backward(dev, B, H, W):
     dparam = torch.randn(B, H, W, 6) * 1e-3
     net.backward(dparam.to(dev).contiguous())
     torch.cuda.synchronize()
-    ref = ref_backward(layers_cpu, acts, dparam)
+    # the reference backward runs on the kernels' own bf16 activations: an activation that rounds
+    # to zero on one side only would flip a ReLU mask, and a bias gradient (a signed sum over all
+    # pixels) moves by ~1/sqrt(pixels) per flipped element
+    acts_gpu = [acts[0]] + [_ws_act(net, i, B, H, W) for i in range(4)]
+    ref = ref_backward(layers_cpu, acts_gpu, dparam)
     from xdn.cnn import unflatten_params
     got = unflatten_params(net.grad.cpu())
     errs = []
@@ -89,8 +91,7 @@ def test_cnn5_forward_backward(dev, B, H, W):
         print(f"  cnn5 layer {l + 1} dW {ew:.2e} db {eb:.2e}", flush=True)
         errs.append((ew, eb))
     for l, (ew, eb) in enumerate(errs):
-        tol = 5e-3 if l >= 3 else 2e-2
-        assert ew <= tol and eb <= tol, l
+        assert ew <= 5e-3 and eb <= 5e-3, l
     check_kernels_on_own_operands(net, imgs, layers_cpu, B, H, W)
 
 
@@ -124,7 +125,9 @@ def check_kernels_on_own_operands(net, imgs, layers_cpu, B, H, W):
     far = ((ref - dy1).abs() > 1.01 * ulp) & diff
     print(f"  dgrad layer 2 on its own operands: {int(diff.sum())} of {diff.numel()} differ by 1 ulp, "
           f"{int(far.sum())} by more", flush=True)
-    assert int(far.sum()) == 0 and int(diff.sum()) <= 1e-3 * diff.numel()
+    # FP32 accumulation (with cancellation) against the exact sum can move a result across more
+    # than one bf16 rounding boundary now and then
+    assert int(far.sum()) <= 2 + 1e-5 * diff.numel() and int(diff.sum()) <= 1e-3 * diff.numel()
 
 
 def test_cnn5_adam_and_pack_roundtrip(dev):
