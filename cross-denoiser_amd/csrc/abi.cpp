@@ -536,16 +536,14 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
     Cnn5Ws ws(workspace, batch, height, width);
     const size_t N = px(height, width);
     const char *pk = static_cast<const char *>(packed);
-    const int nb = xdn::wgrad_blocks(height, width);
     std::vector<xdn::ConvFrame> cf(batch);
     std::vector<xdn::WgFrame> wf(batch);
-    xdn::WgFrame all{};
-    all.part = ws.part;
-    all.nblk = ws.nblk_total;
+    std::vector<xdn::RedJob> jobs;
     int cur = 0;  // dY ping-pong index holding the gradient of the current layer's output
     for (int l = 4; l >= 0; --l)
     {
         // weight gradient of layer l: x = input of layer l, dy = gradient of its pre-activation
+        const int nb = ws.nblk[l];
         for (int b = 0; b < batch; ++b)
         {
             xdn::WgFrame &f = wf[b];
@@ -558,17 +556,15 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
             else
                 f.x[0] = ws.X[l - 1] + o32;
             f.dy = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6) : static_cast<const void *>(ws.dY[cur] + o32);
-            f.part = ws.part + (size_t)b * nb * xdn::WG_PART;
+            f.part = ws.part[l] + (size_t)b * nb * xdn::WG_PART;
             f.nblk = nb;
             f.H = height;
             f.W = width;
         }
         const int wk = l == 0 ? xdn::CONV_L1_WGRAD : (l == 4 ? xdn::CONV_L5_WGRAD : xdn::CONV_MID_WGRAD);
         if (int e = hip_check(xdn::launch_wgrad(wk, batch, wf.data(), S(stream)), "cnn5 wgrad")) return e;
-        float *g = grad + kLay.w_off[l];
-        if (int e = hip_check(xdn::launch_wgrad_reduce(1, &all, &ws.stage, &g, kCnn5[l].ci, kCnn5[l].co, S(stream)),
-                              "cnn5 wgrad reduce"))
-            return e;
+        // the B images' partials of this layer are reduced together into the one gradient
+        jobs.push_back(xdn::RedJob{ws.part[l], ws.stage[l], grad + kLay.w_off[l], batch * nb, kCnn5[l].ci, kCnn5[l].co});
         if (l == 0) break;
         // data gradient into the output of layer l-1, masked by its ReLU
         for (int b = 0; b < batch; ++b)
@@ -587,7 +583,7 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
         if (int e = hip_check(xdn::launch_conv(dk, batch, cf.data(), S(stream)), "cnn5 dgrad")) return e;
         cur = 1 - cur;
     }
-    return XDN_OK;
+    return hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), S(stream)), "cnn5 wgrad reduce");
 }
 
 }  // extern "C"

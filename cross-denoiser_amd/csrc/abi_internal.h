@@ -63,11 +63,13 @@ struct Cnn5Layout {
 };
 inline const Cnn5Layout kLay;
 
-// Workspace of cnn5 for B images of H x W: hidden activations, gradient ping-pong, partials.
+// Workspace of cnn5 for B images of H x W: hidden activations, gradient ping-pong, and per layer
+// the weight-gradient partials of every block and the reduction stage (all five layers are
+// reduced together after the backward pass).
 struct Cnn5Ws {
     uint16_t *X[4], *dY[2];
-    float *part, *stage;
-    int nblk_total;
+    float *part[5], *stage[5];
+    int nblk[5];  // weight-gradient blocks per image and layer
     size_t bytes;
     Cnn5Ws(void *base, int B, int H, int W)
     {
@@ -76,11 +78,14 @@ struct Cnn5Ws {
         size_t o = 0;
         for (int i = 0; i < 4; ++i) X[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
         for (int i = 0; i < 2; ++i) dY[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
-        nblk_total = B * xdn::wgrad_blocks(H, W);
-        part = reinterpret_cast<float *>(p + o);
-        o += al256((size_t)nblk_total * xdn::WG_PART * 4);
-        stage = reinterpret_cast<float *>(p + o);
-        o += al256((size_t)xdn::WG_SPLITS * xdn::WG_PART * 4);
+        for (int l = 0; l < 5; ++l)
+        {
+            nblk[l] = xdn::wgrad_blocks(H, W);
+            part[l] = reinterpret_cast<float *>(p + o);
+            o += al256((size_t)B * nblk[l] * xdn::WG_PART * 4);
+            stage[l] = reinterpret_cast<float *>(p + o);
+            o += al256((size_t)xdn::WG_SPLITS * xdn::WG_PART * 4);
+        }
         bytes = o;
     }
 };

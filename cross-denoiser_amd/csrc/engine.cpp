@@ -4,7 +4,8 @@
 // (scripts/main.py:250-313) is ONE launch over all slots -- the frame-batch tile maps of the
 // kernels handle different resolutions in the same launch -- so n streams cost as many launches
 // as one: cross-regression (3), reprojection, erosion + screening, CNN forward (5), fused filter +
-// loss + backward, loss sums, CNN backward (5 wgrad + 5x2 reductions + 4 dgrad), Adam, pack.
+// loss + backward, loss sums, CNN backward (5 weight gradients, 4 data gradients, 2 reductions),
+// Adam, pack.
 // Every slot owns its history, network, Adam moments and scratch; nothing is shared.
 #include <math.h>
 
@@ -219,11 +220,12 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
         if (int e = hip_check(xdn::launch_filter_train(n, fr.data(), kFilterWin, st), "filter_train")) return e;
         if (int e = hip_check(xdn::launch_sums(n, parts.data(), cnt.data(), scale.data(), outp.data(), st), "loss")) return e;
     }
-    // 6. network backward: per layer one weight-gradient launch, per-slot reductions, one data-gradient launch
+    // 6. network backward: per layer one weight-gradient launch and one data-gradient launch; the
+    //    per-block weight-gradient partials of every (slot, layer) are reduced by one launch pair
     {
         std::vector<xdn::WgFrame> wf(n);
         std::vector<xdn::ConvFrame> cf(n);
-        std::vector<float *> stage(n), grad(n);
+        std::vector<xdn::RedJob> jobs;
         int cur = 0;
         for (int l = 4; l >= 0; --l)
         {
@@ -239,17 +241,14 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
                 else
                     f.x[0] = cw[i].X[l - 1];
                 f.dy = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
-                f.part = cw[i].part;
-                f.nblk = cw[i].nblk_total;
+                f.part = cw[i].part[l];
+                f.nblk = cw[i].nblk[l];
                 f.H = s.height; f.W = s.width;
-                stage[i] = cw[i].stage;
-                grad[i] = s.grad + kLay.w_off[l];
+                jobs.push_back(xdn::RedJob{cw[i].part[l], cw[i].stage[l], s.grad + kLay.w_off[l], cw[i].nblk[l],
+                                           kCnn5[l].ci, kCnn5[l].co});
             }
             const int wk = l == 0 ? xdn::CONV_L1_WGRAD : (l == 4 ? xdn::CONV_L5_WGRAD : xdn::CONV_MID_WGRAD);
             if (int e = hip_check(xdn::launch_wgrad(wk, n, wf.data(), st), "cnn wgrad")) return e;
-            if (int e = hip_check(xdn::launch_wgrad_reduce(n, wf.data(), stage.data(), grad.data(), kCnn5[l].ci,
-                                                           kCnn5[l].co, st), "cnn wgrad reduce"))
-                return e;
             if (l == 0) break;
             for (int i = 0; i < n; ++i)
             {
@@ -266,6 +265,7 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
             if (int e = hip_check(xdn::launch_conv(dk, n, cf.data(), st), "cnn dgrad")) return e;
             cur = 1 - cur;
         }
+        if (int e = hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), st), "cnn wgrad reduce")) return e;
     }
     // 7. Keras Adam on every slot's parameters, then rebuild the bf16 fragments
     {

@@ -56,102 +56,146 @@ static_assert(sizeof(ConvBatch) <= 4096, "kernel argument too large");
 
 constexpr int CT_W = 32, CT_H = 16;
 
+// Persistent and software-pipelined: a block walks the tiles it, it + gridDim.x, ... (gridDim.x a
+// multiple of 8, so a block stays on one XCD and xcd_tile keeps each XCD on a contiguous run of
+// tiles). The global loads of the next tile are issued into registers before the MFMAs of the
+// current one and land in LDS at the top of the next iteration. The kernel fragments of the tile's
+// frame sit in LDS (reloaded when the frame changes), which keeps two blocks per CU.
 template <int IN, int CIN, int EPI>
-__global__ __launch_bounds__(256) void conv3x3(ConvBatch b)
+__global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
 {
     constexpr int PS = CIN + 8;  // LDS pixel stride in bf16 (20 or 12 dwords: conflict-free b128 reads)
     constexpr int LW = CT_W + 2, LH = CT_H + 2;
     constexpr int KC = CIN / 16;
     constexpr int IN_ELEMS = LH * LW * PS, OUT_ELEMS = CT_H * CT_W * 40;  // input tile / epilogue tile
     __shared__ __attribute__((aligned(16))) uint16_t lds[IN_ELEMS > OUT_ELEMS ? IN_ELEMS : OUT_ELEMS];
-    const int tile = xcd_tile(blockIdx.x, b.map.total);
-    const TileLoc L = locate(b.map, tile);
+    __shared__ u32x4 wl[9 * KC * 64];  // kernel fragments of the current frame
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
+    const int total = b.map.total;
+
+    // ---- input tile (+1 px halo, zero outside the image): fetch to registers, commit to LDS ----
+    constexpr int NV = IN == CONV_IN_BF16 ? (LH * LW * (CIN / 8) + 255) / 256 : (LH * LW + 255) / 256;
+    constexpr int NF = IN == CONV_IN_IMAGES ? 15 : 6;
+    u32x4 rv[IN == CONV_IN_BF16 ? NV : 1];
+    float fv[IN == CONV_IN_BF16 ? 1 : NV][NF];
+    auto fetch = [&](int it) {
+        const TileLoc T = locate(b.map, xcd_tile(it, total));
+        const ConvFrame &G = b.f[T.f];
+        const int gH = G.H, gW = G.W, tx0 = T.tx * CT_W, ty0 = T.ty * CT_H;
+#pragma unroll
+        for (int u = 0; u < NV; ++u)
+        {
+            if constexpr (IN == CONV_IN_BF16)
+            {
+                const int i = tid + 256 * u;
+                const int pix = i / (CIN / 8), q = i - pix * (CIN / 8);
+                const int ly = pix / LW, lx = pix - ly * LW;
+                const int gy = ty0 + ly - 1, gx = tx0 + lx - 1;
+                rv[u] = u32x4{0u, 0u, 0u, 0u};
+                if (i < LH * LW * (CIN / 8) && gy >= 0 && gy < gH && gx >= 0 && gx < gW)
+                    rv[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(G.in[0]) +
+                                                             ((size_t)gy * gW + gx) * CIN + q * 8);
+            }
+            else
+            {
+                const int pix = tid + 256 * u;
+                const int ly = pix / LW, lx = pix - ly * LW;
+                const int gy = ty0 + ly - 1, gx = tx0 + lx - 1;
+#pragma unroll
+                for (int k = 0; k < NF; ++k) fv[u][k] = 0.f;
+                if (pix < LH * LW && gy >= 0 && gy < gH && gx >= 0 && gx < gW)
+                {
+                    const size_t p = (size_t)gy * gW + gx;
+                    if constexpr (IN == CONV_IN_IMAGES)
+                    {
+#pragma unroll
+                        for (int im = 0; im < 5; ++im)
+                        {
+                            const float *sp = static_cast<const float *>(G.in[im]) + 3 * p;
+                            fv[u][3 * im] = sp[0]; fv[u][3 * im + 1] = sp[1]; fv[u][3 * im + 2] = sp[2];
+                        }
+                    }
+                    else
+                    {
+                        const float *sp = static_cast<const float *>(G.in[0]) + 6 * p;
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) fv[u][k] = sp[k];
+                    }
+                }
+            }
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int u = 0; u < NV; ++u)
+        {
+            if constexpr (IN == CONV_IN_BF16)
+            {
+                const int i = tid + 256 * u;
+                const int pix = i / (CIN / 8), q = i - pix * (CIN / 8);
+                if (i < LH * LW * (CIN / 8)) *reinterpret_cast<u32x4 *>(&lds[pix * PS + q * 8]) = rv[u];
+            }
+            else
+            {
+                const int pix = tid + 256 * u;
+                if (pix < LH * LW)
+                {
+                    float v[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) v[k] = k < NF ? fv[u][k] : 0.f;
+                    u32x4 a, c;
+                    a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = pack2(v[6], v[7]);
+                    c.x = pack2(v[8], v[9]); c.y = pack2(v[10], v[11]); c.z = pack2(v[12], v[13]);
+                    c.w = pack2(v[14], v[15]);
+                    *reinterpret_cast<u32x4 *>(&lds[pix * PS]) = a;
+                    *reinterpret_cast<u32x4 *>(&lds[pix * PS + 8]) = c;
+                }
+            }
+        }
+    };
+
+    float bias = 0.f;
+    int cur_f = -1;
+    if ((int)blockIdx.x < total) fetch(blockIdx.x);
+    for (int it = blockIdx.x; it < total; it += gridDim.x)
+    {
+    const TileLoc L = locate(b.map, xcd_tile(it, total));
     const ConvFrame &F = b.f[L.f];
     const int H = F.H, W = F.W;
     const int x0 = L.tx * CT_W, y0 = L.ty * CT_H;
-    const int tid = threadIdx.x;
-
-    // ---- stage the input tile (+1 px halo, zero outside the image) ----
-    if constexpr (IN == CONV_IN_BF16)
+    __syncthreads();  // the previous tile's epilogue and MFMAs are done with LDS
+    if (L.f != cur_f)
     {
-        const uint16_t *src = static_cast<const uint16_t *>(F.in[0]);
-        for (int i = tid; i < LH * LW * (CIN / 8); i += 256)
-        {
-            const int pix = i / (CIN / 8), q = i - pix * (CIN / 8);
-            const int ly = pix / LW, lx = pix - ly * LW;
-            const int gy = y0 + ly - 1, gx = x0 + lx - 1;
-            u32x4 v = {0u, 0u, 0u, 0u};
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                v = *reinterpret_cast<const u32x4 *>(src + ((size_t)gy * W + gx) * CIN + q * 8);
-            *reinterpret_cast<u32x4 *>(&lds[pix * PS + q * 8]) = v;
-        }
+        for (int e = tid; e < 9 * KC * 64; e += 256) wl[e] = static_cast<const u32x4 *>(F.wfrag)[e];
+        bias = (EPI == CONV_EPI_MASK) ? 0.f : F.bias[px];
+        cur_f = L.f;
     }
-    else
-    {
-        for (int pix = tid; pix < LH * LW; pix += 256)
-        {
-            const int ly = pix / LW, lx = pix - ly * LW;
-            const int gy = y0 + ly - 1, gx = x0 + lx - 1;
-            float v[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) v[k] = 0.f;
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-            {
-                const size_t p = (size_t)gy * W + gx;
-                if constexpr (IN == CONV_IN_IMAGES)
-                {
-#pragma unroll
-                    for (int im = 0; im < 5; ++im)
-                    {
-                        const float *s = static_cast<const float *>(F.in[im]) + 3 * p;
-                        v[3 * im] = s[0]; v[3 * im + 1] = s[1]; v[3 * im + 2] = s[2];
-                    }
-                }
-                else
-                {
-                    const float *s = static_cast<const float *>(F.in[0]) + 6 * p;
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) v[k] = s[k];
-                }
-            }
-            u32x4 a, c;
-            a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = pack2(v[6], v[7]);
-            c.x = pack2(v[8], v[9]); c.y = pack2(v[10], v[11]); c.z = pack2(v[12], v[13]); c.w = pack2(v[14], v[15]);
-            *reinterpret_cast<u32x4 *>(&lds[pix * PS]) = a;
-            *reinterpret_cast<u32x4 *>(&lds[pix * PS + 8]) = c;
-        }
-    }
-    // ---- kernel fragments (B operands) and bias ----
-    const int lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
-    u32x4 wf[9][KC];
-#pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-        for (int kc = 0; kc < KC; ++kc) wf[t][kc] = static_cast<const u32x4 *>(F.wfrag)[(t * KC + kc) * 64 + lane];
-    const float bias = (EPI == CONV_EPI_MASK) ? 0.f : F.bias[px];
+    commit();
     __syncthreads();
+    if (it + (int)gridDim.x < total) fetch(it + gridDim.x);
 
     f32x16 acc[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc[r][k] = 0.f;
+    // one (tap, channel chunk) per step; steps are not unrolled so only a step's five fragments are
+    // live, which keeps two blocks per CU
+#pragma unroll 2
+    for (int t = 0; t < 9 * KC; ++t)
+    {
+        const int tap = t / KC, kc = t - tap * KC, dy = tap / 3, dx = tap - dy * 3;
+        const u32x4 bw = wl[t * 64 + lane];
+        const uint16_t *ap = &lds[((4 * wv + dy) * LW + px + dx) * PS + kc * 16 + 8 * h];
 #pragma unroll
-    for (int ir = 0; ir < 6; ++ir)  // LDS row 4*wv + ir feeds output rows ir-2 .. ir (kernel row ir - orow)
-#pragma unroll
-        for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-            for (int kc = 0; kc < KC; ++kc)
-            {
-                const u32x4 a = *reinterpret_cast<const u32x4 *>(&lds[((4 * wv + ir) * LW + px + dx) * PS + kc * 16 + 8 * h]);
-#pragma unroll
-                for (int orow = 0; orow < 4; ++orow)
-                {
-                    const int dy = ir - orow;
-                    if (dy < 0 || dy > 2) continue;
-                    acc[orow] = mfma(a, wf[dy * 3 + dx][kc], acc[orow]);
-                }
-            }
+        for (int orow = 0; orow < 4; ++orow)
+        {
+            // output row 4wv + orow reads LDS row 4wv + orow + dy (kernel row dy)
+            const u32x4 a = *reinterpret_cast<const u32x4 *>(ap + orow * LW * PS);
+            acc[orow] = mfma(a, bw, acc[orow]);
+        }
+    }
 
     // ---- epilogue: lane owns output channel px of 16 pixels x = 8(k/4) + 4h + k%4 per row. The
     // tile is transposed through LDS (the input tile is dead after the barrier) so every global
@@ -228,6 +272,7 @@ __global__ __launch_bounds__(256) void conv3x3(ConvBatch b)
             *reinterpret_cast<u32x4 *>(static_cast<uint16_t *>(F.out) + o) = v;
         }
     }
+    }  // tile loop
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -456,47 +501,46 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
     if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
 }
 
-// Stage 1: stage[f][split][e] = sum of partials split*P/S .. ; stage 2 writes the flat gradient
-// of the layer ([9][CI][CO] then [CO], unpadded).
+// Two-stage deterministic reduction of the per-block partials, batched over jobs (a job is one
+// layer of one frame). Stage 1: stage[j][split][e] = sum of the partials split*P/S .. of job j;
+// stage 2 writes the flat gradient of the layer ([9][CI][CO] then [CO], unpadded).
 struct RedBatch {
     int n, splits;
-    const float *part[XDN_MAX_FRAMES];
-    int nblk[XDN_MAX_FRAMES];
-    float *stage[XDN_MAX_FRAMES];
-    float *grad[XDN_MAX_FRAMES];
-    int ci, co;
+    RedJob j[WG_MAX_JOBS];
 };
+static_assert(sizeof(RedBatch) <= 4096, "kernel argument too large");
 
 __global__ __launch_bounds__(256) void wgrad_reduce1(RedBatch b)
 {
-    const int f = blockIdx.z, sp = blockIdx.y;
+    const RedJob &J = b.j[blockIdx.z];
+    const int sp = blockIdx.y;
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= WG_PART) return;
-    const int nb = b.nblk[f];
+    const int nb = J.nblk;
     const int lo = (int)((long)nb * sp / b.splits), hi = (int)((long)nb * (sp + 1) / b.splits);
     float s = 0.f;
-    const float *p = b.part[f] + e;
+    const float *p = J.part + e;
     for (int i = lo; i < hi; ++i) s += p[(size_t)i * WG_PART];
-    b.stage[f][sp * WG_PART + e] = s;
+    J.stage[sp * WG_PART + e] = s;
 }
 
 __global__ __launch_bounds__(256) void wgrad_reduce2(RedBatch b)
 {
-    const int f = blockIdx.y;
+    const RedJob &J = b.j[blockIdx.y];
     const int e = blockIdx.x * 256 + threadIdx.x;
-    const int nw = 9 * b.ci * b.co;
-    if (e >= nw + b.co) return;
+    const int nw = 9 * J.ci * J.co;
+    if (e >= nw + J.co) return;
     int src;
     if (e < nw)
     {
-        const int t = e / (b.ci * b.co), r = e - t * b.ci * b.co, ci = r / b.co, co = r - ci * b.co;
+        const int t = e / (J.ci * J.co), r = e - t * J.ci * J.co, ci = r / J.co, co = r - ci * J.co;
         src = t * 1024 + ci * 32 + co;
     }
     else
         src = 9 * 1024 + (e - nw);
     float s = 0.f;
-    for (int sp = 0; sp < b.splits; ++sp) s += b.stage[f][sp * WG_PART + src];
-    b.grad[f][e] = s;
+    for (int sp = 0; sp < b.splits; ++sp) s += J.stage[sp * WG_PART + src];
+    J.grad[e] = s;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -574,7 +618,10 @@ static hipError_t launch_conv_t(int n, const ConvFrame *fr, hipStream_t st)
     }
     b.map = make_tilemap(n, tx, ty);
     if (b.map.total == 0) return hipSuccess;
-    hipLaunchKernelGGL((conv3x3<IN, CIN, EPI>), dim3(b.map.total), dim3(256), 0, st, b);
+    // persistent grid: two resident blocks per CU, a multiple of the 8 XCDs
+    constexpr int kGrid = 256 * 2;
+    const int grid = b.map.total < kGrid ? b.map.total : kGrid;
+    hipLaunchKernelGGL((conv3x3<IN, CIN, EPI>), dim3(grid), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 
@@ -629,24 +676,22 @@ hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st)
     }
 }
 
-hipError_t launch_wgrad_reduce(int n, const WgFrame *fr, float *const *stage, float *const *grad, int ci, int co,
-                               hipStream_t st)
+hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st)
 {
+    if (n == 0) return hipSuccess;
+    if (n > WG_MAX_JOBS) return hipErrorInvalidValue;
     RedBatch b{};
     b.n = n;
     b.splits = WG_SPLITS;
-    b.ci = ci;
-    b.co = co;
+    int maxw = 0;
     for (int i = 0; i < n; ++i)
     {
-        b.part[i] = fr[i].part;
-        b.nblk[i] = fr[i].nblk;
-        b.stage[i] = stage[i];
-        b.grad[i] = grad[i];
+        b.j[i] = jobs[i];
+        const int w = 9 * jobs[i].ci * jobs[i].co + jobs[i].co;
+        maxw = w > maxw ? w : maxw;
     }
-    if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(wgrad_reduce1, dim3(div_up(WG_PART, 256), WG_SPLITS, n), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(wgrad_reduce2, dim3(div_up(9 * ci * co + co, 256), n), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(wgrad_reduce2, dim3(div_up(maxw, 256), n), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "xdn_common.h"
+
 namespace xdn {
 
 enum { CONV_IN_BF16 = 0, CONV_IN_IMAGES = 1, CONV_IN_F32_6 = 2 };
@@ -44,8 +46,15 @@ struct PackJob {
 
 hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st);
 hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st);
-hipError_t launch_wgrad_reduce(int n, const WgFrame *fr, float *const *stage, float *const *grad, int ci, int co,
-                               hipStream_t st);
+// One reduction job: the nblk partials of one layer of one frame -> its flat gradient.
+struct RedJob {
+    const float *part;  // [nblk][WG_PART]
+    float *stage;       // [WG_SPLITS][WG_PART]
+    float *grad;        // [9][ci][co] then [co]
+    int nblk, ci, co;
+};
+constexpr int WG_MAX_JOBS = 5 * XDN_MAX_FRAMES;
+hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st);
 hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st);
 int wgrad_blocks(int H, int W);
 
