@@ -307,9 +307,10 @@ def main(argv=None):
     ap.add_argument("--mode", choices=("batched", "plain", "graphs", "all"), default="all")
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--fps", type=float, default=60.0, help="mean arrival rate per stream (configuration 3)")
     a = ap.parse_args(argv)
     dev = torch.device("cuda:0")
-    reqs = config3(a.frames, seed=a.seed) if a.config == 3 else config4(a.frames, seed=a.seed)
+    reqs = config3(a.frames, fps=a.fps, seed=a.seed) if a.config == 3 else config4(a.frames, seed=a.seed)
     warm_up(dev, [(r.height, r.width) for r in reqs])
     modes = ("plain", "graphs", "batched") if a.mode == "all" else (a.mode,)
     for m in modes:
@@ -320,7 +321,7 @@ def main(argv=None):
                      max_inflight=1 if m == "plain" else 2)
         res = srv.run(reqs)
         res.update({"config": a.config, "mode": m, "streams": len(reqs), "frames_per_stream": a.frames,
-                    "budget_gb": a.budget_gb,
+                    "budget_gb": a.budget_gb, "arrival_fps_per_stream": a.fps if a.config == 3 else 120.0,
                     "stream_sizes": sorted({f"{r.width}x{r.height}" for r in reqs})})
         print(json.dumps(res), flush=True)
         for d in srv.denoisers.values():

@@ -343,3 +343,58 @@ def test_adam_keras(dev):
         pr -= alpha * mr / (np.sqrt(vr) + 1e-7)
     torch.cuda.synchronize()
     assert rel_l2(to_np(P), pr) < 1e-6
+
+
+# ------------------------------------------------------------------------------------------------
+def test_config0_forward_chain(ops, dev):
+    """The plain method on one 256x256 frame with the previous frame as history (mvec zero,
+    bandwidths fixed at 1): cross-regression, reprojection of the previous results, erosion +
+    temporal screening and the spatial filter, each stage against the FP64 oracle on that stage's
+    own inputs (masks and selections bit-exact, values <= 1e-4 relative L2)."""
+    from xdn import synth
+    H = W = 256
+    spec = synth.StreamSpec(H, W, seed=0, pan=(0.0, 0.0))
+    f0, f1 = synth.make_frame(spec, 0), synth.make_frame(spec, 1)
+    n = lambda t: t.detach().cpu().numpy()
+    g = lambda a: to_dev(np.asarray(a, np.float32), dev)
+    ones5 = np.ones((1, H, W, 5), np.float32)
+
+    def regress(fr):
+        y1 = O.prepare_input(n(fr["color"])).astype(np.float32)
+        y2 = O.prepare_input(n(fr["color2"])).astype(np.float32)
+        got = ops.cross_regression(g(y1), g(y2), g(n(fr["albedo"])), g(n(fr["normal"])), win_size=17, sparsity=4)
+        ref = O.cross_regression(y1, y2, n(fr["albedo"]), n(fr["normal"]))
+        for a, r in zip(got, ref):
+            assert rel_l2(n(a), r) <= TOL
+        return n(got[0]), n(got[1])
+
+    # frame 0: no history; its filtered output (bandwidths 1) becomes the history
+    o1, o2 = regress(f0)
+    acc = ops.spatial_filter(g(o1), g(o2), g(n(f0["albedo"])), g(n(f0["normal"])), g(ones5), win_size=11)
+    acc_ref = O.spatial_filter(o1, o2, n(f0["albedo"]), n(f0["normal"]), ones5)
+    for a, r in zip(acc, acc_ref):
+        assert rel_l2(n(a), r) <= TOL
+    out_log0 = ((n(acc[0]) + n(acc[1])) / (n(acc[2]) + n(acc[3]) + 1e-4)).astype(np.float32)
+    # frame 1
+    p1, p2 = regress(f1)
+    hist = [out_log0, o1, o2]
+    ones1 = np.ones((1, H, W, 1), np.float32)
+    rp_args = (n(f1["linearZ"]), n(f0["linearZ"]), n(f1["normal"]), n(f0["normal"]), n(f1["pnFwidth"]), ones1, ones1)
+    s, rp = ops.Reproject(g(n(f1["mvec"])), [g(h) for h in hist], *[g(a) for a in rp_args])
+    s_ref, rp_ref = O.reproject(n(f1["mvec"]), hist, *rp_args)
+    assert np.array_equal(n(s), s_ref)
+    # mvec zero, identical depth/normal: everything but row 0 / column 0 reprojects
+    assert s_ref[0, 1:, 1:].all() and not s_ref[0, 0, :].any() and not s_ref[0, :, 0].any()
+    for a, r in zip(rp, rp_ref):
+        assert rel_l2(n(a), r) <= 1e-6
+    se = O.erosion(n(s).astype(bool))
+    scr = ops.temporal_screening(g(p1), g(p2), *rp, g(se), win_curr=7, win_prev=1)
+    scr_ref = O.temporal_screening(p1, p2, *[n(a) for a in rp], se, 7, 1)
+    for a, r in zip(scr, scr_ref):
+        assert np.array_equal(n(a), r)
+    acc1 = ops.spatial_filter(g(p1), g(p2), g(n(f1["albedo"])), g(n(f1["normal"])), g(ones5), win_size=11)
+    acc1_ref = O.spatial_filter(p1, p2, n(f1["albedo"]), n(f1["normal"]), ones5)
+    for a, r in zip(acc1, acc1_ref):
+        err = rel_l2(n(a), r)
+        assert err <= TOL
+    _report("configuration 0 chain, last spatial filter acca", rel_l2(n(acc1[0]), acc1_ref[0]))
