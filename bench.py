@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--pipeline", action="store_true",
+                    help="overlap frame t+1's cross-regression with frame t on a second HIP stream")
     return ap.parse_args()
 
 
@@ -203,24 +205,40 @@ def main():
     den = pipeline.Denoiser(H, W, device=dev)
     torch.cuda.synchronize()
     print(f"[bench] {H}x{W}, {args.frames} frames resident, warmup {args.warmup}", file=sys.stderr, flush=True)
-    for i in range(args.warmup):
-        den.step(frames[i % args.frames])
-    torch.cuda.synchronize()
     st = torch.cuda.current_stream()
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    aux = torch.cuda.Stream(dev)
+
+    def run(n, start, pipelined, evs=None):
+        seq = [frames[(start + i) % args.frames] for i in range(n)]
+        rec = (lambda i, s: evs[i + 1].record(s)) if evs is not None else None
+        if pipelined:
+            pipeline.run_pipelined(den, seq, main=st, aux=aux, on_frame=rec)
+        else:
+            for i, f in enumerate(seq):
+                den.step(f)
+                if rec is not None:
+                    rec(i, st)
+
+    def timed(n, start, pipelined):
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        evs[0].record(st)
+        run(n, start, pipelined, evs)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(n)])
+        return max(wall, evs[0].elapsed_time(evs[-1]) / 1e3), lat
+
+    pipelined = args.pipeline
+    run(args.warmup, 0, pipelined)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evs[0].record(st)
-    for i in range(args.steps):
-        den.step(frames[(args.warmup + i) % args.frames])
-        evs[i + 1].record(st)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    lat = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)])
-    gpu_ms = evs[0].elapsed_time(evs[-1])
-    secs = max(wall, gpu_ms / 1e3)
+    secs, lat = timed(args.steps, args.warmup, pipelined)
     fps = args.steps / secs
-    print(f"[bench] timed {args.steps} frames: {fps:.1f} frames/s", file=sys.stderr, flush=True)
+    print(f"[bench] timed {args.steps} frames ({'pipelined' if pipelined else 'frame by frame'}): "
+          f"{fps:.1f} frames/s", file=sys.stderr, flush=True)
+    # the other schedule over the same frames, for reference
+    secs_other, _ = timed(args.steps, args.warmup, not pipelined)
 
     stages = stage_times(den, frames, args.kernel_reps, torch)
     print(f"[bench] stages (ms): {stages}", file=sys.stderr, flush=True)
@@ -254,6 +272,9 @@ def main():
                                "(cnn5 5x conv3x3 32ch)", "height": H, "width": W, "frames": args.frames,
                    "parallelism": "single stream"},
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "schedule": "frame t+1's cross-regression overlaps frame t on a second HIP stream" if pipelined else
+                    "frame by frame, one HIP stream",
+        ("ms_per_step_frame_by_frame" if pipelined else "ms_per_step_pipelined"): round(1e3 * secs_other / args.steps, 3),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roofline, "cpu_baseline": cpu,
     }

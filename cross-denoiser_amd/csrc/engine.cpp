@@ -104,7 +104,13 @@ int xdn_slot_init(const xdn_slot *slot, void *stream)
 
 int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
 {
+    return xdn_denoise_stages(slots, n_slots, lr, XDN_STAGE_ALL, stream);
+}
+
+int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages, void *stream)
+{
     if (n_slots < 0 || n_slots > XDN_MAX_FRAMES) return fail(XDN_EINVAL, "n_slots must be in [0, %d]", XDN_MAX_FRAMES);
+    if (stages & ~XDN_STAGE_ALL) return fail(XDN_EINVAL, "unknown stage bits 0x%x", stages);
     for (int i = 0; i < n_slots; ++i)
         if (int e = check_slot(slots[i])) return e;
     if (n_slots == 0) return XDN_OK;
@@ -120,6 +126,7 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
         cw.emplace_back(sc[i].cnn_ws, 1, slots[i].height, slots[i].width);
     }
     // 1. cross-regression with the log1p prologue (variance launches hold 2 frames per slot)
+    if (stages & XDN_STAGE_REGRESS)
     {
         std::vector<xdn::CrFrame> fr(n);
         for (int i = 0; i < n; ++i)
@@ -139,6 +146,7 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
         if (int e = hip_check(xdn::launch_recon(n, fr.data(), kRegWin, kSparsity, 1, st), "recon")) return e;
     }
     // 2. reprojection of [out_log, ols1, ols2] of the previous frame (main.py:266, opacities one)
+    if (stages & XDN_STAGE_REPROJECT)
     {
         std::vector<xdn::RpFrame> fr(n);
         const int dims[3] = {3, 3, 3};
@@ -156,6 +164,7 @@ int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
         }
         if (int e = hip_check(xdn::launch_reproject(n, fr.data(), 3, dims, st), "reproject")) return e;
     }
+    if (!(stages & XDN_STAGE_FINISH)) return XDN_OK;
     // 3. 3x3 erosion + temporal screening (win_curr 7, win_prev 1, main.py:270-280)
     {
         std::vector<xdn::TsFrame> fr(n);

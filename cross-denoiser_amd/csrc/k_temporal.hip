@@ -46,9 +46,11 @@ struct MinMax9 {
     float mn[9], mx[9];  // sets avg, 1, 2 x channels
 };
 
-__device__ __forceinline__ void emit(const TsFrame &F, size_t c, float sc, const MinMax9 *cur, bool center_ok)
+// Selection of one pixel: current values (a, b, their average m) against the reprojected history
+// (p, p1, p2) with the current window extrema `cur`.
+__device__ __forceinline__ void emit_values(const TsFrame &F, size_t c, float sc, const MinMax9 *cur, bool center_ok,
+                                            float3 a, float3 b, float3 p, float3 p1, float3 p2)
 {
-    const float3 a = ld3(F.c1, c), b = ld3(F.c2, c);
     const float3 m = make_float3((a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f);
     if (sc + 1e-4f < 1.f)
     {
@@ -58,7 +60,6 @@ __device__ __forceinline__ void emit(const TsFrame &F, size_t c, float sc, const
         F.osucc[c] = 0.f;
         return;
     }
-    const float3 p = ld3(F.prev, c), p1 = ld3(F.prev1, c), p2 = ld3(F.prev2, c);
     const float pv[9] = {p.x, p.y, p.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z};
     bool keep[3];
 #pragma unroll
@@ -82,23 +83,35 @@ __device__ __forceinline__ void emit(const TsFrame &F, size_t c, float sc, const
     F.osucc[c] = keep[0] ? 1.f : 0.f;
 }
 
+// Every global load of the block is issued up front (the pixel's history and the success input with
+// the tile staging), so a block makes one round trip to memory instead of three.
 __global__ __launch_bounds__(256) void screening_tile(TsBatch b)
 {
     constexpr int EX = TS_TX + 2 * TS_R, EY = TS_TY + 2 * TS_R;
     __shared__ float v[9][EY][EX];          // avg.xyz, c1.xyz, c2.xyz
     __shared__ float hmn[9][EY][TS_TX];     // row-pass minima
     __shared__ float hmx[9][EY][TS_TX];     // row-pass maxima
+    __shared__ uint8_t ms[EY][EX];          // success mask (fused erosion), 1 outside the image
     const int tile = xcd_tile(blockIdx.x, b.map.total);
     const TileLoc L = locate(b.map, tile);
     const TsFrame &F = b.f[L.f];
     const int H = F.H, W = F.W;
     const int x0 = L.tx * TS_TX, y0 = L.ty * TS_TY;
+    const int olx = threadIdx.x % TS_TX, oly = threadIdx.x / TS_TX;
+    const int cx = x0 + olx, cy = y0 + oly;
+    const bool live = cx < W && cy < H;
+    const size_t c = live ? (size_t)cy * W + cx : 0;
+    // this pixel's history and (unfused) success, in flight during the staging and the passes
+    const float3 p = ld3(F.prev, c), p1 = ld3(F.prev1, c), p2 = ld3(F.prev2, c);
+    const float rs = b.fuse ? 0.f : F.rsucc[c];
     for (int i = threadIdx.x; i < EX * EY; i += 256)
     {
         const int ly = i / EX, lx = i - ly * EX;
-        const int gx = min(max(x0 + lx - TS_R, 0), W - 1), gy = min(max(y0 + ly - TS_R, 0), H - 1);
-        const size_t p = (size_t)gy * W + gx;
-        const float3 a = ld3(F.c1, p), bb = ld3(F.c2, p);
+        const int rx = x0 + lx - TS_R, ry = y0 + ly - TS_R;
+        const int gx = min(max(rx, 0), W - 1), gy = min(max(ry, 0), H - 1);
+        const size_t q = (size_t)gy * W + gx;
+        const float3 a = ld3(F.c1, q), bb = ld3(F.c2, q);
+        if (b.fuse) ms[ly][lx] = (rx == gx && ry == gy) ? (F.succ_mask[q] ? 1 : 0) : 1;
         v[0][ly][lx] = (a.x + bb.x) * 0.5f; v[1][ly][lx] = (a.y + bb.y) * 0.5f; v[2][ly][lx] = (a.z + bb.z) * 0.5f;
         v[3][ly][lx] = a.x; v[4][ly][lx] = a.y; v[5][ly][lx] = a.z;
         v[6][ly][lx] = bb.x; v[7][ly][lx] = bb.y; v[8][ly][lx] = bb.z;
@@ -125,9 +138,7 @@ __global__ __launch_bounds__(256) void screening_tile(TsBatch b)
         for (int k = 0; k < 9; ++k) hmn[k][ly][lx] = mn[k], hmx[k][ly][lx] = mx[k];
     }
     __syncthreads();
-    const int lx = threadIdx.x % TS_TX, ly = threadIdx.x / TS_TX;
-    const int cx = x0 + lx, cy = y0 + ly;
-    if (cx >= W || cy >= H) return;
+    if (!live) return;
     const int lo = max(cy - TS_R, 0) - (y0 - TS_R), hi = min(cy + TS_R, H - 1) - (y0 - TS_R);
     MinMax9 cur;
 #pragma unroll
@@ -136,14 +147,25 @@ __global__ __launch_bounds__(256) void screening_tile(TsBatch b)
 #pragma unroll
         for (int k = 0; k < 9; ++k)
         {
-            cur.mn[k] = fminf(cur.mn[k], hmn[k][yy][lx]);
-            cur.mx[k] = fmaxf(cur.mx[k], hmx[k][yy][lx]);
+            cur.mn[k] = fminf(cur.mn[k], hmn[k][yy][olx]);
+            cur.mx[k] = fmaxf(cur.mx[k], hmx[k][yy][olx]);
         }
-    const size_t c = (size_t)cy * W + cx;
-    const float sc = b.fuse ? eroded(F.succ_mask, cx, cy, W, H) : F.rsucc[c];
-    if (b.fuse && F.succ_e) F.succ_e[c] = sc;
+    float sc = rs;
+    if (b.fuse)
+    {
+        // 3x3 erosion: min over the in-bounds neighbours (outside the image stored as 1)
+        int m = 1;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) m = min(m, (int)ms[oly + TS_R + dy][olx + TS_R + dx]);
+        sc = m ? 1.f : 0.f;
+        if (F.succ_e) F.succ_e[c] = sc;
+    }
+    const float3 a = make_float3(v[3][oly + TS_R][olx + TS_R], v[4][oly + TS_R][olx + TS_R], v[5][oly + TS_R][olx + TS_R]);
+    const float3 bb = make_float3(v[6][oly + TS_R][olx + TS_R], v[7][oly + TS_R][olx + TS_R], v[8][oly + TS_R][olx + TS_R]);
     // win_prev == 1: the previous extrema are the center values, taken iff sc >= 1
-    emit(F, c, sc, &cur, sc >= 1.f);
+    emit_values(F, c, sc, &cur, sc >= 1.f, a, bb, p, p1, p2);
 }
 
 __global__ __launch_bounds__(256) void screening_generic(TsBatch b)

@@ -55,6 +55,7 @@ PROTOTYPES = {
     "xdn_slot_scratch_offsets": (_I, [_I, _I, _P]),
     "xdn_slot_init": (_I, [_P, _P]),
     "xdn_denoise_step": (_I, [_P, _I, _F, _P]),
+    "xdn_denoise_stages": (_I, [_P, _I, _F, _I, _P]),
     "xdn_denoise_graph_create": (_I, [_P, _I, _F, _P, _PP]),
     "xdn_denoise_graph_launch": (_I, [_P, _P]),
     "xdn_denoise_graph_destroy": (_I, [_P]),

@@ -120,20 +120,23 @@ class Denoiser:
         """(out_log, ols1, ols2) of the previous frame."""
         return self.hist[(self.t + 1) % 2]
 
-    def make_slot(self, frame) -> XdnSlot:
+    def make_slot(self, frame, t=None) -> XdnSlot:
+        """Slot of step `t` (default: the next step) for `frame`. For a step other than the next one
+        only the cross-regression stage's fields (frame inputs, ols1/ols2, scratch) are meaningful."""
+        t = self.t if t is None else t
         s = XdnSlot()
         s.height, s.width = self.H, self.W
         if frame is not None:
             for k, name in (("color", "color"), ("color2", "color2"), ("albedo", "albedo"), ("normal", "normal"),
                             ("mvec", "mvec"), ("linearZ", "linear_z"), ("pnFwidth", "pn_fwidth"),
                             ("opacity", "opacity")):
-                t = frame[k]
-                if t.device != self.device or t.dtype != torch.float32 or not t.is_contiguous():
+                ten = frame[k]
+                if ten.device != self.device or ten.dtype != torch.float32 or not ten.is_contiguous():
                     raise ValueError(f"frame[{k!r}] must be a contiguous float32 tensor on {self.device}")
-                if tuple(t.shape[:3]) != (1, self.H, self.W):
-                    raise ValueError(f"frame[{k!r}] has shape {tuple(t.shape)}, stream is {self.H}x{self.W}")
-                setattr(s, name, t.data_ptr())
-        prv, cur = self.hist[(self.t + 1) % 2], self.hist[self.t % 2]
+                if tuple(ten.shape[:3]) != (1, self.H, self.W):
+                    raise ValueError(f"frame[{k!r}] has shape {tuple(ten.shape)}, stream is {self.H}x{self.W}")
+                setattr(s, name, ten.data_ptr())
+        prv, cur = self.hist[(t + 1) % 2], self.hist[t % 2]
         s.prev_out_log, s.prev_ols1, s.prev_ols2 = (t.data_ptr() for t in prv)
         pf = self.prev_frame
         s.prev_linear_z = (pf["linearZ"] if pf is not None else self.zeros3).data_ptr()
@@ -144,7 +147,7 @@ class Denoiser:
         s.params, s.grad, s.adam_m, s.adam_v = (t.data_ptr() for t in (self.params, self.grad, self.adam_m,
                                                                         self.adam_v))
         s.packed = self.packed.data_ptr()
-        s.adam_step = self.t + 1
+        s.adam_step = t + 1
         s.avg_loss = self.avg_loss.data_ptr()
         s.adam_t = self.adam_t.data_ptr()
         return s
@@ -240,3 +243,44 @@ def denoise_step_multi(denoisers, frames, stream=None):
     check(lib.xdn_denoise_step(ctypes.addressof(arr), len(denoisers), lr, st), "denoise_step")
     for d, f in zip(denoisers, frames):
         d.advance(f)
+
+
+STAGE_REGRESS, STAGE_REPROJECT, STAGE_FINISH = 1, 2, 4
+
+
+def run_pipelined(den: Denoiser, frames, main=None, aux=None, on_frame=None):
+    """Steps `den` through `frames` (eager launches) with frame t+1's cross-regression on a second
+    HIP stream, overlapping frame t's screening, network, filter and backward.
+
+    Frame t+1's cross-regression writes the ols buffers frame t reads as history, so it starts
+    once frame t's reprojection is done; frame t's finishing stage waits for its own
+    cross-regression. Results are identical to ``den.step`` frame by frame. ``on_frame(i, stream)``
+    is called after frame i's last stage is enqueued (e.g. to record an event)."""
+    frames = list(frames)
+    if not frames:
+        return None
+    main = main if main is not None else torch.cuda.current_stream(den.device)
+    aux = aux if aux is not None else torch.cuda.Stream(den.device)
+
+    def stages(slot, mask, st):
+        check(lib.xdn_denoise_stages(ctypes.addressof(slot), 1, den.lr, mask, st.cuda_stream), "denoise stages")
+
+    stages(den.make_slot(frames[0]), STAGE_REGRESS, main)
+    regressed = None  # event after the cross-regression of the next frame (on aux)
+    for i, f in enumerate(frames):
+        slot = den.make_slot(f)
+        stages(slot, STAGE_REPROJECT, main)
+        if regressed is not None:
+            main.wait_event(regressed)
+        if i + 1 < len(frames):
+            rp = torch.cuda.Event()
+            rp.record(main)
+            aux.wait_event(rp)
+            stages(den.make_slot(frames[i + 1], t=den.t + 1), STAGE_REGRESS, aux)
+            regressed = torch.cuda.Event()
+            regressed.record(aux)
+        stages(slot, STAGE_FINISH, main)
+        den.advance(f)
+        if on_frame is not None:
+            on_frame(i, main)
+    return den.outputs()

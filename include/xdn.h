@@ -230,6 +230,19 @@ int xdn_slot_init(const xdn_slot *slot, void *stream);
 /* One frame for each of n_slots (<= 16) slots, enqueued on `stream`. */
 int xdn_denoise_step(const xdn_slot *slots, int n_slots, float learning_rate, void *stream);
 
+/* The same step in parts, so that consecutive frames of a stream can overlap on two HIP streams.
+ * XDN_STAGE_REGRESS: cross-regression of this frame (reads only the frame; writes ols1/ols2 and
+ * slot scratch used by nothing else). XDN_STAGE_REPROJECT: reprojection of the history
+ * (prev_out_log/prev_ols1/prev_ols2). XDN_STAGE_FINISH: screening, network, filter, loss,
+ * backward, Adam. FINISH needs both others done; REGRESS of frame t+1 writes the buffers frame t
+ * reads as history, so it may start once frame t's REPROJECT is done. xdn_denoise_step ==
+ * all three on one stream. No counterpart in the reference (its train_step is one TF call chain). */
+#define XDN_STAGE_REGRESS 1
+#define XDN_STAGE_REPROJECT 2
+#define XDN_STAGE_FINISH 4
+#define XDN_STAGE_ALL 7
+int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float learning_rate, int stages, void *stream);
+
 /* The same step captured once into a HIP graph (one graph per frame: the slots' buffer
  * addresses are baked in, so a caller keeps one graph per distinct set of frame / history
  * buffers, e.g. per history parity). Every slot must have adam_t set. `stream` must not be the

@@ -48,3 +48,29 @@ def test_argument_errors_without_gpu():
     assert lib.xdn_cross_regression_workspace_size(1, 1080, 1920, 4) == 270 * 480 * 36 * 4
     rc = lib.xdn_reproject(None, 1, None, (ctypes.c_int * 1)(5), *([None] * 8), None, 1, 8, 8, None)
     assert rc == -1 and "channels" in lib.xdn_last_error().decode()
+
+
+def test_make_slot_parity_selects_history_buffers():
+    """A slot for step t takes its outputs from history buffer t % 2 and its history from the other
+    (host logic only: CPU tensors stand in for device buffers)."""
+    import torch
+    from xdn import pipeline
+
+    class Fake(pipeline.Denoiser):
+        def __init__(self):  # no device allocation: only the fields make_slot reads
+            self.H, self.W, self.t = 8, 12, 3
+            self.device = torch.device("cpu")
+            z = lambda: torch.zeros(1, 8, 12, 3)
+            self.hist = [[z(), z(), z()], [z(), z(), z()]]
+            self.out, self.zeros3, self.scratch = z(), z(), torch.zeros(16, dtype=torch.uint8)
+            self.params = self.grad = self.adam_m = self.adam_v = torch.zeros(4)
+            self.packed, self.avg_loss = torch.zeros(4, dtype=torch.uint8), torch.zeros(1)
+            self.adam_t, self.prev_frame = torch.zeros(1, dtype=torch.int32), None
+
+    d = Fake()
+    frame = {k: torch.zeros(1, 8, 12, 3 if k != "opacity" else 1) for k in
+             ("color", "color2", "albedo", "normal", "mvec", "linearZ", "pnFwidth", "opacity")}
+    s3, s4 = d.make_slot(frame), d.make_slot(frame, t=4)
+    assert s3.ols1 == d.hist[1][1].data_ptr() and s3.prev_ols1 == d.hist[0][1].data_ptr()
+    assert s4.ols1 == d.hist[0][1].data_ptr() and s4.prev_ols1 == d.hist[1][1].data_ptr()
+    assert s3.adam_step == 4 and s4.adam_step == 5 and s3.color == frame["color"].data_ptr()

@@ -134,3 +134,22 @@ def test_server_graph_mode_runs(dev):
     reqs = [StreamReq(i, 64, 96, 6, poisson_arrivals(200.0, 6, rng=rng), seed=i) for i in range(3)]
     res = Server(dev, hbm_budget=1 << 40, pool_frames=2, batching=False, graphs=True).run(reqs)
     assert res["frames"] == 18 and res["p99_ms"] >= res["p50_ms"] > 0
+
+
+def test_pipelined_frames_equal_sequential_steps(dev):
+    """Frame t+1's cross-regression on a second stream (overlapping frame t) gives results identical
+    to stepping frame by frame."""
+    from xdn import pipeline, synth
+    H, W = 48, 64
+    spec = synth.StreamSpec(H, W, seed=41, pan=(1.25, 0.5), disocclusion=0.1)
+    frames = [{k: v.to(dev) for k, v in synth.make_frame(spec, t).items()} for t in range(5)]
+    seq = pipeline.Denoiser(H, W, device=dev, seed=9)
+    for f in frames:
+        seq.step(f)
+    pip = pipeline.Denoiser(H, W, device=dev, seed=9)
+    pipeline.run_pipelined(pip, frames)
+    torch.cuda.synchronize()
+    for k in ("out", "out_log", "ols1", "ols2", "loss"):
+        assert torch.equal(seq.outputs()[k], pip.outputs()[k]), k
+    assert torch.equal(seq.params, pip.params)
+    assert pip.t == seq.t == 5
