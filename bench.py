@@ -240,6 +240,11 @@ def main():
     # the other schedule over the same frames, for reference
     secs_other, _ = timed(args.steps, args.warmup, not pipelined)
 
+    # denoising quality on the last timed frame (RelL2 of scripts/main.py:191-196 against the
+    # generator's clean reference), outside the timed region
+    last = frames[(args.warmup + args.steps - 1) % args.frames]
+    quality = {"relL2_output": round(pipeline.rel_l2(den.outputs()["out"], last["ref"]), 5),
+               "relL2_noisy_input": round(pipeline.rel_l2(0.5 * (last["color"] + last["color2"]), last["ref"]), 5)}
     stages = stage_times(den, frames, args.kernel_reps, torch)
     print(f"[bench] stages (ms): {stages}", file=sys.stderr, flush=True)
     # dominant custom kernel
@@ -276,6 +281,7 @@ def main():
                     "frame by frame, one HIP stream",
         ("ms_per_step_frame_by_frame" if pipelined else "ms_per_step_pipelined"): round(1e3 * secs_other / args.steps, 3),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "quality": quality,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -245,6 +245,13 @@ def denoise_step_multi(denoisers, frames, stream=None):
         d.advance(f)
 
 
+def rel_l2(pred: torch.Tensor, ref: torch.Tensor) -> float:
+    """Mean relative squared error of the frame driver's report (RelL2, scripts/main.py:191-196):
+    mean((ref - pred)^2 / (mean_c(ref)^2 + 1e-2)). An evaluation metric, not part of the step."""
+    m = ref.mean(dim=-1, keepdim=True)
+    return float(((ref - pred) ** 2 / (m * m + 1e-2)).mean())
+
+
 STAGE_REGRESS, STAGE_REPROJECT, STAGE_FINISH = 1, 2, 4
 
 

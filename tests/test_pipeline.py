@@ -153,3 +153,22 @@ def test_pipelined_frames_equal_sequential_steps(dev):
         assert torch.equal(seq.outputs()[k], pip.outputs()[k]), k
     assert torch.equal(seq.params, pip.params)
     assert pip.t == seq.t == 5
+
+
+def test_online_denoiser_reduces_error(dev):
+    """End to end on a synthetic sequence with a known clean reference: after a few frames the
+    denoised output's RelL2 (scripts/main.py:191-196) is far below that of the noisy input."""
+    from xdn import pipeline, synth
+    H, W = 96, 128
+    spec = synth.StreamSpec(H, W, seed=3, pan=(0.5, -0.25))
+    den = pipeline.Denoiser(H, W, device=dev)
+    errs = []
+    for t in range(12):
+        f = {k: v.to(dev) for k, v in synth.make_frame(spec, t).items()}
+        out = den.step(f)
+        torch.cuda.synchronize()
+        noisy = 0.5 * (f["color"] + f["color2"])
+        errs.append((pipeline.rel_l2(out["out"], f["ref"]), pipeline.rel_l2(noisy, f["ref"])))
+    print("  relL2 out/input by frame: " + " ".join(f"{a:.3f}/{b:.3f}" for a, b in errs), flush=True)
+    assert all(np.isfinite(a) for a, _ in errs)
+    assert errs[-1][0] < 0.25 * errs[-1][1]
