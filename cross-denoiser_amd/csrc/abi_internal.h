@@ -8,6 +8,7 @@
 #include "../../include/xdn.h"
 #include "xdn_cnn.h"
 #include "xdn_common.h"
+#include "xdn_unet.h"
 
 namespace xdn_abi {
 
@@ -88,6 +89,130 @@ struct Cnn5Ws {
         }
         bytes = o;
     }
+};
+
+// ---------------------------------------------------------------------------------------------
+// unet: the reference's bandwidth U-Net (scripts/main.py:83-118). Parameters in Keras order, per
+// layer kernel then bias: 3x3 kernels [3][3][ci][co], transposed 1x1 kernels [co][ci], the head
+// [ci][co]. Levels: 0 = full resolution, 1 = half, 2 = quarter.
+// ---------------------------------------------------------------------------------------------
+enum { UL_CONV = 0, UL_CONVT = 1, UL_HEAD = 2 };
+struct UnetLayer {
+    int type, ci, co, level;  // level of the layer's input
+};
+constexpr int UNET_LAYERS = 12;
+constexpr UnetLayer kUnet[UNET_LAYERS] = {
+    {UL_CONV, 15, 8, 0},  {UL_CONV, 8, 8, 0},   {UL_CONV, 8, 16, 1},   {UL_CONV, 16, 16, 1},
+    {UL_CONV, 16, 32, 2}, {UL_CONV, 32, 32, 2}, {UL_CONVT, 32, 32, 2}, {UL_CONV, 48, 16, 1},
+    {UL_CONV, 16, 16, 1}, {UL_CONVT, 16, 16, 1}, {UL_CONV, 24, 8, 0},  {UL_HEAD, 8, 6, 0}};
+
+// FP32 parameter offsets (floats); packed bytes: bf16 fragments, padded biases of the 3x3 layers
+// and an FP32 copy of the 1x1 layers' kernel + bias.
+struct UnetLayout {
+    size_t w_off[UNET_LAYERS], b_off[UNET_LAYERS], count;
+    size_t fwd_off[UNET_LAYERS], dg_off[UNET_LAYERS], bias_off[UNET_LAYERS], master_off[UNET_LAYERS], packed;
+    UnetLayout()
+    {
+        size_t o = 0, q = 0;
+        for (int l = 0; l < UNET_LAYERS; ++l)
+        {
+            const UnetLayer &L = kUnet[l];
+            const size_t nw = (L.type == UL_CONV ? 9 : 1) * (size_t)L.ci * L.co;
+            w_off[l] = o;
+            o += nw;
+            b_off[l] = o;
+            o += L.co;
+            fwd_off[l] = dg_off[l] = bias_off[l] = master_off[l] = 0;
+            if (L.type == UL_CONV)
+            {
+                const size_t kcf = (L.ci + 15) / 16, ntf = (L.co + 31) / 32;
+                const size_t kcd = (L.co + 15) / 16, ntd = (L.ci + 31) / 32;
+                fwd_off[l] = q;
+                q += al256(9 * kcf * ntf * 64 * 16);
+                dg_off[l] = q;
+                q += l > 0 ? al256(9 * kcd * ntd * 64 * 16) : 0;
+                bias_off[l] = q;
+                q += al256(ntf * 32 * 4);
+            }
+            else
+            {
+                master_off[l] = q;
+                q += al256((nw + L.co) * 4);
+            }
+        }
+        count = o;
+        packed = q;
+    }
+};
+inline const UnetLayout kULay;
+
+// Workspace buffers of the U-Net, [B][h][w][ch] each (bf16 unless f32).
+enum {
+    UB_A0, UB_A1, UB_P1, UB_A2, UB_A3, UB_P2, UB_A4, UB_A5, UB_A6, UB_A7, UB_A8, UB_A9, UB_A10,
+    UB_DY10, UB_DC5, UB_DY8, UB_DY7, UB_DC4, UB_DY5, UB_DY4, UB_DP2, UB_DY3, UB_DY2, UB_DP1, UB_DY1, UB_DY0,
+    UNET_BUFS
+};
+struct UnetBuf {
+    int level, ch, f32;
+};
+constexpr UnetBuf kUnetBuf[UNET_BUFS] = {
+    {0, 8, 0},  {0, 8, 0},  {1, 8, 0},  {1, 16, 0}, {1, 16, 0}, {2, 16, 0}, {2, 32, 0},
+    {2, 32, 0}, {1, 32, 0}, {1, 16, 0}, {1, 16, 0}, {0, 16, 0}, {0, 8, 0},              // A0 .. A10
+    {0, 8, 0},  {0, 24, 1}, {1, 16, 0}, {1, 16, 0}, {1, 48, 1}, {2, 32, 0}, {2, 32, 0},
+    {2, 16, 1}, {1, 16, 0}, {1, 16, 0}, {1, 8, 1},  {0, 8, 0},  {0, 8, 0}};             // gradients
+// Weight-gradient partial sets: one per 3x3 layer (two for c4_1: input channels 0-31 and 32-47),
+// the two transposed layers and the head. level = resolution the set's blocks walk.
+constexpr int UNET_JOBS = 13;
+struct UnetJob {
+    int layer, level;
+};
+constexpr UnetJob kUnetJob[UNET_JOBS] = {{0, 0}, {1, 0}, {2, 1}, {3, 1}, {4, 2},  {5, 2}, {6, 2},
+                                         {7, 1}, {7, 1}, {8, 1}, {9, 1}, {10, 0}, {11, 0}};
+
+inline int unet_job_blocks(int job, int h, int w)
+{
+    const int t = kUnet[kUnetJob[job].layer].type;
+    const long np = (long)h * w;
+    if (t == UL_CONV) return xdn::ublocks(h, w);
+    const long per = t == UL_CONVT ? 2048 : 4096;  // pixels per block
+    const long cap = t == UL_CONVT ? 256 : 512;
+    long nb = (np + per - 1) / per;
+    nb = nb < 1 ? 1 : (nb > cap ? cap : nb);
+    return (int)nb;
+}
+
+struct UnetWs {
+    char *buf[UNET_BUFS];
+    size_t off[UNET_BUFS], frame_bytes[UNET_BUFS];
+    float *part[UNET_JOBS], *stage[UNET_JOBS];
+    int nblk[UNET_JOBS];
+    int h[3], w[3];
+    size_t bytes;
+    UnetWs(void *base, int B, int H, int W)
+    {
+        h[0] = H, w[0] = W, h[1] = H / 2, w[1] = W / 2, h[2] = H / 4, w[2] = W / 4;
+        char *p = static_cast<char *>(base);
+        size_t o = 0;
+        for (int i = 0; i < UNET_BUFS; ++i)
+        {
+            const UnetBuf &U = kUnetBuf[i];
+            frame_bytes[i] = (size_t)h[U.level] * w[U.level] * U.ch * (U.f32 ? 4 : 2);
+            off[i] = o;
+            buf[i] = p + o;
+            o += al256(frame_bytes[i] * B);
+        }
+        for (int j = 0; j < UNET_JOBS; ++j)
+        {
+            const int lv = kUnetJob[j].level;
+            nblk[j] = unet_job_blocks(j, h[lv], w[lv]);
+            part[j] = reinterpret_cast<float *>(p + o);
+            o += al256((size_t)B * nblk[j] * xdn::WG_PART * 4);
+            stage[j] = reinterpret_cast<float *>(p + o);
+            o += al256((size_t)xdn::WG_SPLITS * xdn::WG_PART * 4);
+        }
+        bytes = o;
+    }
+    void *at(int i, int b) const { return buf[i] + (size_t)b * frame_bytes[i]; }
 };
 
 }  // namespace xdn_abi

@@ -528,19 +528,24 @@ __global__ __launch_bounds__(256) void wgrad_reduce2(RedBatch b)
 {
     const RedJob &J = b.j[blockIdx.y];
     const int e = blockIdx.x * 256 + threadIdx.x;
-    const int nw = 9 * J.ci * J.co;
-    if (e >= nw + J.co) return;
-    int src;
+    const int cc = J.ci_cnt < 0 ? J.ci : J.ci_cnt;
+    const int nw = J.taps * cc * J.co;
+    if (e >= nw + (J.bias ? J.co : 0)) return;
+    int src, dst;
     if (e < nw)
     {
-        const int t = e / (J.ci * J.co), r = e - t * J.ci * J.co, ci = r / J.co, co = r - ci * J.co;
-        src = t * 1024 + ci * 32 + co;
+        const int t = e / (cc * J.co), r = e - t * cc * J.co, cl = r / J.co, co = r - cl * J.co;
+        src = t * 1024 + cl * 32 + co;
+        dst = J.transpose ? co * J.ci + J.ci_off + cl : (t * J.ci + J.ci_off + cl) * J.co + co;
     }
     else
+    {
         src = 9 * 1024 + (e - nw);
+        dst = J.taps * J.ci * J.co + (e - nw);
+    }
     float s = 0.f;
     for (int sp = 0; sp < b.splits; ++sp) s += J.stage[sp * WG_PART + src];
-    J.grad[e] = s;
+    J.grad[dst] = s;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -678,21 +683,25 @@ hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st)
 
 hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st)
 {
-    if (n == 0) return hipSuccess;
-    if (n > WG_MAX_JOBS) return hipErrorInvalidValue;
-    RedBatch b{};
-    b.n = n;
-    b.splits = WG_SPLITS;
-    int maxw = 0;
-    for (int i = 0; i < n; ++i)
+    for (int j0 = 0; j0 < n; j0 += WG_MAX_JOBS)
     {
-        b.j[i] = jobs[i];
-        const int w = 9 * jobs[i].ci * jobs[i].co + jobs[i].co;
-        maxw = w > maxw ? w : maxw;
+        const int m = n - j0 < WG_MAX_JOBS ? n - j0 : WG_MAX_JOBS;
+        RedBatch b{};
+        b.n = m;
+        b.splits = WG_SPLITS;
+        int maxw = 0;
+        for (int i = 0; i < m; ++i)
+        {
+            const RedJob &J = jobs[j0 + i];
+            b.j[i] = J;
+            const int w = J.taps * (J.ci_cnt < 0 ? J.ci : J.ci_cnt) * J.co + J.co;
+            maxw = w > maxw ? w : maxw;
+        }
+        hipLaunchKernelGGL(wgrad_reduce1, dim3(div_up(WG_PART, 256), WG_SPLITS, m), dim3(256), 0, st, b);
+        hipLaunchKernelGGL(wgrad_reduce2, dim3(div_up(maxw, 256), m), dim3(256), 0, st, b);
+        if (hipError_t e = hipGetLastError()) return e;
     }
-    hipLaunchKernelGGL(wgrad_reduce1, dim3(div_up(WG_PART, 256), WG_SPLITS, n), dim3(256), 0, st, b);
-    hipLaunchKernelGGL(wgrad_reduce2, dim3(div_up(maxw, 256), n), dim3(256), 0, st, b);
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st)

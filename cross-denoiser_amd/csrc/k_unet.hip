@@ -1,0 +1,1047 @@
+// Kernels of the reference's bandwidth U-Net (scripts/main.py:83-118) on gfx950.
+//
+//   uconv3x3<SRC, CA, CB, CO, EPI>  3x3 'same' convolution as an implicit GEMM on
+//     v_mfma_f32_32x32x16_bf16 (M = 32 px of a row, N = 32 output channels per N-tile, K = 16
+//     input channels per chunk) over concat[a (CA ch), b (CB ch)] staged in LDS with zero padding
+//     to whole K chunks; 16x32-px tiles, every A fragment feeds up to three output rows. Forward
+//     (bias + ReLU -> bf16) and data gradient (flipped, transposed fragments; ReLU-masked bf16 or
+//     FP32 for concat and pooled inputs). Channel counts are the stored ones (8, 16, 24, 32, 48).
+//   uwgrad<SRC, CA, CB, CO, MT>  weight gradient over 8x32-px strips, operands from pixel-major
+//     LDS tiles through ds_read_b64_tr_b16, one 32-channel input slice (MT) per instantiation,
+//     software-pipelined strip loop (as conv3x3_wgrad of k_cnn.hip).
+//   maxpool2 / pool_bwd          2x2 max pool and its gradient (to the first maximum, plus the
+//     skip-connection gradient, ReLU-masked).
+//   convT1x1s2 / convT_bwd       transposed 1x1 stride-2 convolution and its gradients.
+//   head1x1 / head_bwd           the 1x1 8->6 output convolution and its gradients.
+//   upack                        FP32 master weights -> bf16 fragments.
+#include "xdn_common.h"
+#include "xdn_unet.h"
+
+namespace xdn {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s lds_v4s;
+
+__device__ __forceinline__ uint32_t u_f2bf(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint32_t u_pack2(float a, float b) { return u_f2bf(a) | (u_f2bf(b) << 16); }
+__device__ __forceinline__ float u_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float u_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+__device__ __forceinline__ f32x16 u_mfma(u32x4 a, u32x4 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+// transposing LDS read, see ld_tr of k_cnn.hip
+__device__ __forceinline__ uint2 u_ld_tr(const uint16_t *p)
+{
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)p));
+}
+// bf16 word pair: keep each half where the mask half is > 0
+__device__ __forceinline__ uint32_t u_keep(uint32_t mw, uint32_t vw)
+{
+    const uint32_t lo = mw & 0xFFFFu, hi = mw >> 16;
+    const uint32_t klo = ((lo & 0x8000u) == 0 && lo != 0) ? 0xFFFFu : 0u;
+    const uint32_t khi = ((hi & 0x8000u) == 0 && hi != 0) ? 0xFFFF0000u : 0u;
+    return vw & (klo | khi);
+}
+__device__ __forceinline__ float u_bf(const uint16_t *p, int i) { return __uint_as_float((uint32_t)p[i] << 16); }
+
+static int udiv_up(int a, int b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------------------------
+// 3x3 convolution (forward / data gradient)
+// ---------------------------------------------------------------------------------------------
+struct UcBatch {
+    TileMap map;
+    UcFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(UcBatch) <= 4096, "kernel argument too large");
+
+constexpr int UT_W = 32, UT_H = 16;
+
+template <int SRC, int CA, int CB, int CO, int EPI>
+__global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
+{
+    constexpr int CI = SRC == UC_SRC_IMAGES ? 15 : CA + CB;
+    constexpr int KC = (CI + 15) / 16, CIP = KC * 16, PS = CIP + 8;
+    constexpr int NT = (CO + 31) / 32;
+    constexpr int LW = UT_W + 2, LH = UT_H + 2;
+    constexpr int OS = CO + 8;  // epilogue pixel stride (bf16)
+    constexpr int IN_ELEMS = LH * LW * PS, OUT_ELEMS = EPI == UC_EPI_F32 ? 0 : UT_H * UT_W * OS;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[IN_ELEMS > OUT_ELEMS ? IN_ELEMS : OUT_ELEMS];
+    const int tile = xcd_tile(blockIdx.x, b.map.total);
+    const TileLoc L = locate(b.map, tile);
+    const UcFrame &F = b.f[L.f];
+    const int H = F.H, W = F.W;
+    const int x0 = L.tx * UT_W, y0 = L.ty * UT_H;
+    const int tid = threadIdx.x;
+
+    if constexpr (SRC == UC_SRC_BF16)
+    {
+        constexpr int QA = CA / 8, QB = CB / 8, QP = CIP / 8;
+        for (int i = tid; i < LH * LW * QP; i += 256)
+        {
+            const int pix = i / QP, q = i - pix * QP;
+            const int ly = pix / LW, lx = pix - ly * LW;
+            const int gy = y0 + ly - 1, gx = x0 + lx - 1;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+            {
+                const size_t p = (size_t)gy * W + gx;
+                if (q < QA)
+                    v = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.a) + p * CA + q * 8);
+                else if (q < QA + QB)
+                    v = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.b) + p * CB + (q - QA) * 8);
+            }
+            *reinterpret_cast<u32x4 *>(&lds[pix * PS + q * 8]) = v;
+        }
+    }
+    else
+    {
+        for (int pix = tid; pix < LH * LW; pix += 256)
+        {
+            const int ly = pix / LW, lx = pix - ly * LW;
+            const int gy = y0 + ly - 1, gx = x0 + lx - 1;
+            float v[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = 0.f;
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+            {
+                const size_t p = (size_t)gy * W + gx;
+#pragma unroll
+                for (int im = 0; im < 5; ++im)
+                {
+                    const float *s = F.img[im] + 3 * p;
+                    v[3 * im] = s[0]; v[3 * im + 1] = s[1]; v[3 * im + 2] = s[2];
+                }
+            }
+            u32x4 a, c;
+            a.x = u_pack2(v[0], v[1]); a.y = u_pack2(v[2], v[3]); a.z = u_pack2(v[4], v[5]); a.w = u_pack2(v[6], v[7]);
+            c.x = u_pack2(v[8], v[9]); c.y = u_pack2(v[10], v[11]); c.z = u_pack2(v[12], v[13]);
+            c.w = u_pack2(v[14], v[15]);
+            *reinterpret_cast<u32x4 *>(&lds[pix * PS]) = a;
+            *reinterpret_cast<u32x4 *>(&lds[pix * PS + 8]) = c;
+        }
+    }
+    const int lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
+    u32x4 wf[9][KC][NT];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                wf[t][kc][nt] = static_cast<const u32x4 *>(F.wfrag)[((t * KC + kc) * NT + nt) * 64 + lane];
+    float bias[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) bias[nt] = EPI == UC_EPI_RELU ? F.bias[nt * 32 + px] : 0.f;
+    __syncthreads();
+
+    f32x16 acc[4][NT];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc[r][nt][k] = 0.f;
+#pragma unroll
+    for (int ir = 0; ir < 6; ++ir)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+            {
+                const u32x4 a = *reinterpret_cast<const u32x4 *>(&lds[((4 * wv + ir) * LW + px + dx) * PS + kc * 16 + 8 * h]);
+#pragma unroll
+                for (int orow = 0; orow < 4; ++orow)
+                {
+                    const int dy = ir - orow;
+                    if (dy < 0 || dy > 2) continue;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) acc[orow][nt] = u_mfma(a, wf[dy * 3 + dx][kc][nt], acc[orow][nt]);
+                }
+            }
+
+    // epilogue: lane owns output channel nt*32 + px of 16 pixels x = 8(k/4) + 4h + k%4 per row
+    if constexpr (EPI == UC_EPI_F32)
+    {
+#pragma unroll
+        for (int orow = 0; orow < 4; ++orow)
+        {
+            const int gy = y0 + 4 * wv + orow;
+            if (gy >= H) continue;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+            {
+                const int n = nt * 32 + px;
+                if (n >= CO) continue;
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                {
+                    const int gx = x0 + 8 * (k >> 2) + 4 * h + (k & 3);
+                    if (gx < W) static_cast<float *>(F.out)[((size_t)gy * W + gx) * CO + n] = acc[orow][nt][k];
+                }
+            }
+        }
+    }
+    else
+    {
+        __syncthreads();  // the input tile is dead
+#pragma unroll
+        for (int orow = 0; orow < 4; ++orow)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+            {
+                const int n = nt * 32 + px;
+                if (n >= CO) continue;
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                {
+                    const int x = 8 * (k >> 2) + 4 * h + (k & 3);
+                    const float v = acc[orow][nt][k];
+                    lds[((4 * wv + orow) * UT_W + x) * OS + n] =
+                        (uint16_t)u_f2bf(EPI == UC_EPI_RELU ? fmaxf(v + bias[nt], 0.f) : v);
+                }
+            }
+        __syncthreads();
+        constexpr int QO = CO / 8;
+        for (int i = tid; i < UT_H * UT_W * QO; i += 256)
+        {
+            const int pix = i / QO, q = i - pix * QO;
+            const int gy = y0 + pix / UT_W, gx = x0 + (pix & (UT_W - 1));
+            if (gy >= H || gx >= W) continue;
+            u32x4 v = *reinterpret_cast<const u32x4 *>(&lds[pix * OS + q * 8]);
+            const size_t o = ((size_t)gy * W + gx) * CO + q * 8;
+            if constexpr (EPI == UC_EPI_MASK)
+            {
+                const u32x4 m = *reinterpret_cast<const u32x4 *>(F.mask + o);
+                v.x = u_keep(m.x, v.x); v.y = u_keep(m.y, v.y); v.z = u_keep(m.z, v.z); v.w = u_keep(m.w, v.w);
+            }
+            *reinterpret_cast<u32x4 *>(static_cast<uint16_t *>(F.out) + o) = v;
+        }
+    }
+}
+
+template <int SRC, int CA, int CB, int CO, int EPI>
+static hipError_t uconv_t(int n, const UcFrame *fr, hipStream_t st)
+{
+    UcBatch b{};
+    int tx[XDN_MAX_FRAMES], ty[XDN_MAX_FRAMES];
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        tx[i] = udiv_up(fr[i].W, UT_W);
+        ty[i] = udiv_up(fr[i].H, UT_H);
+    }
+    b.map = make_tilemap(n, tx, ty);
+    if (b.map.total == 0) return hipSuccess;
+    hipLaunchKernelGGL((uconv3x3<SRC, CA, CB, CO, EPI>), dim3(b.map.total), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_uconv(int kind, int n, const UcFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    switch (kind)
+    {
+    case UC_F_0: return uconv_t<UC_SRC_IMAGES, 0, 0, 8, UC_EPI_RELU>(n, fr, st);
+    case UC_F_8_8: return uconv_t<UC_SRC_BF16, 8, 0, 8, UC_EPI_RELU>(n, fr, st);
+    case UC_F_8_16: return uconv_t<UC_SRC_BF16, 8, 0, 16, UC_EPI_RELU>(n, fr, st);
+    case UC_F_16_16: return uconv_t<UC_SRC_BF16, 16, 0, 16, UC_EPI_RELU>(n, fr, st);
+    case UC_F_16_32: return uconv_t<UC_SRC_BF16, 16, 0, 32, UC_EPI_RELU>(n, fr, st);
+    case UC_F_32_32: return uconv_t<UC_SRC_BF16, 32, 0, 32, UC_EPI_RELU>(n, fr, st);
+    case UC_F_48_16: return uconv_t<UC_SRC_BF16, 32, 16, 16, UC_EPI_RELU>(n, fr, st);
+    case UC_F_24_8: return uconv_t<UC_SRC_BF16, 16, 8, 8, UC_EPI_RELU>(n, fr, st);
+    case UC_D_8_8: return uconv_t<UC_SRC_BF16, 8, 0, 8, UC_EPI_MASK>(n, fr, st);
+    case UC_D_16_8: return uconv_t<UC_SRC_BF16, 16, 0, 8, UC_EPI_F32>(n, fr, st);
+    case UC_D_16_16: return uconv_t<UC_SRC_BF16, 16, 0, 16, UC_EPI_MASK>(n, fr, st);
+    case UC_D_32_16: return uconv_t<UC_SRC_BF16, 32, 0, 16, UC_EPI_F32>(n, fr, st);
+    case UC_D_32_32: return uconv_t<UC_SRC_BF16, 32, 0, 32, UC_EPI_MASK>(n, fr, st);
+    case UC_D_16_48: return uconv_t<UC_SRC_BF16, 16, 0, 48, UC_EPI_F32>(n, fr, st);
+    case UC_D_8_24: return uconv_t<UC_SRC_BF16, 8, 0, 24, UC_EPI_F32>(n, fr, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Weight gradient of a 3x3 layer: input slice [32 MT, 32 MT + 32) of concat[a, b] (or the images)
+// ---------------------------------------------------------------------------------------------
+struct UwBatch {
+    int n;
+    int blk_begin[XDN_MAX_FRAMES + 1];
+    UwFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(UwBatch) <= 4096, "kernel argument too large");
+
+constexpr int US_W = 32, US_H = 8;                // strip: 8 rows x 32 px
+constexpr int UX_C = US_W + 2, UX_R = US_H + 2;   // staged X: 10 rows x 34 px (1-px halo)
+
+template <int SRC, int CA, int CB, int CO, int MT>
+__global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
+{
+    constexpr int XS = 32, DS = 32;  // staged channels (zero-padded)
+    constexpr int X_ELEMS = UX_R * UX_C * XS, D_ELEMS = US_H * US_W * DS;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[X_ELEMS + D_ELEMS];
+    __shared__ float dbs[4][32];
+    uint16_t *xs = lds, *ds = lds + X_ELEMS;
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const UwFrame &F = b.f[f];
+    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
+    const int H = F.H, W = F.W;
+    const int sx = (W + US_W - 1) / US_W, sy = (H + US_H - 1) / US_H;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, h = lane >> 5, ch = lane & 31;
+    const int rr = 2 * wv + h;
+    const int grp = (lane >> 4) & 1, s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3;
+    const int cb = 16 * grp + 4 * qq;
+
+    constexpr int NX = SRC == UC_SRC_BF16 ? (UX_R * UX_C * 4 + 255) / 256 : (UX_R * UX_C + 255) / 256;
+    constexpr int ND = US_H * US_W * 4 / 256;
+    u32x4 rx[SRC == UC_SRC_BF16 ? NX : 1], rd[ND];
+    float fx[SRC == UC_SRC_BF16 ? 1 : NX][15];
+    auto fetch = [&](int s) {
+        const int y0 = (s / sx) * US_H, x0 = (s % sx) * US_W;
+#pragma unroll
+        for (int u = 0; u < NX; ++u)
+        {
+            if constexpr (SRC == UC_SRC_BF16)
+            {
+                const int i = tid + 256 * u, pix = i >> 2, q = i & 3;
+                const int r = pix / UX_C, col = pix - r * UX_C;
+                const int gy = y0 - 1 + r, gx = x0 - 1 + col;
+                const int g = 32 * MT + 8 * q;  // first concat channel of this chunk
+                rx[u] = u32x4{0u, 0u, 0u, 0u};
+                if (i < UX_R * UX_C * 4 && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                {
+                    const size_t p = (size_t)gy * W + gx;
+                    if (g < CA)
+                        rx[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.a) + p * CA + g);
+                    else if (g < CA + CB)
+                        rx[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.b) + p * CB + (g - CA));
+                }
+            }
+            else
+            {
+                const int pix = tid + 256 * u;
+                const int r = pix / UX_C, col = pix - r * UX_C;
+                const int gy = y0 - 1 + r, gx = x0 - 1 + col;
+#pragma unroll
+                for (int j = 0; j < 15; ++j) fx[u][j] = 0.f;
+                if (pix < UX_R * UX_C && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                {
+                    const size_t p = (size_t)gy * W + gx;
+#pragma unroll
+                    for (int im = 0; im < 5; ++im)
+                    {
+                        const float *sp = F.img[im] + 3 * p;
+                        fx[u][3 * im] = sp[0]; fx[u][3 * im + 1] = sp[1]; fx[u][3 * im + 2] = sp[2];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < ND; ++u)
+        {
+            const int i = tid + 256 * u, pix = i >> 2, q = i & 3;
+            const int r = pix / US_W, col = pix - r * US_W;
+            const int gy = y0 + r, gx = x0 + col;
+            rd[u] = u32x4{0u, 0u, 0u, 0u};
+            if (q < CO / 8 && gy < H && gx < W)
+                rd[u] = *reinterpret_cast<const u32x4 *>(F.dy + ((size_t)gy * W + gx) * CO + q * 8);
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int u = 0; u < NX; ++u)
+        {
+            if constexpr (SRC == UC_SRC_BF16)
+            {
+                const int i = tid + 256 * u;
+                if (i < UX_R * UX_C * 4) *reinterpret_cast<u32x4 *>(&xs[(i >> 2) * XS + (i & 3) * 8]) = rx[u];
+            }
+            else
+            {
+                const int pix = tid + 256 * u;
+                if (pix < UX_R * UX_C)
+                {
+                    u32x4 a, c, z = {0u, 0u, 0u, 0u};
+                    a.x = u_pack2(fx[u][0], fx[u][1]); a.y = u_pack2(fx[u][2], fx[u][3]);
+                    a.z = u_pack2(fx[u][4], fx[u][5]); a.w = u_pack2(fx[u][6], fx[u][7]);
+                    c.x = u_pack2(fx[u][8], fx[u][9]); c.y = u_pack2(fx[u][10], fx[u][11]);
+                    c.z = u_pack2(fx[u][12], fx[u][13]); c.w = u_pack2(fx[u][14], 0.f);
+                    *reinterpret_cast<u32x4 *>(&xs[pix * XS]) = a;
+                    *reinterpret_cast<u32x4 *>(&xs[pix * XS + 8]) = c;
+                    *reinterpret_cast<u32x4 *>(&xs[pix * XS + 16]) = z;
+                    *reinterpret_cast<u32x4 *>(&xs[pix * XS + 24]) = z;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < ND; ++u)
+        {
+            const int i = tid + 256 * u;
+            *reinterpret_cast<u32x4 *>(&ds[(i >> 2) * DS + (i & 3) * 8]) = rd[u];
+        }
+    };
+
+    f32x16 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
+    float db = 0.f;
+    const int nstrip = sx * sy;
+    if (li < nstrip) fetch(li);
+    for (int s = li; s < nstrip; s += nblk)
+    {
+        __syncthreads();
+        commit();
+        __syncthreads();
+        if (s + nblk < nstrip) fetch(s + nblk);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            const uint16_t *dp = &ds[(rr * US_W + 8 * c + jq) * DS + cb];
+            const uint2 b0 = u_ld_tr(dp), b1 = u_ld_tr(dp + 4 * DS);
+            const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
+            db += (u_lo(B.x) + u_hi(B.x)) + (u_lo(B.y) + u_hi(B.y)) + (u_lo(B.z) + u_hi(B.z)) + (u_lo(B.w) + u_hi(B.w));
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int sh = 0; sh < 3; ++sh)
+                {
+                    const uint16_t *xp = &xs[((rr + dy) * UX_C + 8 * c + sh + jq) * XS + cb];
+                    const uint2 a0 = u_ld_tr(xp), a1 = u_ld_tr(xp + 4 * XS);
+                    acc[dy * 3 + sh] = u_mfma(u32x4{a0.x, a0.y, a1.x, a1.y}, B, acc[dy * 3 + sh]);
+                }
+        }
+    }
+    float *out = F.part + (size_t)li * WG_PART;
+    float *red = reinterpret_cast<float *>(lds);  // [4 waves][1024]
+    static_assert(sizeof(lds) >= 4 * 1024 * sizeof(float), "reduction buffer");
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+    {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+        {
+            const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
+            red[wv * 1024 + ci * 32 + ch] = acc[t][k];
+        }
+        __syncthreads();
+        for (int e = tid; e < 1024; e += 256) out[t * 1024 + e] = (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]);
+    }
+    db += __shfl_xor(db, 32);
+    if (h == 0) dbs[wv][ch] = db;
+    __syncthreads();
+    if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
+}
+
+int ublocks(int H, int W)
+{
+    const int strips = udiv_up(W, US_W) * udiv_up(H, US_H);
+    const int nb = udiv_up(strips, 16);
+    return nb < 1 ? 1 : nb;
+}
+
+template <int SRC, int CA, int CB, int CO, int MT>
+static hipError_t uwgrad_t(int n, const UwFrame *fr, hipStream_t st)
+{
+    UwBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.blk_begin[i] = acc;
+        acc += fr[i].nblk;
+    }
+    b.blk_begin[n] = acc;
+    if (acc == 0) return hipSuccess;
+    hipLaunchKernelGGL((uwgrad<SRC, CA, CB, CO, MT>), dim3(acc), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_uwgrad(int kind, int n, const UwFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    switch (kind)
+    {
+    case UW_0: return uwgrad_t<UC_SRC_IMAGES, 0, 0, 8, 0>(n, fr, st);
+    case UW_8_8: return uwgrad_t<UC_SRC_BF16, 8, 0, 8, 0>(n, fr, st);
+    case UW_8_16: return uwgrad_t<UC_SRC_BF16, 8, 0, 16, 0>(n, fr, st);
+    case UW_16_16: return uwgrad_t<UC_SRC_BF16, 16, 0, 16, 0>(n, fr, st);
+    case UW_16_32: return uwgrad_t<UC_SRC_BF16, 16, 0, 32, 0>(n, fr, st);
+    case UW_32_32: return uwgrad_t<UC_SRC_BF16, 32, 0, 32, 0>(n, fr, st);
+    case UW_48_16_M0: return uwgrad_t<UC_SRC_BF16, 32, 16, 16, 0>(n, fr, st);
+    case UW_48_16_M1: return uwgrad_t<UC_SRC_BF16, 32, 16, 16, 1>(n, fr, st);
+    case UW_24_8: return uwgrad_t<UC_SRC_BF16, 16, 8, 8, 0>(n, fr, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Max pool and its gradient
+// ---------------------------------------------------------------------------------------------
+struct PoolBatch {
+    int n, total;
+    int begin[XDN_MAX_FRAMES + 1];  // first item of each frame
+    PoolFrame f[XDN_MAX_FRAMES];
+};
+struct PoolBwBatch {
+    int n, total;
+    int begin[XDN_MAX_FRAMES + 1];
+    PoolBwFrame f[XDN_MAX_FRAMES];
+};
+
+__device__ __forceinline__ int item_frame(const int *begin, int n, int i)
+{
+    int f = 0;
+    while (f + 1 < n && i >= begin[f + 1]) ++f;
+    return f;
+}
+
+// one item = one pooled pixel x one 8-channel chunk
+template <int C>
+__global__ __launch_bounds__(256) void maxpool2(PoolBatch b)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= b.total) return;
+    const int f = item_frame(b.begin, b.n, i);
+    const PoolFrame &F = b.f[f];
+    const int W2 = F.W / 2;
+    const int li = i - b.begin[f], q = li % (C / 8), p2 = li / (C / 8);
+    const int y = p2 / W2, x = p2 - y * W2;
+    u32x4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        v[k] = *reinterpret_cast<const u32x4 *>(F.in + ((size_t)(2 * y + (k >> 1)) * F.W + 2 * x + (k & 1)) * C + q * 8);
+    uint32_t o[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+    {
+        float lo = -__builtin_inff(), hi = -__builtin_inff();
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const uint32_t word = w == 0 ? v[k].x : (w == 1 ? v[k].y : (w == 2 ? v[k].z : v[k].w));
+            lo = fmaxf(lo, u_lo(word));
+            hi = fmaxf(hi, u_hi(word));
+        }
+        o[w] = (__float_as_uint(lo) >> 16) | (__float_as_uint(hi) & 0xFFFF0000u);
+    }
+    *reinterpret_cast<u32x4 *>(F.out + (size_t)p2 * C + q * 8) = u32x4{o[0], o[1], o[2], o[3]};
+}
+
+// one item = one full-resolution pixel x one 8-channel chunk
+template <int C>
+__global__ __launch_bounds__(256) void pool_bwd(PoolBwBatch b)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= b.total) return;
+    const int f = item_frame(b.begin, b.n, i);
+    const PoolBwFrame &F = b.f[f];
+    const int W = F.W, W2 = W / 2;
+    const int li = i - b.begin[f], q = li % (C / 8), p = li / (C / 8);
+    const int y = p / W, x = p - y * W;
+    const int wy = y & ~1, wx = x & ~1, me = ((y & 1) << 1) | (x & 1);
+    const uint16_t *act = F.act;
+    const float *dp = F.dpool + ((size_t)(y >> 1) * W2 + (x >> 1)) * C + q * 8;
+    const float *sk = F.dskip + (size_t)p * F.sstride + F.soff + q * 8;
+    uint32_t o[4];
+#pragma unroll
+    for (int j = 0; j < 8; j += 2)
+    {
+        float r[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+        {
+            const int c = q * 8 + j + e;
+            // first maximum of the window in row-major order
+            int arg = 0;
+            float best = u_bf(act, ((size_t)wy * W + wx) * C + c);
+#pragma unroll
+            for (int k = 1; k < 4; ++k)
+            {
+                const float v = u_bf(act, ((size_t)(wy + (k >> 1)) * W + wx + (k & 1)) * C + c);
+                if (v > best) best = v, arg = k;
+            }
+            const float g = (arg == me ? dp[j + e] : 0.f) + sk[j + e];
+            const float a = u_bf(act, (size_t)p * C + c);
+            r[e] = a > 0.f ? g : 0.f;
+        }
+        o[j >> 1] = u_pack2(r[0], r[1]);
+    }
+    *reinterpret_cast<u32x4 *>(F.dy + (size_t)p * C + q * 8) = u32x4{o[0], o[1], o[2], o[3]};
+}
+
+template <int C>
+static hipError_t maxpool_t(int n, const PoolFrame *fr, hipStream_t st)
+{
+    PoolBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.begin[i] = acc;
+        acc += (fr[i].H / 2) * (fr[i].W / 2) * (C / 8);
+    }
+    b.begin[n] = b.total = acc;
+    if (acc == 0) return hipSuccess;
+    hipLaunchKernelGGL(maxpool2<C>, dim3(udiv_up(acc, 256)), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_maxpool(int C, int n, const PoolFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    if (C == 8) return maxpool_t<8>(n, fr, st);
+    if (C == 16) return maxpool_t<16>(n, fr, st);
+    return hipErrorInvalidValue;
+}
+
+template <int C>
+static hipError_t pool_bwd_t(int n, const PoolBwFrame *fr, hipStream_t st)
+{
+    PoolBwBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.begin[i] = acc;
+        acc += fr[i].H * fr[i].W * (C / 8);
+    }
+    b.begin[n] = b.total = acc;
+    if (acc == 0) return hipSuccess;
+    hipLaunchKernelGGL(pool_bwd<C>, dim3(udiv_up(acc, 256)), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool_bwd(int C, int n, const PoolBwFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    if (C == 8) return pool_bwd_t<8>(n, fr, st);
+    if (C == 16) return pool_bwd_t<16>(n, fr, st);
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Transposed 1x1 stride-2 convolution
+// ---------------------------------------------------------------------------------------------
+struct CtBatch {
+    int n;
+    int blk_begin[XDN_MAX_FRAMES + 1];
+    CtFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(CtBatch) <= 4096, "kernel argument too large");
+
+// forward: one thread per coarse pixel writes its 2x2 output block
+template <int C>
+__global__ __launch_bounds__(256) void convT1x1s2(CtBatch b)
+{
+    __shared__ float ws[C][C + 1];  // [co][ci]
+    __shared__ float bs[C];
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const CtFrame &F = b.f[f];
+    for (int e = threadIdx.x; e < C * C; e += 256) ws[e / C][e % C] = F.w[e];
+    if (threadIdx.x < C) bs[threadIdx.x] = F.b[threadIdx.x];
+    __syncthreads();
+    const int li = blockIdx.x - b.blk_begin[f];
+    const int p = li * 256 + threadIdx.x, h = F.h, w = F.w_;
+    if (p >= h * w) return;
+    const int y = p / w, x = p - y * w, W2 = 2 * w;
+    float in[C];
+#pragma unroll
+    for (int c = 0; c < C; c += 8)
+    {
+        const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * C + c);
+        in[c] = u_lo(v.x); in[c + 1] = u_hi(v.x); in[c + 2] = u_lo(v.y); in[c + 3] = u_hi(v.y);
+        in[c + 4] = u_lo(v.z); in[c + 5] = u_hi(v.z); in[c + 6] = u_lo(v.w); in[c + 7] = u_hi(v.w);
+    }
+    uint32_t outw[C / 2], biasw[C / 2];
+#pragma unroll
+    for (int co = 0; co < C; co += 2)
+    {
+        float s0 = bs[co], s1 = bs[co + 1];
+#pragma unroll
+        for (int ci = 0; ci < C; ++ci)
+        {
+            s0 += ws[co][ci] * in[ci];
+            s1 += ws[co + 1][ci] * in[ci];
+        }
+        outw[co / 2] = u_pack2(s0, s1);
+        biasw[co / 2] = u_pack2(bs[co], bs[co + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        uint16_t *o = F.out + ((size_t)(2 * y + (k >> 1)) * W2 + 2 * x + (k & 1)) * C;
+#pragma unroll
+        for (int c = 0; c < C / 2; c += 4)
+        {
+            const uint32_t *src = k == 0 ? outw : biasw;
+            *reinterpret_cast<u32x4 *>(o + 2 * c) = u32x4{src[c], src[c + 1], src[c + 2], src[c + 3]};
+        }
+    }
+}
+
+// backward: blocks walk coarse pixels in chunks of 256; dIn per pixel, dW = sum in (x) dOut(2y,2x)
+// through LDS, db = sum of dOut over all output positions.
+template <int C>
+__global__ __launch_bounds__(256) void convT_bwd(CtBatch b)
+{
+    __shared__ float ws[C][C + 1];   // [co][ci]
+    __shared__ float xin[256][C + 1];
+    __shared__ float gout[256][C + 1];
+    __shared__ float dbl[256 / 64][C];
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const CtFrame &F = b.f[f];
+    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
+    const int h = F.h, w = F.w_, np = h * w, W2 = 2 * w;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < C * C; e += 256) ws[e / C][e % C] = F.w[e];
+    // dW accumulators: thread t owns the (ci, co) pairs e = t + 256 k
+    constexpr int NE = C * C / 256 > 0 ? C * C / 256 : 1;
+    float dw[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) dw[k] = 0.f;
+    float dbp[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) dbp[c] = 0.f;
+    for (int base = li * 256; base < np; base += nblk * 256)
+    {
+        __syncthreads();  // ws written / previous chunk's LDS reads done
+        const int p = base + tid;
+        if (p < np)
+        {
+            const int y = p / w, x = p - y * w;
+#pragma unroll
+            for (int c = 0; c < C; c += 8)
+            {
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * C + c);
+                xin[tid][c] = u_lo(v.x); xin[tid][c + 1] = u_hi(v.x); xin[tid][c + 2] = u_lo(v.y);
+                xin[tid][c + 3] = u_hi(v.y); xin[tid][c + 4] = u_lo(v.z); xin[tid][c + 5] = u_hi(v.z);
+                xin[tid][c + 6] = u_lo(v.w); xin[tid][c + 7] = u_hi(v.w);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+            {
+                const float *g = F.dout + ((size_t)(2 * y + (k >> 1)) * W2 + 2 * x + (k & 1)) * F.dstride;
+#pragma unroll
+                for (int c = 0; c < C; ++c)
+                {
+                    const float v = g[c];
+                    dbp[c] += v;
+                    if (k == 0) gout[tid][c] = v;
+                }
+            }
+            // dIn = W dOut(2y,2x) (W is [co][ci]), masked by the ReLU of the input
+            uint32_t o[C / 2];
+#pragma unroll
+            for (int ci = 0; ci < C; ci += 2)
+            {
+                float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+                for (int co = 0; co < C; ++co)
+                {
+                    s0 += ws[co][ci] * gout[tid][co];
+                    s1 += ws[co][ci + 1] * gout[tid][co];
+                }
+                o[ci / 2] = u_pack2(xin[tid][ci] > 0.f ? s0 : 0.f, xin[tid][ci + 1] > 0.f ? s1 : 0.f);
+            }
+#pragma unroll
+            for (int c = 0; c < C / 2; c += 4)
+                *reinterpret_cast<u32x4 *>(F.din + (size_t)p * C + 2 * c) = u32x4{o[c], o[c + 1], o[c + 2], o[c + 3]};
+        }
+        else
+        {
+#pragma unroll
+            for (int c = 0; c < C; ++c) xin[tid][c] = gout[tid][c] = 0.f;
+        }
+        __syncthreads();
+        const int cnt = min(256, np - base);
+#pragma unroll
+        for (int k = 0; k < NE; ++k)
+        {
+            const int e = tid + 256 * k;
+            if (e >= C * C) break;
+            const int ci = e / C, co = e - ci * C;
+            float s = 0.f;
+            for (int q = 0; q < cnt; ++q) s += xin[q][ci] * gout[q][co];
+            dw[k] += s;
+        }
+    }
+    // partial: tap 0 rows [ci][co] of the [9][32][32] layout, bias at 9*1024
+    float *out = F.part + (size_t)li * WG_PART;
+#pragma unroll
+    for (int k = 0; k < NE; ++k)
+    {
+        const int e = tid + 256 * k;
+        if (e < C * C) out[(e / C) * 32 + (e % C)] = dw[k];
+    }
+    // bias: reduce dbp over the block (wave shuffles, then the 4 waves)
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+    {
+        float v = dbp[c];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        dbp[c] = v;
+    }
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < C; ++c) dbl[tid >> 6][c] = dbp[c];
+    __syncthreads();
+    if (tid < C) out[9 * 1024 + tid] = (dbl[0][tid] + dbl[1][tid]) + (dbl[2][tid] + dbl[3][tid]);
+}
+
+template <int C, bool BWD>
+static hipError_t convT_t(int n, const CtFrame *fr, hipStream_t st)
+{
+    CtBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.blk_begin[i] = acc;
+        acc += BWD ? fr[i].nblk : udiv_up(fr[i].h * fr[i].w_, 256);
+    }
+    b.blk_begin[n] = acc;
+    if (acc == 0) return hipSuccess;
+    if (BWD)
+        hipLaunchKernelGGL(convT_bwd<C>, dim3(acc), dim3(256), 0, st, b);
+    else
+        hipLaunchKernelGGL(convT1x1s2<C>, dim3(acc), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_convT(int C, int n, const CtFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    if (C == 16) return convT_t<16, false>(n, fr, st);
+    if (C == 32) return convT_t<32, false>(n, fr, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_convT_bwd(int C, int n, const CtFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    if (C == 16) return convT_t<16, true>(n, fr, st);
+    if (C == 32) return convT_t<32, true>(n, fr, st);
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Head: 1x1 convolution 8 -> 6
+// ---------------------------------------------------------------------------------------------
+struct HeadBatch {
+    int n;
+    int blk_begin[XDN_MAX_FRAMES + 1];
+    HeadFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(HeadBatch) <= 4096, "kernel argument too large");
+
+__global__ __launch_bounds__(256) void head1x1(HeadBatch b)
+{
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const HeadFrame &F = b.f[f];
+    const int p = (blockIdx.x - b.blk_begin[f]) * 256 + threadIdx.x;
+    if (p >= F.H * F.W) return;
+    const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * 8);
+    const float in[8] = {u_lo(v.x), u_hi(v.x), u_lo(v.y), u_hi(v.y), u_lo(v.z), u_hi(v.z), u_lo(v.w), u_hi(v.w)};
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+    {
+        float s = F.b[k];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s += in[c] * F.w[c * 6 + k];
+        F.param[(size_t)p * 6 + k] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void head_bwd(HeadBatch b)
+{
+    __shared__ float red[4][54];
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const HeadFrame &F = b.f[f];
+    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
+    const int np = F.H * F.W, tid = threadIdx.x;
+    float w[48];
+#pragma unroll
+    for (int e = 0; e < 48; ++e) w[e] = F.w[e];
+    float g[54];
+#pragma unroll
+    for (int e = 0; e < 54; ++e) g[e] = 0.f;
+    for (int p = li * 256 + tid; p < np; p += nblk * 256)
+    {
+        const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * 8);
+        const float in[8] = {u_lo(v.x), u_hi(v.x), u_lo(v.y), u_hi(v.y), u_lo(v.z), u_hi(v.z), u_lo(v.w), u_hi(v.w)};
+        float dp[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dp[k] = F.dparam[(size_t)p * 6 + k];
+        uint32_t o[4];
+#pragma unroll
+        for (int c = 0; c < 8; c += 2)
+        {
+            float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+            {
+                s0 += w[c * 6 + k] * dp[k];
+                s1 += w[(c + 1) * 6 + k] * dp[k];
+            }
+            o[c / 2] = u_pack2(in[c] > 0.f ? s0 : 0.f, in[c + 1] > 0.f ? s1 : 0.f);
+        }
+        *reinterpret_cast<u32x4 *>(F.din + (size_t)p * 8) = u32x4{o[0], o[1], o[2], o[3]};
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) g[c * 6 + k] += in[c] * dp[k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) g[48 + k] += dp[k];
+    }
+#pragma unroll
+    for (int e = 0; e < 54; ++e)
+    {
+        float v = g[e];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        g[e] = v;
+    }
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int e = 0; e < 54; ++e) red[tid >> 6][e] = g[e];
+    __syncthreads();
+    float *out = F.part + (size_t)li * WG_PART;
+    if (tid < 48) out[(tid / 6) * 32 + tid % 6] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+    else if (tid < 54) out[9 * 1024 + tid - 48] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+}
+
+hipError_t launch_head(int n, const HeadFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    HeadBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.blk_begin[i] = acc;
+        acc += udiv_up(fr[i].H * fr[i].W, 256);
+    }
+    b.blk_begin[n] = acc;
+    hipLaunchKernelGGL(head1x1, dim3(acc), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_head_bwd(int n, const HeadFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    HeadBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.blk_begin[i] = acc;
+        acc += fr[i].nblk;
+    }
+    b.blk_begin[n] = acc;
+    if (acc == 0) return hipSuccess;
+    hipLaunchKernelGGL(head_bwd, dim3(acc), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Weight packing: fwd frag[t][kc][nt][lane] = W[t][kc*16 + 8h + j][nt*32 + lane%32],
+// dgrad frag[t][kc][nt][lane] = W[8-t][nt*32 + lane%32][kc*16 + 8h + j]; zero outside ci x co
+// ---------------------------------------------------------------------------------------------
+struct UPackBatch {
+    int n;
+    UPackJob j[12 * 2];
+};
+static_assert(sizeof(UPackBatch) <= 4096, "kernel argument too large");
+
+__global__ __launch_bounds__(64) void upack(UPackBatch b)
+{
+    const UPackJob &J = b.j[blockIdx.y];
+    const int lane = threadIdx.x, h = lane >> 5, n = lane & 31;
+    const int kcf = (J.ci + 15) / 16, ntf = (J.co + 31) / 32;
+    const int kcd = (J.co + 15) / 16, ntd = (J.ci + 31) / 32;
+    const int nf = 9 * kcf * ntf, nd = J.frag_dgrad ? 9 * kcd * ntd : 0;
+    const int item = blockIdx.x;
+    if (item < nf)
+    {
+        const int t = item / (kcf * ntf), r = item - t * kcf * ntf, kc = r / ntf, nt = r - kc * ntf;
+        const int co = nt * 32 + n;
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 8; j += 2)
+        {
+            const int c0 = kc * 16 + 8 * h + j, c1 = c0 + 1;
+            const float a = (c0 < J.ci && co < J.co) ? J.w[(t * J.ci + c0) * J.co + co] : 0.f;
+            const float bb = (c1 < J.ci && co < J.co) ? J.w[(t * J.ci + c1) * J.co + co] : 0.f;
+            w[j >> 1] = u_pack2(a, bb);
+        }
+        static_cast<u32x4 *>(J.frag_fwd)[item * 64 + lane] = u32x4{w[0], w[1], w[2], w[3]};
+        if (t == 0 && kc == 0 && h == 0) J.bias32[nt * 32 + n] = co < J.co ? J.b[co] : 0.f;
+    }
+    else if (item < nf + nd)
+    {
+        const int it = item - nf;
+        const int t = it / (kcd * ntd), r = it - t * kcd * ntd, kc = r / ntd, nt = r - kc * ntd;
+        const int ci = nt * 32 + n;
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 8; j += 2)
+        {
+            const int o0 = kc * 16 + 8 * h + j, o1 = o0 + 1;
+            const float a = (o0 < J.co && ci < J.ci) ? J.w[((8 - t) * J.ci + ci) * J.co + o0] : 0.f;
+            const float bb = (o1 < J.co && ci < J.ci) ? J.w[((8 - t) * J.ci + ci) * J.co + o1] : 0.f;
+            w[j >> 1] = u_pack2(a, bb);
+        }
+        static_cast<u32x4 *>(J.frag_dgrad)[it * 64 + lane] = u32x4{w[0], w[1], w[2], w[3]};
+    }
+}
+
+hipError_t launch_upack(int n, const UPackJob *jobs, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > 24) return hipErrorInvalidValue;
+    UPackBatch b{};
+    b.n = n;
+    int maxi = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.j[i] = jobs[i];
+        const UPackJob &J = jobs[i];
+        const int items = 9 * ((J.ci + 15) / 16) * ((J.co + 31) / 32) +
+                          (J.frag_dgrad ? 9 * ((J.co + 15) / 16) * ((J.ci + 31) / 32) : 0);
+        maxi = items > maxi ? items : maxi;
+    }
+    hipLaunchKernelGGL(upack, dim3(maxi, n), dim3(64), 0, st, b);
+    return hipGetLastError();
+}
+
+}  // namespace xdn

@@ -47,13 +47,17 @@ struct PackJob {
 hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st);
 hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st);
 // One reduction job: the nblk partials of one layer of one frame -> its flat gradient.
+// The partials hold [taps][32 ci][32 co] + [32 co] of an input-channel slice ci_off .. ci_off +
+// ci_cnt - 1 (ci_cnt < 0: all ci); the gradient is written in the layer's flat layout
+// [taps][ci][co] (or [co][ci] when transpose, taps 1) followed by [co] when bias.
 struct RedJob {
     const float *part;  // [nblk][WG_PART]
     float *stage;       // [WG_SPLITS][WG_PART]
-    float *grad;        // [9][ci][co] then [co]
+    float *grad;        // the layer's weights, then its bias
     int nblk, ci, co;
+    int taps = 9, ci_off = 0, ci_cnt = -1, bias = 1, transpose = 0;
 };
-constexpr int WG_MAX_JOBS = 5 * XDN_MAX_FRAMES;
+constexpr int WG_MAX_JOBS = 64;  // per launch; longer job lists are split
 hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st);
 hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st);
 int wgrad_blocks(int H, int W);

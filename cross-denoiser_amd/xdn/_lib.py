@@ -51,6 +51,13 @@ PROTOTYPES = {
     "xdn_cnn5_pack": (_I, [_P, _P, _P]),
     "xdn_cnn5_forward": (_I, [_P] * 7 + [_P, _Z] + [_I] * 3 + [_P]),
     "xdn_cnn5_backward": (_I, [_P] * 8 + [_P, _Z] + [_I] * 3 + [_P]),
+    "xdn_unet_param_count": (_Z, []),
+    "xdn_unet_packed_bytes": (_Z, []),
+    "xdn_unet_workspace_size": (_Z, [_I, _I, _I]),
+    "xdn_unet_workspace_layout": (_I, [_I, _I, _I, _I, _P, _P]),
+    "xdn_unet_pack": (_I, [_P, _P, _P]),
+    "xdn_unet_forward": (_I, [_P] * 7 + [_P, _Z] + [_I] * 3 + [_P]),
+    "xdn_unet_backward": (_I, [_P] * 8 + [_P, _Z] + [_I] * 3 + [_P]),
     "xdn_slot_scratch_bytes": (_Z, [_I, _I]),
     "xdn_slot_scratch_offsets": (_I, [_I, _I, _P]),
     "xdn_slot_init": (_I, [_P, _P]),

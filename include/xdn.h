@@ -186,6 +186,34 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
                       size_t workspace_bytes, int batch, int height, int width, void *stream);
 
 /* ---------------------------------------------------------------------------------------------
+ * unet: the reference's own bandwidth network (scripts/main.py:83-118 unet_simple + the 1x1 head
+ * of :106-108; 31,414 parameters): conv3x3 15->8->8, 2x2 max pool, 8->16->16, pool, 16->32->32,
+ * transposed 1x1 stride-2 conv 32->32, concat with the 16-channel skip, conv3x3 48->16->16,
+ * transposed conv 16->16, concat with the 8-channel skip, conv3x3 24->8, conv1x1 8->6 (FP32).
+ * ReLU after every 3x3 convolution; the transposed convolutions are linear. 3x3 layers run on
+ * BF16 MFMA with FP32 accumulation, the 1x1 layers in FP32; activations are stored as bf16.
+ * Same calling convention as cnn5 (the two are interchangeable in front of xdn_filter_train):
+ *   params   FP32 [xdn_unet_param_count()] in Keras order, per layer kernel then bias: 3x3
+ *            kernels [3][3][ci][co], transposed kernels [co][ci] (Keras 1x1xCOxCI), head [ci][co]
+ *   packed   xdn_unet_packed_bytes(), rebuilt by xdn_unet_pack after every parameter update
+ *   workspace xdn_unet_workspace_size(B,H,W) bytes; height and width must be multiples of 4
+ * xdn_unet_workspace_layout gives the offset and size of workspace buffer `index` for
+ * inspection: 0-12 the activations c1_1, c1_2, pool1, c2_1, c2_2, pool2, c3_1, c3_2, c3_3, c4_1,
+ * c4_2, c4_3, c5_1 (bf16 [B][h][w][ch]); 13-25 the backward gradients in the order they are made.
+ * ------------------------------------------------------------------------------------------- */
+size_t xdn_unet_param_count(void);
+size_t xdn_unet_packed_bytes(void);
+size_t xdn_unet_workspace_size(int batch, int height, int width);
+int xdn_unet_workspace_layout(int batch, int height, int width, int index, size_t *offset, size_t *bytes);
+int xdn_unet_pack(const float *params, void *packed, void *stream);
+int xdn_unet_forward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,
+                     const float *normal, const void *packed, float *param, void *workspace, size_t workspace_bytes,
+                     int batch, int height, int width, void *stream);
+int xdn_unet_backward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,
+                      const float *normal, const float *dparam, const void *packed, float *grad, void *workspace,
+                      size_t workspace_bytes, int batch, int height, int width, void *stream);
+
+/* ---------------------------------------------------------------------------------------------
  * Batched online denoising step: train_step (scripts/main.py:250-313) for many render streams.
  * One call advances every slot by one frame, each stage being ONE launch over all slots (their
  * resolutions may differ): cross-regression with the log1p prologue, reprojection of the history
