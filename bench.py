@@ -49,7 +49,51 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--pipeline", action="store_true",
                     help="overlap frame t+1's cross-regression with frame t on a second HIP stream")
+    ap.add_argument("--no-unet", action="store_true",
+                    help="skip the secondary measurement of the step with the reference's U-Net")
     return ap.parse_args()
+
+
+def unet_leg(frames, H, W, steps, warmup, reps, torch):
+    """The same per-frame step with the reference's own bandwidth network (the U-Net of
+    scripts/main.py:83-118) in place of cnn5, and the U-Net's forward / backward alone on that
+    step's network inputs (HIP events on the stream they run on)."""
+    from xdn import cnn, pipeline
+    dev = frames[0]["color"].device
+    den = pipeline.Denoiser(H, W, device=dev, network="unet")
+    st = torch.cuda.current_stream()
+    for i in range(warmup):
+        den.step(frames[i % len(frames)])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    for i in range(steps):
+        den.step(frames[(warmup + i) % len(frames)])
+    e1.record(st)
+    torch.cuda.synchronize()
+    secs = max(time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3)
+    f = frames[(warmup + steps - 1) % len(frames)]
+    net = cnn.HipUNet(dev, shape=(1, H, W))
+    inputs = (den.outputs()["ols1"], den.outputs()["ols2"], den.buffer("prev_out", 3), f["albedo"], f["normal"])
+    dparam = den.buffer("dparam", 6).clone()
+    net.forward(*inputs)
+    net.backward(dparam)
+    res = {}
+    for name, fn in (("forward_ms", lambda: net.forward(*inputs)), ("backward_ms", lambda: net.backward(dparam))):
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        res[name] = round(e0.elapsed_time(e1) / reps, 4)
+    quality = round(pipeline.rel_l2(den.outputs()["out"], f["ref"]), 5)
+    den.close()
+    return {"network": "unet (scripts/main.py:83-118, 31,414 parameters)", "steps": steps, "warmup": warmup,
+            "value": round(steps / secs, 3), "unit": "frames/s", "ms_per_step": round(1e3 * secs / steps, 3),
+            **res, "relL2_output": quality,
+            "paper_ms_rtx4090": {"unet_inference": 4.60, "backpropagation": 11.30}}
 
 
 def stage_times(den, frames, reps, torch):
@@ -264,6 +308,10 @@ def main():
                 "note": "VALU-bound FP32 kernel (no MFMA); peak is the published FP32 rate (the non-packed VALU "
                         "rate is half of it); traffic = FETCH_SIZE + WRITE_SIZE bytes per launch from the committed "
                         "counter profile, copy-calibrated"}
+    unet = None
+    if not args.no_unet:
+        unet = unet_leg(frames, H, W, max(8, args.steps // 2), 4, args.kernel_reps, torch)
+        print(f"[bench] unet step: {unet}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(torch, H, W)
@@ -282,6 +330,7 @@ def main():
         ("ms_per_step_frame_by_frame" if pipelined else "ms_per_step_pipelined"): round(1e3 * secs_other / args.steps, 3),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "quality": quality,
+        "reference_network_step": unet,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 #include "../../include/xdn.h"
 #include "xdn_cnn.h"
 #include "xdn_common.h"
@@ -214,5 +216,27 @@ struct UnetWs {
     }
     void *at(int i, int b) const { return buf[i] + (size_t)b * frame_bytes[i]; }
 };
+
+// One image of a U-Net launch set (abi_unet.cpp): its workspace view, inputs and outputs. The
+// images of one set may differ in size and weights; each launch covers all of them.
+struct UnetFrame {
+    char *buf[UNET_BUFS];
+    float *part[UNET_JOBS];
+    int nblk[UNET_JOBS], h[3], w[3];
+    const float *img[5];   // the five 3-channel network inputs [h0][w0][3]
+    const char *pk;        // packed weights
+    float *param;          // network output [h0][w0][6]
+    const float *dparam;   // its gradient (backward)
+};
+// image b of a workspace laid out for B images
+UnetFrame unet_frame(const UnetWs &ws, int b, const float *const img[5], const void *packed, float *param,
+                     const float *dparam);
+int unet_forward_frames(const UnetFrame *f, int n, hipStream_t st);
+// the backward pass up to the weight-gradient partials (reduced by the caller's jobs)
+int unet_backward_frames(const UnetFrame *f, int n, hipStream_t st);
+// reduction job of partial set `set` (nblk_total partials from part) into the flat gradient
+xdn::RedJob unet_red_job(int set, float *part, float *stage, float *grad, int nblk_total);
+// pack jobs of one parameter vector (3x3 fragments and FP32 copies of the 1x1 layers)
+void unet_pack_jobs(const float *params, void *packed, std::vector<xdn::UPackJob> &jobs);
 
 }  // namespace xdn_abi

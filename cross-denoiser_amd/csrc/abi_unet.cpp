@@ -1,5 +1,6 @@
-// extern "C" entry points of the U-Net (declared in include/xdn.h): the reference's bandwidth
-// network, scripts/main.py:83-118, forward and backward as launches of the k_unet.hip kernels.
+// The U-Net (the reference's bandwidth network, scripts/main.py:83-118): forward and backward as
+// launches of the k_unet.hip kernels over a set of images (each with its own size, workspace and
+// weights), and the extern "C" entry points of include/xdn.h on top of them.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -12,56 +13,273 @@ using namespace xdn_abi;
 
 namespace {
 
-struct Ctx {
-    const UnetWs &ws;
-    const char *pk;
-    const float *img[5];
-    int B;
-    hipStream_t st;
-    size_t n0() const { return (size_t)ws.h[0] * ws.w[0]; }
-};
+using Frames = const UnetFrame *;
 
-const void *pk_at(const Ctx &c, size_t off) { return c.pk + off; }
-
-// 3x3 convolution of layer l: forward (dgrad = false) from concat[a, b] (or the images when a < 0)
-// into buffer out; data gradient from buffer a into out, masked by buffer mask (or FP32 when
-// mask < 0).
-int conv(const Ctx &c, int kind, int l, int a, int b, int out, int mask, bool dgrad, const char *what)
+// 3x3 convolution of layer l over every image: forward from concat[a, b] (the images when a < 0)
+// into buffer out, or (dgrad) data gradient from buffer a into out, masked by buffer mask's ReLU
+// (FP32 output when mask < 0).
+int conv(Frames F, int n, hipStream_t st, int kind, int l, int a, int b, int out, int mask, bool dgrad,
+         const char *what)
 {
-    const int lv = dgrad ? kUnetBuf[out].level : kUnet[l].level;
-    std::vector<xdn::UcFrame> fr(c.B);
-    for (int i = 0; i < c.B; ++i)
+    const int lv = kUnet[l].level;
+    std::vector<xdn::UcFrame> fr(n);
+    for (int i = 0; i < n; ++i)
     {
+        const UnetFrame &U = F[i];
         xdn::UcFrame &f = fr[i];
         memset(&f, 0, sizeof(f));
         if (a >= 0)
-            f.a = c.ws.at(a, i);
+            f.a = U.buf[a];
         else
-            for (int k = 0; k < 5; ++k) f.img[k] = c.img[k] + (size_t)i * c.n0() * 3;
-        f.b = b >= 0 ? c.ws.at(b, i) : nullptr;
-        f.out = c.ws.at(out, i);
-        f.mask = mask >= 0 ? static_cast<const uint16_t *>(c.ws.at(mask, i)) : nullptr;
-        f.wfrag = pk_at(c, dgrad ? kULay.dg_off[l] : kULay.fwd_off[l]);
-        f.bias = reinterpret_cast<const float *>(pk_at(c, kULay.bias_off[l]));
-        f.H = c.ws.h[lv];
-        f.W = c.ws.w[lv];
+            for (int k = 0; k < 5; ++k) f.img[k] = U.img[k];
+        f.b = b >= 0 ? U.buf[b] : nullptr;
+        f.out = U.buf[out];
+        f.mask = mask >= 0 ? reinterpret_cast<const uint16_t *>(U.buf[mask]) : nullptr;
+        f.wfrag = U.pk + (dgrad ? kULay.dg_off[l] : kULay.fwd_off[l]);
+        f.bias = reinterpret_cast<const float *>(U.pk + kULay.bias_off[l]);
+        f.H = U.h[lv];
+        f.W = U.w[lv];
     }
-    return hip_check(xdn::launch_uconv(kind, c.B, fr.data(), c.st), what);
+    return hip_check(xdn::launch_uconv(kind, n, fr.data(), st), what);
 }
 
-int pool(const Ctx &c, int C, int in, int out)
+int pool(Frames F, int n, hipStream_t st, int C, int in, int out)
 {
     const int lv = kUnetBuf[in].level;
-    std::vector<xdn::PoolFrame> fr(c.B);
-    for (int i = 0; i < c.B; ++i)
-        fr[i] = xdn::PoolFrame{static_cast<const uint16_t *>(c.ws.at(in, i)), static_cast<uint16_t *>(c.ws.at(out, i)),
-                               c.ws.h[lv], c.ws.w[lv]};
-    return hip_check(xdn::launch_maxpool(C, c.B, fr.data(), c.st), "unet max pool");
+    std::vector<xdn::PoolFrame> fr(n);
+    for (int i = 0; i < n; ++i)
+        fr[i] = xdn::PoolFrame{reinterpret_cast<const uint16_t *>(F[i].buf[in]),
+                               reinterpret_cast<uint16_t *>(F[i].buf[out]), F[i].h[lv], F[i].w[lv]};
+    return hip_check(xdn::launch_maxpool(C, n, fr.data(), st), "unet max pool");
 }
 
-const float *master(const Ctx &c, int l) { return reinterpret_cast<const float *>(pk_at(c, kULay.master_off[l])); }
+const float *master(const UnetFrame &U, int l) { return reinterpret_cast<const float *>(U.pk + kULay.master_off[l]); }
+
+// transposed layer l forward: buffer in -> buffer out
+int convT(Frames F, int n, hipStream_t st, int l, int in, int out)
+{
+    const int lv = kUnet[l].level;
+    std::vector<xdn::CtFrame> fr(n);
+    for (int i = 0; i < n; ++i)
+    {
+        xdn::CtFrame &f = fr[i];
+        memset(&f, 0, sizeof(f));
+        f.in = reinterpret_cast<const uint16_t *>(F[i].buf[in]);
+        f.out = reinterpret_cast<uint16_t *>(F[i].buf[out]);
+        f.w = master(F[i], l);
+        f.b = f.w + kUnet[l].ci * kUnet[l].co;
+        f.h = F[i].h[lv];
+        f.w_ = F[i].w[lv];
+    }
+    return hip_check(xdn::launch_convT(kUnet[l].co, n, fr.data(), st), "unet transposed conv");
+}
+
+// weight gradient of partial set j over concat[a, b] (or the images) and dY buffer dy
+int wgrad(Frames F, int n, hipStream_t st, int kind, int j, int a, int b, int dy)
+{
+    const int lv = kUnetJob[j].level;
+    std::vector<xdn::UwFrame> fr(n);
+    for (int i = 0; i < n; ++i)
+    {
+        const UnetFrame &U = F[i];
+        xdn::UwFrame &f = fr[i];
+        memset(&f, 0, sizeof(f));
+        if (a >= 0)
+            f.a = U.buf[a];
+        else
+            for (int k = 0; k < 5; ++k) f.img[k] = U.img[k];
+        f.b = b >= 0 ? U.buf[b] : nullptr;
+        f.dy = reinterpret_cast<const uint16_t *>(U.buf[dy]);
+        f.part = U.part[j];
+        f.nblk = U.nblk[j];
+        f.H = U.h[lv];
+        f.W = U.w[lv];
+    }
+    return hip_check(xdn::launch_uwgrad(kind, n, fr.data(), st), "unet wgrad");
+}
+
+// transposed layer l backward (partial set j): input buffer in, FP32 gradient of its output in
+// buffer dout (its first C channels); the gradient into its input -> din, masked by in's ReLU
+int convT_bwd(Frames F, int n, hipStream_t st, int l, int j, int in, int dout, int din)
+{
+    const int lv = kUnet[l].level;
+    std::vector<xdn::CtFrame> fr(n);
+    for (int i = 0; i < n; ++i)
+    {
+        const UnetFrame &U = F[i];
+        xdn::CtFrame &f = fr[i];
+        memset(&f, 0, sizeof(f));
+        f.in = reinterpret_cast<const uint16_t *>(U.buf[in]);
+        f.dout = reinterpret_cast<const float *>(U.buf[dout]);
+        f.din = reinterpret_cast<uint16_t *>(U.buf[din]);
+        f.w = master(U, l);
+        f.b = f.w + kUnet[l].ci * kUnet[l].co;
+        f.part = U.part[j];
+        f.h = U.h[lv];
+        f.w_ = U.w[lv];
+        f.dstride = kUnetBuf[dout].ch;
+        f.nblk = U.nblk[j];
+    }
+    return hip_check(xdn::launch_convT_bwd(kUnet[l].co, n, fr.data(), st), "unet transposed conv backward");
+}
+
+// gradient into a pooled activation: pooled gradient dp + skip gradient (channels soff.. of
+// buffer skip) -> dy, masked by act's ReLU
+int pool_bwd(Frames F, int n, hipStream_t st, int C, int dp, int act, int skip, int soff, int dy)
+{
+    const int lv = kUnetBuf[act].level;
+    std::vector<xdn::PoolBwFrame> fr(n);
+    for (int i = 0; i < n; ++i)
+    {
+        const UnetFrame &U = F[i];
+        fr[i] = xdn::PoolBwFrame{reinterpret_cast<const float *>(U.buf[dp]), reinterpret_cast<const uint16_t *>(U.buf[act]),
+                                 reinterpret_cast<const float *>(U.buf[skip]), reinterpret_cast<uint16_t *>(U.buf[dy]),
+                                 U.h[lv], U.w[lv], kUnetBuf[skip].ch, soff};
+    }
+    return hip_check(xdn::launch_pool_bwd(C, n, fr.data(), st), "unet max pool backward");
+}
+
+int head(Frames F, int n, hipStream_t st, bool bwd)
+{
+    std::vector<xdn::HeadFrame> fr(n);
+    for (int i = 0; i < n; ++i)
+    {
+        const UnetFrame &U = F[i];
+        xdn::HeadFrame &f = fr[i];
+        memset(&f, 0, sizeof(f));
+        f.in = reinterpret_cast<const uint16_t *>(U.buf[UB_A10]);
+        f.param = U.param;
+        f.dparam = U.dparam;
+        f.din = reinterpret_cast<uint16_t *>(U.buf[UB_DY10]);
+        f.w = master(U, 11);
+        f.b = f.w + 48;
+        f.part = U.part[12];
+        f.H = U.h[0];
+        f.W = U.w[0];
+        f.nblk = U.nblk[12];
+    }
+    return bwd ? hip_check(xdn::launch_head_bwd(n, fr.data(), st), "unet head backward")
+               : hip_check(xdn::launch_head(n, fr.data(), st), "unet head");
+}
 
 }  // namespace
+
+namespace xdn_abi {
+
+UnetFrame unet_frame(const UnetWs &ws, int b, const float *const img[5], const void *packed, float *param,
+                     const float *dparam)
+{
+    UnetFrame U;
+    for (int i = 0; i < UNET_BUFS; ++i) U.buf[i] = static_cast<char *>(ws.at(i, b));
+    for (int j = 0; j < UNET_JOBS; ++j)
+    {
+        U.part[j] = ws.part[j] + (size_t)b * ws.nblk[j] * xdn::WG_PART;
+        U.nblk[j] = ws.nblk[j];
+    }
+    for (int k = 0; k < 3; ++k) U.h[k] = ws.h[k], U.w[k] = ws.w[k];
+    for (int k = 0; k < 5; ++k) U.img[k] = img[k];
+    U.pk = static_cast<const char *>(packed);
+    U.param = param;
+    U.dparam = dparam;
+    return U;
+}
+
+int unet_forward_frames(const UnetFrame *F, int n, hipStream_t st)
+{
+    using namespace xdn;
+    int e = 0;
+    if ((e = conv(F, n, st, UC_F_0, 0, -1, -1, UB_A0, -1, false, "unet c1_1"))) return e;
+    if ((e = conv(F, n, st, UC_F_8_8, 1, UB_A0, -1, UB_A1, -1, false, "unet c1_2"))) return e;
+    if ((e = pool(F, n, st, 8, UB_A1, UB_P1))) return e;
+    if ((e = conv(F, n, st, UC_F_8_16, 2, UB_P1, -1, UB_A2, -1, false, "unet c2_1"))) return e;
+    if ((e = conv(F, n, st, UC_F_16_16, 3, UB_A2, -1, UB_A3, -1, false, "unet c2_2"))) return e;
+    if ((e = pool(F, n, st, 16, UB_A3, UB_P2))) return e;
+    if ((e = conv(F, n, st, UC_F_16_32, 4, UB_P2, -1, UB_A4, -1, false, "unet c3_1"))) return e;
+    if ((e = conv(F, n, st, UC_F_32_32, 5, UB_A4, -1, UB_A5, -1, false, "unet c3_2"))) return e;
+    if ((e = convT(F, n, st, 6, UB_A5, UB_A6))) return e;                                       // c3_3
+    if ((e = conv(F, n, st, UC_F_48_16, 7, UB_A6, UB_A3, UB_A7, -1, false, "unet c4_1"))) return e;  // [c3_3, c2_2]
+    if ((e = conv(F, n, st, UC_F_16_16, 8, UB_A7, -1, UB_A8, -1, false, "unet c4_2"))) return e;
+    if ((e = convT(F, n, st, 9, UB_A8, UB_A9))) return e;                                       // c4_3
+    if ((e = conv(F, n, st, UC_F_24_8, 10, UB_A9, UB_A1, UB_A10, -1, false, "unet c5_1"))) return e;  // [c4_3, c1_2]
+    return head(F, n, st, false);
+}
+
+int unet_backward_frames(const UnetFrame *F, int n, hipStream_t st)
+{
+    using namespace xdn;
+    int e = 0;
+    if ((e = head(F, n, st, true))) return e;                                    // dY10, set 12
+    if ((e = wgrad(F, n, st, UW_24_8, 11, UB_A9, UB_A1, UB_DY10))) return e;     // c5_1
+    if ((e = conv(F, n, st, UC_D_8_24, 10, UB_DY10, -1, UB_DC5, -1, true, "unet c5_1 dgrad"))) return e;
+    if ((e = convT_bwd(F, n, st, 9, 10, UB_A8, UB_DC5, UB_DY8))) return e;       // c4_3 <- dC5[:, 0:16]
+    if ((e = wgrad(F, n, st, UW_16_16, 9, UB_A7, -1, UB_DY8))) return e;         // c4_2
+    if ((e = conv(F, n, st, UC_D_16_16, 8, UB_DY8, -1, UB_DY7, UB_A7, true, "unet c4_2 dgrad"))) return e;
+    if ((e = wgrad(F, n, st, UW_48_16_M0, 7, UB_A6, UB_A3, UB_DY7))) return e;   // c4_1, inputs 0-31
+    if ((e = wgrad(F, n, st, UW_48_16_M1, 8, UB_A6, UB_A3, UB_DY7))) return e;   // c4_1, inputs 32-47
+    if ((e = conv(F, n, st, UC_D_16_48, 7, UB_DY7, -1, UB_DC4, -1, true, "unet c4_1 dgrad"))) return e;
+    if ((e = convT_bwd(F, n, st, 6, 6, UB_A5, UB_DC4, UB_DY5))) return e;        // c3_3 <- dC4[:, 0:32]
+    if ((e = wgrad(F, n, st, UW_32_32, 5, UB_A4, -1, UB_DY5))) return e;         // c3_2
+    if ((e = conv(F, n, st, UC_D_32_32, 5, UB_DY5, -1, UB_DY4, UB_A4, true, "unet c3_2 dgrad"))) return e;
+    if ((e = wgrad(F, n, st, UW_16_32, 4, UB_P2, -1, UB_DY4))) return e;         // c3_1
+    if ((e = conv(F, n, st, UC_D_32_16, 4, UB_DY4, -1, UB_DP2, -1, true, "unet c3_1 dgrad"))) return e;
+    // c2_2's output feeds the second pool and the skip into c4_1 (dC4[:, 32:48])
+    if ((e = pool_bwd(F, n, st, 16, UB_DP2, UB_A3, UB_DC4, 32, UB_DY3))) return e;
+    if ((e = wgrad(F, n, st, UW_16_16, 3, UB_A2, -1, UB_DY3))) return e;         // c2_2
+    if ((e = conv(F, n, st, UC_D_16_16, 3, UB_DY3, -1, UB_DY2, UB_A2, true, "unet c2_2 dgrad"))) return e;
+    if ((e = wgrad(F, n, st, UW_8_16, 2, UB_P1, -1, UB_DY2))) return e;          // c2_1
+    if ((e = conv(F, n, st, UC_D_16_8, 2, UB_DY2, -1, UB_DP1, -1, true, "unet c2_1 dgrad"))) return e;
+    // c1_2's output feeds the first pool and the skip into c5_1 (dC5[:, 16:24])
+    if ((e = pool_bwd(F, n, st, 8, UB_DP1, UB_A1, UB_DC5, 16, UB_DY1))) return e;
+    if ((e = wgrad(F, n, st, UW_8_8, 1, UB_A0, -1, UB_DY1))) return e;           // c1_2
+    if ((e = conv(F, n, st, UC_D_8_8, 1, UB_DY1, -1, UB_DY0, UB_A0, true, "unet c1_2 dgrad"))) return e;
+    return wgrad(F, n, st, UW_0, 0, -1, -1, UB_DY0);                             // c1_1
+}
+
+xdn::RedJob unet_red_job(int set, float *part, float *stage, float *grad, int nblk_total)
+{
+    const int l = kUnetJob[set].layer;
+    const UnetLayer &L = kUnet[l];
+    xdn::RedJob r;
+    r.part = part;
+    r.stage = stage;
+    r.grad = grad + kULay.w_off[l];
+    r.nblk = nblk_total;
+    r.ci = L.ci;
+    r.co = L.co;
+    r.taps = L.type == UL_CONV ? 9 : 1;
+    r.transpose = L.type == UL_CONVT;  // Keras kernel [co][ci]; partials hold [ci][co]
+    if (set == 7) r.ci_off = 0, r.ci_cnt = 32;               // c4_1, input channels 0-31 (+ bias)
+    if (set == 8) r.ci_off = 32, r.ci_cnt = 16, r.bias = 0;  // c4_1, input channels 32-47
+    return r;
+}
+
+void unet_pack_jobs(const float *params, void *packed, std::vector<xdn::UPackJob> &jobs)
+{
+    char *pk = static_cast<char *>(packed);
+    for (int l = 0; l < UNET_LAYERS; ++l)
+    {
+        const UnetLayer &L = kUnet[l];
+        xdn::UPackJob j{};
+        j.w = params + kULay.w_off[l];
+        j.b = params + kULay.b_off[l];
+        if (L.type == UL_CONV)
+        {
+            j.frag_fwd = pk + kULay.fwd_off[l];
+            j.frag_dgrad = l > 0 ? pk + kULay.dg_off[l] : nullptr;
+            j.bias32 = reinterpret_cast<float *>(pk + kULay.bias_off[l]);
+            j.ci = L.ci;
+            j.co = L.co;
+        }
+        else
+        {
+            j.frag_fwd = pk + kULay.master_off[l];
+            j.copy_n = L.ci * L.co + L.co;  // kernel then bias, contiguous
+        }
+        jobs.push_back(j);
+    }
+}
+
+}  // namespace xdn_abi
 
 extern "C" {
 
@@ -87,32 +305,8 @@ int xdn_unet_workspace_layout(int batch, int height, int width, int index, size_
 
 int xdn_unet_pack(const float *params, void *packed, void *stream)
 {
-    char *pk = static_cast<char *>(packed);
     std::vector<xdn::UPackJob> jobs;
-    for (int l = 0; l < UNET_LAYERS; ++l)
-    {
-        const UnetLayer &L = kUnet[l];
-        if (L.type == UL_CONV)
-        {
-            xdn::UPackJob j{};
-            j.w = params + kULay.w_off[l];
-            j.b = params + kULay.b_off[l];
-            j.frag_fwd = pk + kULay.fwd_off[l];
-            j.frag_dgrad = l > 0 ? pk + kULay.dg_off[l] : nullptr;
-            j.bias32 = reinterpret_cast<float *>(pk + kULay.bias_off[l]);
-            j.ci = L.ci;
-            j.co = L.co;
-            jobs.push_back(j);
-        }
-        else
-        {
-            const size_t n = (size_t)L.ci * L.co + L.co;  // kernel then bias, contiguous
-            if (int e = hip_check(hipMemcpyAsync(pk + kULay.master_off[l], params + kULay.w_off[l], n * 4,
-                                                 hipMemcpyDeviceToDevice, S(stream)),
-                                  "unet pack copy"))
-                return e;
-        }
-    }
+    unet_pack_jobs(params, packed, jobs);
     return hip_check(xdn::launch_upack((int)jobs.size(), jobs.data(), S(stream)), "unet pack");
 }
 
@@ -126,60 +320,30 @@ static int unet_check(int B, int H, int W, size_t ws_bytes)
     return XDN_OK;
 }
 
+static std::vector<UnetFrame> batch_frames(const UnetWs &ws, int B, const float *const in[5], const void *packed,
+                                           float *param, const float *dparam)
+{
+    const size_t n0 = px(ws.h[0], ws.w[0]);
+    std::vector<UnetFrame> F(B);
+    for (int b = 0; b < B; ++b)
+    {
+        const float *img[5];
+        for (int k = 0; k < 5; ++k) img[k] = in[k] + (size_t)b * n0 * 3;
+        F[b] = unet_frame(ws, b, img, packed, param ? param + (size_t)b * n0 * 6 : nullptr,
+                          dparam ? dparam + (size_t)b * n0 * 6 : nullptr);
+    }
+    return F;
+}
+
 int xdn_unet_forward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,
                      const float *normal, const void *packed, float *param, void *workspace, size_t workspace_bytes,
                      int batch, int height, int width, void *stream)
 {
     if (int e = unet_check(batch, height, width, workspace_bytes)) return e;
     const UnetWs ws(workspace, batch, height, width);
-    const Ctx c{ws, static_cast<const char *>(packed), {ols1, ols2, prev_out, albedo, normal}, batch, S(stream)};
-    using namespace xdn;
-    int e = 0;
-    if ((e = conv(c, UC_F_0, 0, -1, -1, UB_A0, -1, false, "unet c1_1"))) return e;
-    if ((e = conv(c, UC_F_8_8, 1, UB_A0, -1, UB_A1, -1, false, "unet c1_2"))) return e;
-    if ((e = pool(c, 8, UB_A1, UB_P1))) return e;
-    if ((e = conv(c, UC_F_8_16, 2, UB_P1, -1, UB_A2, -1, false, "unet c2_1"))) return e;
-    if ((e = conv(c, UC_F_16_16, 3, UB_A2, -1, UB_A3, -1, false, "unet c2_2"))) return e;
-    if ((e = pool(c, 16, UB_A3, UB_P2))) return e;
-    if ((e = conv(c, UC_F_16_32, 4, UB_P2, -1, UB_A4, -1, false, "unet c3_1"))) return e;
-    if ((e = conv(c, UC_F_32_32, 5, UB_A4, -1, UB_A5, -1, false, "unet c3_2"))) return e;
-    std::vector<CtFrame> ct(batch);
-    for (int l : {6, 9})
-    {
-        // c3_3: A5 (quarter) -> A6 (half); c4_3: A8 (half) -> A9 (full)
-        const int in = l == 6 ? UB_A5 : UB_A8, out = l == 6 ? UB_A6 : UB_A9, lv = kUnet[l].level;
-        for (int i = 0; i < batch; ++i)
-        {
-            CtFrame &f = ct[i];
-            memset(&f, 0, sizeof(f));
-            f.in = static_cast<const uint16_t *>(ws.at(in, i));
-            f.out = static_cast<uint16_t *>(ws.at(out, i));
-            f.w = master(c, l);
-            f.b = f.w + kUnet[l].ci * kUnet[l].co;
-            f.h = ws.h[lv];
-            f.w_ = ws.w[lv];
-        }
-        if ((e = hip_check(launch_convT(kUnet[l].co, batch, ct.data(), c.st), "unet transposed conv"))) return e;
-        if (l == 6)
-        {
-            if ((e = conv(c, UC_F_48_16, 7, UB_A6, UB_A3, UB_A7, -1, false, "unet c4_1"))) return e;
-            if ((e = conv(c, UC_F_16_16, 8, UB_A7, -1, UB_A8, -1, false, "unet c4_2"))) return e;
-        }
-    }
-    if ((e = conv(c, UC_F_24_8, 10, UB_A9, UB_A1, UB_A10, -1, false, "unet c5_1"))) return e;
-    std::vector<HeadFrame> hf(batch);
-    for (int i = 0; i < batch; ++i)
-    {
-        HeadFrame &f = hf[i];
-        memset(&f, 0, sizeof(f));
-        f.in = static_cast<const uint16_t *>(ws.at(UB_A10, i));
-        f.param = param + (size_t)i * c.n0() * 6;
-        f.w = master(c, 11);
-        f.b = f.w + 48;
-        f.H = height;
-        f.W = width;
-    }
-    return hip_check(launch_head(batch, hf.data(), c.st), "unet head");
+    const float *in[5] = {ols1, ols2, prev_out, albedo, normal};
+    const std::vector<UnetFrame> F = batch_frames(ws, batch, in, packed, param, nullptr);
+    return unet_forward_frames(F.data(), batch, S(stream));
 }
 
 int xdn_unet_backward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,
@@ -188,146 +352,13 @@ int xdn_unet_backward(const float *ols1, const float *ols2, const float *prev_ou
 {
     if (int e = unet_check(batch, height, width, workspace_bytes)) return e;
     const UnetWs ws(workspace, batch, height, width);
-    const Ctx c{ws, static_cast<const char *>(packed), {ols1, ols2, prev_out, albedo, normal}, batch, S(stream)};
-    using namespace xdn;
-    int e = 0;
-    std::vector<RedJob> jobs;
-    auto red = [&](int j, int taps, int ci_off, int ci_cnt, int bias, int transpose) {
-        const int l = kUnetJob[j].layer;
-        RedJob r;
-        r.part = ws.part[j];
-        r.stage = ws.stage[j];
-        r.grad = grad + kULay.w_off[l];
-        r.nblk = batch * ws.nblk[j];
-        r.ci = kUnet[l].ci;
-        r.co = kUnet[l].co;
-        r.taps = taps;
-        r.ci_off = ci_off;
-        r.ci_cnt = ci_cnt;
-        r.bias = bias;
-        r.transpose = transpose;
-        jobs.push_back(r);
-    };
-    // weight gradient of partial set j over concat[a, b] (or the images) and dY buffer dy
-    auto wgrad = [&](int kind, int j, int a, int b, int dy) {
-        const int lv = kUnetJob[j].level;
-        std::vector<UwFrame> fr(batch);
-        for (int i = 0; i < batch; ++i)
-        {
-            UwFrame &f = fr[i];
-            memset(&f, 0, sizeof(f));
-            if (a >= 0)
-                f.a = ws.at(a, i);
-            else
-                for (int k = 0; k < 5; ++k) f.img[k] = c.img[k] + (size_t)i * c.n0() * 3;
-            f.b = b >= 0 ? ws.at(b, i) : nullptr;
-            f.dy = static_cast<const uint16_t *>(ws.at(dy, i));
-            f.part = ws.part[j] + (size_t)i * ws.nblk[j] * WG_PART;
-            f.nblk = ws.nblk[j];
-            f.H = ws.h[lv];
-            f.W = ws.w[lv];
-        }
-        return hip_check(launch_uwgrad(kind, batch, fr.data(), c.st), "unet wgrad");
-    };
-    // transposed layer l (partial set j): input buffer in, gradient of its output in FP32 buffer
-    // dout (first C of its channels), gradient into its input -> din (masked by in's ReLU)
-    auto convT_bwd = [&](int l, int j, int in, int dout, int din) {
-        const int lv = kUnet[l].level;
-        std::vector<CtFrame> fr(batch);
-        for (int i = 0; i < batch; ++i)
-        {
-            CtFrame &f = fr[i];
-            memset(&f, 0, sizeof(f));
-            f.in = static_cast<const uint16_t *>(ws.at(in, i));
-            f.dout = static_cast<const float *>(ws.at(dout, i));
-            f.din = static_cast<uint16_t *>(ws.at(din, i));
-            f.w = master(c, l);
-            f.b = f.w + kUnet[l].ci * kUnet[l].co;
-            f.part = ws.part[j] + (size_t)i * ws.nblk[j] * WG_PART;
-            f.h = ws.h[lv];
-            f.w_ = ws.w[lv];
-            f.dstride = kUnetBuf[dout].ch;
-            f.nblk = ws.nblk[j];
-        }
-        return hip_check(launch_convT_bwd(kUnet[l].co, batch, fr.data(), c.st), "unet transposed conv backward");
-    };
-    // gradient into a pooled activation: pooled gradient dp + skip gradient (channels soff.. of
-    // buffer skip) -> dy, masked by act's ReLU
-    auto pool_bwd = [&](int C, int dp, int act, int skip, int soff, int dy) {
-        const int lv = kUnetBuf[act].level;
-        std::vector<PoolBwFrame> fr(batch);
-        for (int i = 0; i < batch; ++i)
-            fr[i] = PoolBwFrame{static_cast<const float *>(ws.at(dp, i)), static_cast<const uint16_t *>(ws.at(act, i)),
-                                static_cast<const float *>(ws.at(skip, i)), static_cast<uint16_t *>(ws.at(dy, i)),
-                                ws.h[lv], ws.w[lv], kUnetBuf[skip].ch, soff};
-        return hip_check(launch_pool_bwd(C, batch, fr.data(), c.st), "unet max pool backward");
-    };
-
-    // head: dY10 = ReLU'(A10) * W dparam; partial set 12
-    {
-        std::vector<HeadFrame> hf(batch);
-        for (int i = 0; i < batch; ++i)
-        {
-            HeadFrame &f = hf[i];
-            memset(&f, 0, sizeof(f));
-            f.in = static_cast<const uint16_t *>(ws.at(UB_A10, i));
-            f.dparam = dparam + (size_t)i * c.n0() * 6;
-            f.din = static_cast<uint16_t *>(ws.at(UB_DY10, i));
-            f.w = master(c, 11);
-            f.b = f.w + 48;
-            f.part = ws.part[12] + (size_t)i * ws.nblk[12] * WG_PART;
-            f.H = height;
-            f.W = width;
-            f.nblk = ws.nblk[12];
-        }
-        if ((e = hip_check(launch_head_bwd(batch, hf.data(), c.st), "unet head backward"))) return e;
-        red(12, 1, 0, -1, 1, 0);
-    }
-    // c5_1 over concat[A9, A1]
-    if ((e = wgrad(UW_24_8, 11, UB_A9, UB_A1, UB_DY10))) return e;
-    red(11, 9, 0, -1, 1, 0);
-    if ((e = conv(c, UC_D_8_24, 10, UB_DY10, -1, UB_DC5, -1, true, "unet c5_1 dgrad"))) return e;
-    // c4_3: gradient of its output = dC5[:, 0:16]
-    if ((e = convT_bwd(9, 10, UB_A8, UB_DC5, UB_DY8))) return e;
-    red(10, 1, 0, -1, 1, 1);
-    // c4_2
-    if ((e = wgrad(UW_16_16, 9, UB_A7, -1, UB_DY8))) return e;
-    red(9, 9, 0, -1, 1, 0);
-    if ((e = conv(c, UC_D_16_16, 8, UB_DY8, -1, UB_DY7, UB_A7, true, "unet c4_2 dgrad"))) return e;
-    // c4_1 over concat[A6, A3]: two 32-channel input slices
-    if ((e = wgrad(UW_48_16_M0, 7, UB_A6, UB_A3, UB_DY7))) return e;
-    red(7, 9, 0, 32, 1, 0);
-    if ((e = wgrad(UW_48_16_M1, 8, UB_A6, UB_A3, UB_DY7))) return e;
-    red(8, 9, 32, 16, 0, 0);
-    if ((e = conv(c, UC_D_16_48, 7, UB_DY7, -1, UB_DC4, -1, true, "unet c4_1 dgrad"))) return e;
-    // c3_3: gradient of its output = dC4[:, 0:32]
-    if ((e = convT_bwd(6, 6, UB_A5, UB_DC4, UB_DY5))) return e;
-    red(6, 1, 0, -1, 1, 1);
-    // c3_2, c3_1
-    if ((e = wgrad(UW_32_32, 5, UB_A4, -1, UB_DY5))) return e;
-    red(5, 9, 0, -1, 1, 0);
-    if ((e = conv(c, UC_D_32_32, 5, UB_DY5, -1, UB_DY4, UB_A4, true, "unet c3_2 dgrad"))) return e;
-    if ((e = wgrad(UW_16_32, 4, UB_P2, -1, UB_DY4))) return e;
-    red(4, 9, 0, -1, 1, 0);
-    if ((e = conv(c, UC_D_32_16, 4, UB_DY4, -1, UB_DP2, -1, true, "unet c3_1 dgrad"))) return e;
-    // second pool: A3 receives the pooled gradient and the skip gradient dC4[:, 32:48]
-    if ((e = pool_bwd(16, UB_DP2, UB_A3, UB_DC4, 32, UB_DY3))) return e;
-    // c2_2, c2_1
-    if ((e = wgrad(UW_16_16, 3, UB_A2, -1, UB_DY3))) return e;
-    red(3, 9, 0, -1, 1, 0);
-    if ((e = conv(c, UC_D_16_16, 3, UB_DY3, -1, UB_DY2, UB_A2, true, "unet c2_2 dgrad"))) return e;
-    if ((e = wgrad(UW_8_16, 2, UB_P1, -1, UB_DY2))) return e;
-    red(2, 9, 0, -1, 1, 0);
-    if ((e = conv(c, UC_D_16_8, 2, UB_DY2, -1, UB_DP1, -1, true, "unet c2_1 dgrad"))) return e;
-    // first pool: A1 receives the pooled gradient and the skip gradient dC5[:, 16:24]
-    if ((e = pool_bwd(8, UB_DP1, UB_A1, UB_DC5, 16, UB_DY1))) return e;
-    // c1_2, c1_1
-    if ((e = wgrad(UW_8_8, 1, UB_A0, -1, UB_DY1))) return e;
-    red(1, 9, 0, -1, 1, 0);
-    if ((e = conv(c, UC_D_8_8, 1, UB_DY1, -1, UB_DY0, UB_A0, true, "unet c1_2 dgrad"))) return e;
-    if ((e = wgrad(UW_0, 0, -1, -1, UB_DY0))) return e;
-    red(0, 9, 0, -1, 1, 0);
-    return hip_check(launch_wgrad_reduce((int)jobs.size(), jobs.data(), c.st), "unet wgrad reduce");
+    const float *in[5] = {ols1, ols2, prev_out, albedo, normal};
+    const std::vector<UnetFrame> F = batch_frames(ws, batch, in, packed, nullptr, dparam);
+    if (int e = unet_backward_frames(F.data(), batch, S(stream))) return e;
+    // the B images' partials of each set are contiguous: one job sums them into the one gradient
+    std::vector<xdn::RedJob> jobs;
+    for (int j = 0; j < UNET_JOBS; ++j) jobs.push_back(unet_red_job(j, ws.part[j], ws.stage[j], grad, batch * ws.nblk[j]));
+    return hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), S(stream)), "unet wgrad reduce");
 }
 
 }  // extern "C"

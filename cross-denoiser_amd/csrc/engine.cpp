@@ -3,9 +3,10 @@
 // One call advances every given slot (stream) by one frame. Each stage of train_step
 // (scripts/main.py:250-313) is ONE launch over all slots -- the frame-batch tile maps of the
 // kernels handle different resolutions in the same launch -- so n streams cost as many launches
-// as one: cross-regression (3), reprojection, erosion + screening, CNN forward (5), fused filter +
-// loss + backward, loss sums, CNN backward (5 weight gradients, 4 data gradients, 2 reductions),
-// Adam, pack.
+// as one: cross-regression (3), reprojection, erosion + screening, network forward (cnn5: 5
+// convolutions; the U-Net: 14 launches), fused filter + loss + backward, loss sums, network
+// backward (cnn5: 5 weight gradients, 4 data gradients; the U-Net: 23 launches), one pair of
+// weight-gradient reductions, Adam, pack.
 // Every slot owns its history, network, Adam moments and scratch; nothing is shared.
 #include <math.h>
 
@@ -49,7 +50,9 @@ struct SlotScratch {
         dparam = (float *)take(n * 24); param = (float *)take(n * 24);
         ones = (float *)take(n * 4);
         loss_parts = (float *)take((size_t)xdn::filter_train_tiles(H, W) * 4);
-        cnn_ws_bytes = Cnn5Ws(nullptr, 1, H, W).bytes;
+        // either network's workspace (one slot can be admitted with either)
+        const size_t c5 = Cnn5Ws(nullptr, 1, H, W).bytes, un = UnetWs(nullptr, 1, H, W).bytes;
+        cnn_ws_bytes = c5 > un ? c5 : un;
         cnn_ws = take(cnn_ws_bytes);
         adam_alpha = (float *)take(4);
         bytes = o;
@@ -67,6 +70,7 @@ int check_slot(const xdn_slot &s)
         return fail(XDN_EINVAL, "slot size %dx%d too small", s.height, s.width);
     if (s.height % kSparsity || s.width % kSparsity)
         return fail(XDN_EINVAL, "Image height and width must be divisible by sparsity");
+    if (s.network != XDN_NET_CNN5 && s.network != XDN_NET_UNET) return fail(XDN_EINVAL, "unknown network %d", s.network);
     return XDN_OK;
 }
 
@@ -99,7 +103,8 @@ int xdn_slot_init(const xdn_slot *slot, void *stream)
     const size_t n = px(slot->height, slot->width);
     hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(256), 0, S(stream), sc.ones, n, 1.0f);
     if (int e = hip_check(hipGetLastError(), "slot init")) return e;
-    return xdn_cnn5_pack(slot->params, slot->packed, stream);
+    return slot->network == XDN_NET_UNET ? xdn_unet_pack(slot->params, slot->packed, stream)
+                                         : xdn_cnn5_pack(slot->params, slot->packed, stream);
 }
 
 int xdn_denoise_step(const xdn_slot *slots, int n_slots, float lr, void *stream)
@@ -112,18 +117,32 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
     if (n_slots < 0 || n_slots > XDN_MAX_FRAMES) return fail(XDN_EINVAL, "n_slots must be in [0, %d]", XDN_MAX_FRAMES);
     if (stages & ~XDN_STAGE_ALL) return fail(XDN_EINVAL, "unknown stage bits 0x%x", stages);
     for (int i = 0; i < n_slots; ++i)
+    {
         if (int e = check_slot(slots[i])) return e;
+        if (slots[i].network != slots[0].network) return fail(XDN_EINVAL, "one call takes slots of one network");
+    }
     if (n_slots == 0) return XDN_OK;
+    const bool unet = slots[0].network == XDN_NET_UNET;
     hipStream_t st = S(stream);
     const int n = n_slots;
     std::vector<SlotScratch> sc;
     std::vector<Cnn5Ws> cw;
+    std::vector<UnetWs> uw;
+    std::vector<UnetFrame> uf;
     sc.reserve(n);
-    cw.reserve(n);
+    (unet ? uw.reserve(n) : cw.reserve(n));
     for (int i = 0; i < n; ++i)
     {
-        sc.emplace_back(slots[i].scratch, slots[i].height, slots[i].width);
-        cw.emplace_back(sc[i].cnn_ws, 1, slots[i].height, slots[i].width);
+        const xdn_slot &s = slots[i];
+        sc.emplace_back(s.scratch, s.height, s.width);
+        if (unet)
+        {
+            uw.emplace_back(sc[i].cnn_ws, 1, s.height, s.width);
+            const float *img[5] = {s.ols1, s.ols2, sc[i].prev_out, s.albedo, s.normal};
+            uf.push_back(unet_frame(uw[i], 0, img, s.packed, sc[i].param, sc[i].dparam));
+        }
+        else
+            cw.emplace_back(sc[i].cnn_ws, 1, s.height, s.width);
     }
     // 1. cross-regression with the log1p prologue (variance launches hold 2 frames per slot)
     if (stages & XDN_STAGE_REGRESS)
@@ -180,6 +199,11 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
         if (int e = hip_check(xdn::launch_screening(n, fr.data(), 7, 1, true, st), "screening")) return e;
     }
     // 4. network forward on [ols1, ols2, prev_out, albedo, normal] (main.py:291)
+    if (unet)
+    {
+        if (int e = unet_forward_frames(uf.data(), n, st)) return e;
+    }
+    else
     {
         std::vector<xdn::ConvFrame> fr(n);
         for (int l = 0; l < 5; ++l)
@@ -231,6 +255,16 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
     }
     // 6. network backward: per layer one weight-gradient launch and one data-gradient launch; the
     //    per-block weight-gradient partials of every (slot, layer) are reduced by one launch pair
+    if (unet)
+    {
+        if (int e = unet_backward_frames(uf.data(), n, st)) return e;
+        std::vector<xdn::RedJob> jobs;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < UNET_JOBS; ++j)
+                jobs.push_back(unet_red_job(j, uw[i].part[j], uw[i].stage[j], slots[i].grad, uw[i].nblk[j]));
+        if (int e = hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), st), "unet wgrad reduce")) return e;
+    }
+    else
     {
         std::vector<xdn::WgFrame> wf(n);
         std::vector<xdn::ConvFrame> cf(n);
@@ -279,7 +313,8 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
     // 7. Keras Adam on every slot's parameters, then rebuild the bf16 fragments
     {
         std::vector<xdn::AdamJob> jobs(n);
-        std::vector<xdn::PackJob> pj(5 * n);
+        std::vector<xdn::PackJob> pj(unet ? 0 : 5 * n);
+        std::vector<xdn::UPackJob> upj;
         std::vector<int *> counters;
         std::vector<float *> alphas;
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-7f;
@@ -288,7 +323,7 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
             const xdn_slot &s = slots[i];
             const int t = s.adam_step < 1 ? 1 : s.adam_step;
             jobs[i].param = s.params; jobs[i].grad = s.grad; jobs[i].m = s.adam_m; jobs[i].v = s.adam_v;
-            jobs[i].count = (int)kLay.count;
+            jobs[i].count = (int)(unet ? kULay.count : kLay.count);
             jobs[i].alpha = (float)((double)lr * sqrt(1.0 - pow((double)b2, t)) / (1.0 - pow((double)b1, t)));
             jobs[i].alpha_dev = s.adam_t ? sc[i].adam_alpha : nullptr;
             if (s.adam_t)
@@ -297,6 +332,11 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
                 alphas.push_back(sc[i].adam_alpha);
             }
             char *pk = static_cast<char *>(s.packed);
+            if (unet)
+            {
+                unet_pack_jobs(s.params, s.packed, upj);
+                continue;
+            }
             for (int l = 0; l < 5; ++l)
             {
                 xdn::PackJob &j = pj[5 * i + l];
@@ -312,7 +352,12 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
                                   "adam step counter"))
                 return e;
         if (int e = hip_check(xdn::launch_adam(n, jobs.data(), b1, b2, eps, st), "adam")) return e;
-        if (int e = hip_check(xdn::launch_pack(5 * n, pj.data(), st), "pack")) return e;
+        if (unet)
+        {
+            if (int e = hip_check(xdn::launch_upack((int)upj.size(), upj.data(), st), "unet pack")) return e;
+        }
+        else if (int e = hip_check(xdn::launch_pack(5 * n, pj.data(), st), "pack"))
+            return e;
     }
     return XDN_OK;
 }

@@ -977,20 +977,33 @@ hipError_t launch_head_bwd(int n, const HeadFrame *fr, hipStream_t st)
 // Weight packing: fwd frag[t][kc][nt][lane] = W[t][kc*16 + 8h + j][nt*32 + lane%32],
 // dgrad frag[t][kc][nt][lane] = W[8-t][nt*32 + lane%32][kc*16 + 8h + j]; zero outside ci x co
 // ---------------------------------------------------------------------------------------------
+constexpr int UPACK_MAX = 24;  // jobs per launch
 struct UPackBatch {
     int n;
-    UPackJob j[12 * 2];
+    UPackJob j[UPACK_MAX];
 };
 static_assert(sizeof(UPackBatch) <= 4096, "kernel argument too large");
+
+__host__ __device__ inline int upack_items(const UPackJob &J)
+{
+    if (J.copy_n > 0) return (J.copy_n + 63) / 64;
+    return 9 * ((J.ci + 15) / 16) * ((J.co + 31) / 32) + (J.frag_dgrad ? 9 * ((J.co + 15) / 16) * ((J.ci + 31) / 32) : 0);
+}
 
 __global__ __launch_bounds__(64) void upack(UPackBatch b)
 {
     const UPackJob &J = b.j[blockIdx.y];
     const int lane = threadIdx.x, h = lane >> 5, n = lane & 31;
+    const int item = blockIdx.x;
+    if (J.copy_n > 0)
+    {
+        const int e = item * 64 + lane;
+        if (e < J.copy_n) static_cast<float *>(J.frag_fwd)[e] = J.w[e];
+        return;
+    }
     const int kcf = (J.ci + 15) / 16, ntf = (J.co + 31) / 32;
     const int kcd = (J.co + 15) / 16, ntd = (J.ci + 31) / 32;
     const int nf = 9 * kcf * ntf, nd = J.frag_dgrad ? 9 * kcd * ntd : 0;
-    const int item = blockIdx.x;
     if (item < nf)
     {
         const int t = item / (kcf * ntf), r = item - t * kcf * ntf, kc = r / ntf, nt = r - kc * ntf;
@@ -1027,21 +1040,23 @@ __global__ __launch_bounds__(64) void upack(UPackBatch b)
 
 hipError_t launch_upack(int n, const UPackJob *jobs, hipStream_t st)
 {
-    if (n <= 0) return hipSuccess;
-    if (n > 24) return hipErrorInvalidValue;
-    UPackBatch b{};
-    b.n = n;
-    int maxi = 0;
-    for (int i = 0; i < n; ++i)
+    for (int c0 = 0; c0 < n; c0 += UPACK_MAX)
     {
-        b.j[i] = jobs[i];
-        const UPackJob &J = jobs[i];
-        const int items = 9 * ((J.ci + 15) / 16) * ((J.co + 31) / 32) +
-                          (J.frag_dgrad ? 9 * ((J.co + 15) / 16) * ((J.ci + 31) / 32) : 0);
-        maxi = items > maxi ? items : maxi;
+        const int m = n - c0 < UPACK_MAX ? n - c0 : UPACK_MAX;
+        UPackBatch b{};
+        b.n = m;
+        int maxi = 0;
+        for (int i = 0; i < m; ++i)
+        {
+            b.j[i] = jobs[c0 + i];
+            const int items = upack_items(b.j[i]);
+            maxi = items > maxi ? items : maxi;
+        }
+        if (maxi == 0) continue;
+        hipLaunchKernelGGL(upack, dim3(maxi, m), dim3(64), 0, st, b);
+        if (hipError_t e = hipGetLastError()) return e;
     }
-    hipLaunchKernelGGL(upack, dim3(maxi, n), dim3(64), 0, st, b);
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 }  // namespace xdn

@@ -135,12 +135,13 @@ hipError_t launch_head(int n, const HeadFrame *fr, hipStream_t st);
 hipError_t launch_head_bwd(int n, const HeadFrame *fr, hipStream_t st);
 
 // Weight packing of one 3x3 layer: forward fragments [9][KC][NT][64] (K = ci, N = co) and, when
-// frag_dgrad is set, data-gradient fragments (K = co, N = ci, taps flipped).
+// frag_dgrad is set, data-gradient fragments (K = co, N = ci, taps flipped). A job with copy_n > 0
+// instead copies copy_n floats from w to frag_fwd (the FP32 1x1 layers).
 struct UPackJob {
     const float *w, *b;         // FP32 master [3][3][ci][co], [co]
     void *frag_fwd, *frag_dgrad;
     float *bias32;              // [NT * 32]
-    int ci, co;
+    int ci, co, copy_n;
 };
 hipError_t launch_upack(int n, const UPackJob *jobs, hipStream_t st);
 

@@ -7,9 +7,10 @@ of slots (up to 16, mixed resolutions) by one frame with one launch per stage:
   1. cross-regression with the log1p prologue   -> ols1, ols2
   2. reprojection of [out_log, ols1, ols2]      (opacities one, main.py:266)
   3. 3x3 erosion + temporal screening           (win_curr 7, win_prev 1)
-  4. bandwidth CNN forward on [ols1, ols2, prev_out, albedo, normal]
+  4. bandwidth network forward on [ols1, ols2, prev_out, albedo, normal]: cnn5 (the benchmark's
+     5-layer CNN) or the reference's U-Net (network="unet")
   5. fused spatial filter + temporal accumulation + loss + backward to the network output
-  6. CNN backward, Keras Adam, bf16 fragment repack
+  6. network backward, Keras Adam, bf16 fragment repack
   7. history handoff (main.py:338-341): out_log/ols1/ols2 ping-pong, the frame's linearZ/normal
 
 Labels follow main.py:283-286; frame-0 history is zeros (main.py:323-330).
@@ -40,7 +41,10 @@ class XdnSlot(ctypes.Structure):
                     "color", "color2", "albedo", "normal", "mvec", "linear_z", "pn_fwidth", "opacity",
                     "prev_out_log", "prev_ols1", "prev_ols2", "prev_linear_z", "prev_normal",
                     "out_log", "ols1", "ols2", "out", "scratch", "params", "grad", "adam_m", "adam_v", "packed")] +
-                [("adam_step", ctypes.c_int), ("avg_loss", ctypes.c_void_p), ("adam_t", ctypes.c_void_p)])
+                [("adam_step", ctypes.c_int), ("avg_loss", ctypes.c_void_p), ("adam_t", ctypes.c_void_p),
+                 ("network", ctypes.c_int)])
+
+NETWORKS = {"cnn5": 0, "unet": 1}  # XDN_NET_CNN5, XDN_NET_UNET
 
 
 def filter_train(ols1, ols2, albedo, normal, param, success, success_prev, prev_out, opacity, prev_ols1, prev_ols2,
@@ -65,18 +69,22 @@ def filter_train(ols1, ols2, albedo, normal, param, success, success_prev, prev_
     return out_log, out, loss, avg, dparam
 
 
-def slot_bytes(H: int, W: int) -> int:
+def slot_bytes(H: int, W: int, network: str = "cnn5") -> int:
     """HBM resident per stream slot: scratch, history, outputs and network state."""
     hist = 7 * H * W * 3 * 4  # 2 x (out_log, ols1, ols2) + out
-    net = 4 * int(lib.xdn_cnn5_param_count()) * 4 + int(lib.xdn_cnn5_packed_bytes()) + 4
+    fn = lambda what: getattr(lib, f"xdn_{network}_{what}")
+    net = 4 * int(fn("param_count")()) * 4 + int(fn("packed_bytes")()) + 4
     return int(lib.xdn_slot_scratch_bytes(H, W)) + hist + net
 
 
 class Denoiser:
-    """One render stream: resident history, bandwidth network (cnn5) and its Adam state."""
+    """One render stream: resident history, bandwidth network (cnn5 or unet) and its Adam state."""
 
     def __init__(self, H: int, W: int, device="cuda", seed: int = 1234, lr: float = LEARNING_RATE,
-                 use_graphs: bool = False):
+                 use_graphs: bool = False, network: str = "cnn5"):
+        if network not in NETWORKS:
+            raise ValueError(f"unknown network {network!r} (one of {sorted(NETWORKS)})")
+        self.network = network
         if H % SPARSITY or W % SPARSITY or H <= REGRESSION_WIDTH // 2 or W <= REGRESSION_WIDTH // 2:
             raise ValueError(f"stream size {H}x{W}: height and width must be divisible by {SPARSITY} and > 8")
         self.device = torch.device(device)
@@ -87,12 +95,12 @@ class Denoiser:
         self.scratch = torch.empty(int(lib.xdn_slot_scratch_bytes(H, W)), device=dev, dtype=torch.uint8)
         self.hist = [[e3(), e3(), e3()], [e3(), e3(), e3()]]  # [out_log, ols1, ols2] ping-pong
         self.out = e3()
-        flat = cnn_mod.flatten_params(cnn_mod.init_params(seed=seed))
-        self.params = flat.to(dev)
+        layers = cnn_mod.unet_init_params(seed=seed) if network == "unet" else cnn_mod.init_params(seed=seed)
+        self.params = cnn_mod.flatten_params(layers).to(dev)
         self.grad = torch.zeros_like(self.params)
         self.adam_m = torch.zeros_like(self.params)
         self.adam_v = torch.zeros_like(self.params)
-        self.packed = torch.empty(int(lib.xdn_cnn5_packed_bytes()), device=dev, dtype=torch.uint8)
+        self.packed = torch.empty(int(getattr(lib, f"xdn_{network}_packed_bytes")()), device=dev, dtype=torch.uint8)
         self.avg_loss = torch.zeros(1, device=dev, dtype=torch.float32)
         self.adam_t = torch.zeros(1, device=dev, dtype=torch.int32)  # Adam updates done (device counter)
         self.use_graphs = use_graphs
@@ -150,11 +158,13 @@ class Denoiser:
         s.adam_step = t + 1
         s.avg_loss = self.avg_loss.data_ptr()
         s.adam_t = self.adam_t.data_ptr()
+        s.network = NETWORKS[self.network]
         return s
 
     def repack(self):
         """Rebuild the packed bf16 fragments after the parameters were replaced."""
-        check(lib.xdn_cnn5_pack(ptr(self.params), ptr(self.packed), self._stream()), "cnn5 pack")
+        check(getattr(lib, f"xdn_{self.network}_pack")(ptr(self.params), ptr(self.packed), self._stream()),
+              f"{self.network} pack")
 
     def set_step(self, t: int):
         """Resume at update t (after the parameters and Adam moments were restored)."""

@@ -130,11 +130,12 @@ class Admission:
 # ------------------------------------------------------------------------------------------------
 class Server:
     def __init__(self, device="cuda", hbm_budget=64 << 30, max_batch=16, lr=1e-3, pool_frames=4, batching=True,
-                 idle_after=0.05, graphs=False, max_inflight=2):
+                 idle_after=0.05, graphs=False, max_inflight=2, network="cnn5"):
         """batching: same-stage kernels of all ready streams in one launch (xdn_denoise_step over
         many slots); otherwise one frame at a time. graphs (with batching off): each frame is the
         stream's captured step graph (one per frame buffers and history parity). max_inflight:
-        batches on the device at once (each on its own HIP stream)."""
+        batches on the device at once (each on its own HIP stream). network: every stream's
+        bandwidth network ("cnn5" or "unet")."""
         import torch
         self.torch = torch
         self.device = torch.device(device)
@@ -142,6 +143,7 @@ class Server:
         self.max_batch = max_batch if batching else 1
         self.graphs = graphs and not batching
         self.lr = lr
+        self.network = network
         self.pool_frames = pool_frames
         self.idle_after = idle_after
         self.max_inflight = max(1, int(max_inflight))
@@ -160,7 +162,7 @@ class Server:
     def run(self, reqs: list, log=None) -> dict:
         from . import pipeline
         torch = self.torch
-        rejected = [r for r in reqs if pipeline.slot_bytes(r.height, r.width) > self.adm.budget]
+        rejected = [r for r in reqs if pipeline.slot_bytes(r.height, r.width, self.network) > self.adm.budget]
         reqs = [r for r in reqs if r not in rejected]
         for r in reqs:
             self.pools[r.sid] = self._frame_pool(r)
@@ -205,7 +207,7 @@ class Server:
                     continue
                 if sid not in self.denoisers:
                     r = by_sid[sid]
-                    ok, ev_s = self.adm.try_admit(sid, pipeline.slot_bytes(r.height, r.width), now, busy,
+                    ok, ev_s = self.adm.try_admit(sid, pipeline.slot_bytes(r.height, r.width, self.network), now, busy,
                                                   self.idle_after)
                     for s in ev_s:
                         self._evict(s)
@@ -258,7 +260,7 @@ class Server:
         history restarts from zero, as at frame 0)."""
         from . import pipeline
         d = pipeline.Denoiser(r.height, r.width, device=self.device, seed=1234 + r.sid, lr=self.lr,
-                              use_graphs=self.graphs)
+                              use_graphs=self.graphs, network=self.network)
         saved = self.saved.pop(r.sid, None)
         if saved is not None:
             for t, src in zip((d.params, d.adam_m, d.adam_v), saved[:3]):
@@ -277,7 +279,7 @@ class Server:
 # ------------------------------------------------------------------------------------------------
 # Command line: python -m xdn.serving --config 3 --mode all
 # ------------------------------------------------------------------------------------------------
-def warm_up(device, sizes, lr=1e-3):
+def warm_up(device, sizes, lr=1e-3, network="cnn5"):
     """First launches of every kernel at every resolution (module load, code object upload)."""
     import torch
 
@@ -285,7 +287,7 @@ def warm_up(device, sizes, lr=1e-3):
     for (h, w) in sorted(set(sizes)):
         spec = synth.StreamSpec(h, w, seed=99)
         fr = [synth.make_frame(spec, t, device=device) for t in range(2)]
-        d = pipeline.Denoiser(h, w, device=device, lr=lr)
+        d = pipeline.Denoiser(h, w, device=device, lr=lr, network=network)
         d.step(fr[0])
         d.step(fr[1])
         pipeline.denoise_step_multi([d], [fr[0]])
@@ -308,19 +310,20 @@ def main(argv=None):
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--fps", type=float, default=60.0, help="mean arrival rate per stream (configuration 3)")
+    ap.add_argument("--network", choices=("cnn5", "unet"), default="cnn5")
     a = ap.parse_args(argv)
     dev = torch.device("cuda:0")
     reqs = config3(a.frames, fps=a.fps, seed=a.seed) if a.config == 3 else config4(a.frames, seed=a.seed)
-    warm_up(dev, [(r.height, r.width) for r in reqs])
+    warm_up(dev, [(r.height, r.width) for r in reqs], network=a.network)
     modes = ("plain", "graphs", "batched") if a.mode == "all" else (a.mode,)
     for m in modes:
         print(f"[serving] config {a.config}, mode {m}, {len(reqs)} streams x {a.frames} frames", file=sys.stderr,
               flush=True)
         # plain: the one-frame-at-a-time loop (one frame on the device at a time, eager launches)
         srv = Server(dev, hbm_budget=int(a.budget_gb * (1 << 30)), batching=(m == "batched"), graphs=(m == "graphs"),
-                     max_inflight=1 if m == "plain" else 2)
+                     max_inflight=1 if m == "plain" else 2, network=a.network)
         res = srv.run(reqs)
-        res.update({"config": a.config, "mode": m, "streams": len(reqs), "frames_per_stream": a.frames,
+        res.update({"config": a.config, "mode": m, "network": a.network, "streams": len(reqs), "frames_per_stream": a.frames,
                     "budget_gb": a.budget_gb, "arrival_fps_per_stream": a.fps if a.config == 3 else 120.0,
                     "stream_sizes": sorted({f"{r.width}x{r.height}" for r in reqs})})
         print(json.dumps(res), flush=True)

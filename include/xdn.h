@@ -217,8 +217,9 @@ int xdn_unet_backward(const float *ols1, const float *ols2, const float *prev_ou
  * Batched online denoising step: train_step (scripts/main.py:250-313) for many render streams.
  * One call advances every slot by one frame, each stage being ONE launch over all slots (their
  * resolutions may differ): cross-regression with the log1p prologue, reprojection of the history
- * (opacities one, main.py:266), erosion + temporal screening (win 7/1), cnn5 forward, fused
- * filter + temporal accumulation + loss + backward, cnn5 backward, Keras Adam, fragment repack.
+ * (opacities one, main.py:266), erosion + temporal screening (win 7/1), network forward (cnn5 or
+ * the reference's U-Net, per slot.network; one call takes slots of one network), fused filter +
+ * temporal accumulation + loss + backward, network backward, Keras Adam, fragment repack.
  * Labels and history handoff follow main.py:283-286 and :338-341: the caller passes this frame's
  * out_log/ols1/ols2 buffers as the next frame's prev_* (ping-pong) together with this frame's
  * linear_z/normal. Frame 0 of a stream uses zero history (main.py:323-330).
@@ -234,7 +235,8 @@ typedef struct xdn_slot {
     float *out_log, *ols1, *ols2, *out;
     /* slot-owned scratch of xdn_slot_scratch_bytes(H, W) bytes, prepared once by xdn_slot_init */
     void *scratch;
-    /* network state: params, grad, Adam m and v [xdn_cnn5_param_count()], packed fragments */
+    /* network state: params, grad, Adam m and v [param count of the slot's network], packed
+     * fragments [packed bytes of the network] */
     float *params, *grad, *adam_m, *adam_v;
     void *packed;
     int adam_step; /* step number of this update, from 1 (ignored when adam_t is set) */
@@ -243,14 +245,18 @@ typedef struct xdn_slot {
      * increments it and forms the Adam bias correction on the device, so a captured step graph
      * stays valid on replay; NULL uses adam_step from the host. */
     int *adam_t;
+    int network; /* XDN_NET_CNN5 (xdn_cnn5_*) or XDN_NET_UNET (xdn_unet_*) */
 } xdn_slot;
+#define XDN_NET_CNN5 0
+#define XDN_NET_UNET 1
 
 size_t xdn_slot_scratch_bytes(int height, int width);
 /* Byte offsets inside the scratch of its XDN_SLOT_NBUF intermediate buffers, in this order:
  * y1, y2 [H,W,3]; var1, var2 [H,W,1]; regression coefficients; reprojection success [H,W,1] u8;
  * reprojected out_log, ols1, ols2 [H,W,3]; screened prev_out, prev_ols1, prev_ols2 [H,W,3];
  * success_prev, eroded success [H,W,1]; dparam, network output [H,W,6]; ones [H,W,1];
- * loss partials; cnn5 workspace; Adam step size [1]. For inspection and tests. */
+ * loss partials; network workspace (sized for either network); Adam step size [1]. For
+ * inspection and tests. */
 #define XDN_SLOT_NBUF 20
 int xdn_slot_scratch_offsets(int height, int width, size_t *offsets);
 /* Fills the slot's constant scratch and packs its network; call once when the slot is admitted. */

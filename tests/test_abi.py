@@ -66,6 +66,7 @@ def test_make_slot_parity_selects_history_buffers():
             self.params = self.grad = self.adam_m = self.adam_v = torch.zeros(4)
             self.packed, self.avg_loss = torch.zeros(4, dtype=torch.uint8), torch.zeros(1)
             self.adam_t, self.prev_frame = torch.zeros(1, dtype=torch.int32), None
+            self.network = "unet"
 
     d = Fake()
     frame = {k: torch.zeros(1, 8, 12, 3 if k != "opacity" else 1) for k in
@@ -74,3 +75,22 @@ def test_make_slot_parity_selects_history_buffers():
     assert s3.ols1 == d.hist[1][1].data_ptr() and s3.prev_ols1 == d.hist[0][1].data_ptr()
     assert s4.ols1 == d.hist[0][1].data_ptr() and s4.prev_ols1 == d.hist[1][1].data_ptr()
     assert s3.adam_step == 4 and s4.adam_step == 5 and s3.color == frame["color"].data_ptr()
+    assert s3.network == pipeline.NETWORKS["unet"] == 1
+
+
+def test_slot_network_checks_without_gpu():
+    """Slot argument checks (network kind, one network per call) run before any launch."""
+    from xdn import pipeline
+    from xdn._lib import lib
+    a, b = pipeline.XdnSlot(), pipeline.XdnSlot()
+    for s in (a, b):
+        s.height, s.width = 64, 64
+    a.network = 5
+    assert lib.xdn_denoise_step(ctypes.addressof(a), 1, 1e-3, None) == -1
+    assert "unknown network" in lib.xdn_last_error().decode()
+    pair = (pipeline.XdnSlot * 2)()
+    for i, net in enumerate((0, 1)):
+        pair[i].height, pair[i].width, pair[i].network = 64, 64, net
+    assert lib.xdn_denoise_step(ctypes.addressof(pair), 2, 1e-3, None) == -1
+    assert "one network" in lib.xdn_last_error().decode()
+    assert pipeline.slot_bytes(1080, 1920, "unet") > 0 and pipeline.slot_bytes(1080, 1920, "cnn5") > 0

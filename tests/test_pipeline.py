@@ -83,7 +83,8 @@ def test_denoiser_trains(dev):
     assert np.mean(losses[-5:]) < 0.98 * np.mean(losses[:5])
 
 
-def test_batched_step_equals_single_streams(dev):
+@pytest.mark.parametrize("network", ["cnn5", "unet"])
+def test_batched_step_equals_single_streams(dev, network):
     """Streams of different resolutions advanced in one batched step give the same results as
     the same streams advanced one at a time (each result is the same alone and in the mix)."""
     from xdn import pipeline, synth
@@ -91,8 +92,8 @@ def test_batched_step_equals_single_streams(dev):
     specs = [synth.StreamSpec(h, w, seed=21, stream=i, pan=(0.75 * (i + 1), -0.5), disocclusion=0.1)
              for i, (h, w) in enumerate(sizes)]
     frames = [[{k: v.to(dev) for k, v in synth.make_frame(sp, t).items()} for t in range(3)] for sp in specs]
-    alone = [pipeline.Denoiser(h, w, device=dev, seed=100 + i) for i, (h, w) in enumerate(sizes)]
-    mixed = [pipeline.Denoiser(h, w, device=dev, seed=100 + i) for i, (h, w) in enumerate(sizes)]
+    alone = [pipeline.Denoiser(h, w, device=dev, seed=100 + i, network=network) for i, (h, w) in enumerate(sizes)]
+    mixed = [pipeline.Denoiser(h, w, device=dev, seed=100 + i, network=network) for i, (h, w) in enumerate(sizes)]
     for t in range(3):
         for i, d in enumerate(alone):
             d.step(frames[i][t])
@@ -104,15 +105,16 @@ def test_batched_step_equals_single_streams(dev):
         assert torch.equal(a.params, m.params)
 
 
-def test_step_graph_replays_match_eager(dev):
+@pytest.mark.parametrize("network", ["cnn5", "unet"])
+def test_step_graph_replays_match_eager(dev, network):
     """Captured per-frame step graphs (one per frame buffers and history parity) give bit-identical
     results to eager steps, replay after replay, with eager work and synchronizations in between."""
     from xdn import pipeline, synth
     H, W = 48, 64
     spec = synth.StreamSpec(H, W, seed=31, pan=(1.5, -0.5), disocclusion=0.1)
     pool = [{k: v.to(dev) for k, v in synth.make_frame(spec, t).items()} for t in range(2)]
-    eager = pipeline.Denoiser(H, W, device=dev, seed=7)
-    graph = pipeline.Denoiser(H, W, device=dev, seed=7, use_graphs=True)
+    eager = pipeline.Denoiser(H, W, device=dev, seed=7, network=network)
+    graph = pipeline.Denoiser(H, W, device=dev, seed=7, use_graphs=True, network=network)
     for t in range(7):
         f = pool[t % 2]
         a = eager.step(f)
@@ -155,13 +157,14 @@ def test_pipelined_frames_equal_sequential_steps(dev):
     assert pip.t == seq.t == 5
 
 
-def test_online_denoiser_reduces_error(dev):
+@pytest.mark.parametrize("network", ["cnn5", "unet"])
+def test_online_denoiser_reduces_error(dev, network):
     """End to end on a synthetic sequence with a known clean reference: after a few frames the
     denoised output's RelL2 (scripts/main.py:191-196) is far below that of the noisy input."""
     from xdn import pipeline, synth
     H, W = 96, 128
     spec = synth.StreamSpec(H, W, seed=3, pan=(0.5, -0.25))
-    den = pipeline.Denoiser(H, W, device=dev)
+    den = pipeline.Denoiser(H, W, device=dev, network=network)
     errs = []
     for t in range(12):
         f = {k: v.to(dev) for k, v in synth.make_frame(spec, t).items()}
@@ -172,3 +175,33 @@ def test_online_denoiser_reduces_error(dev):
     print("  relL2 out/input by frame: " + " ".join(f"{a:.3f}/{b:.3f}" for a, b in errs), flush=True)
     assert all(np.isfinite(a) for a, _ in errs)
     assert errs[-1][0] < 0.25 * errs[-1][1]
+
+
+def test_unet_step_uses_the_unet_ops(dev):
+    """In a step with network="unet" the network output, its gradient and the Adam update are those
+    of the standalone U-Net entry points (xdn_unet_forward / _backward, xdn_adam) on the step's own
+    network inputs."""
+    from xdn import cnn, pipeline, synth
+    H, W = 48, 64
+    spec = synth.StreamSpec(H, W, seed=13, pan=(1.0, 0.5), disocclusion=0.1)
+    frames = [{k: v.to(dev) for k, v in synth.make_frame(spec, t).items()} for t in range(2)]
+    den = pipeline.Denoiser(H, W, device=dev, seed=21, network="unet")
+    net = cnn.HipUNet(dev, seed=21, shape=(1, H, W))
+    den.step(frames[0])
+    net.forward(*[t.clone() for t in (den.outputs()["ols1"], den.outputs()["ols2"])],
+                den.buffer("prev_out", 3).clone(), frames[0]["albedo"], frames[0]["normal"])
+    torch.cuda.synchronize()
+    assert torch.equal(net.out, den.buffer("param", 6))
+    net.backward(den.buffer("dparam", 6).clone())
+    net.adam_step(pipeline.LEARNING_RATE)
+    torch.cuda.synchronize()
+    assert torch.equal(net.grad, den.grad)
+    assert torch.allclose(net.params, den.params, rtol=1e-6, atol=1e-9)
+    # second frame: history present, same network state on both sides
+    net.params.copy_(den.params)
+    net.pack()
+    den.step(frames[1])
+    net.forward(den.outputs()["ols1"], den.outputs()["ols2"], den.buffer("prev_out", 3), frames[1]["albedo"],
+                frames[1]["normal"])
+    torch.cuda.synchronize()
+    assert torch.equal(net.out, den.buffer("param", 6))
