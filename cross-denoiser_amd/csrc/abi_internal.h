@@ -160,8 +160,8 @@ struct UnetBuf {
 constexpr UnetBuf kUnetBuf[UNET_BUFS] = {
     {0, 8, 0},  {0, 8, 0},  {1, 8, 0},  {1, 16, 0}, {1, 16, 0}, {2, 16, 0}, {2, 32, 0},
     {2, 32, 0}, {1, 32, 0}, {1, 16, 0}, {1, 16, 0}, {0, 16, 0}, {0, 8, 0},              // A0 .. A10
-    {0, 8, 0},  {0, 24, 1}, {1, 16, 0}, {1, 16, 0}, {1, 48, 1}, {2, 32, 0}, {2, 32, 0},
-    {2, 16, 1}, {1, 16, 0}, {1, 16, 0}, {1, 8, 1},  {0, 8, 0},  {0, 8, 0}};             // gradients
+    {0, 8, 0},  {0, 24, 0}, {1, 16, 0}, {1, 16, 0}, {1, 48, 0}, {2, 32, 0}, {2, 32, 0},
+    {2, 16, 0}, {1, 16, 0}, {1, 16, 0}, {1, 8, 0},  {0, 8, 0},  {0, 8, 0}};             // gradients
 // Weight-gradient partial sets: one per 3x3 layer (two for c4_1: input channels 0-31 and 32-47),
 // the two transposed layers and the head. level = resolution the set's blocks walk.
 constexpr int UNET_JOBS = 13;
@@ -171,13 +171,25 @@ struct UnetJob {
 constexpr UnetJob kUnetJob[UNET_JOBS] = {{0, 0}, {1, 0}, {2, 1}, {3, 1}, {4, 2},  {5, 2}, {6, 2},
                                          {7, 1}, {7, 1}, {8, 1}, {9, 1}, {10, 0}, {11, 0}};
 
+// input channels of partial set j's slice (c4_1 is split 32 + 16)
+inline int unet_job_slice(int job)
+{
+    return job == 7 ? 32 : (job == 8 ? 16 : kUnet[kUnetJob[job].layer].ci);
+}
+// floats of one block's compact partial of set j
+inline int unet_job_pstride(int job)
+{
+    const UnetLayer &L = kUnet[kUnetJob[job].layer];
+    return xdn::compact_part(L.type == UL_CONV ? 9 : 1, unet_job_slice(job), L.co);
+}
+
 inline int unet_job_blocks(int job, int h, int w)
 {
     const int t = kUnet[kUnetJob[job].layer].type;
     const long np = (long)h * w;
     if (t == UL_CONV) return xdn::ublocks(h, w);
-    const long per = t == UL_CONVT ? 2048 : 4096;  // pixels per block
-    const long cap = t == UL_CONVT ? 256 : 512;
+    const long per = t == UL_CONVT ? 512 : 4096;  // pixels per block
+    const long cap = 512;
     long nb = (np + per - 1) / per;
     nb = nb < 1 ? 1 : (nb > cap ? cap : nb);
     return (int)nb;
@@ -208,9 +220,9 @@ struct UnetWs {
             const int lv = kUnetJob[j].level;
             nblk[j] = unet_job_blocks(j, h[lv], w[lv]);
             part[j] = reinterpret_cast<float *>(p + o);
-            o += al256((size_t)B * nblk[j] * xdn::WG_PART * 4);
+            o += al256((size_t)B * nblk[j] * unet_job_pstride(j) * 4);
             stage[j] = reinterpret_cast<float *>(p + o);
-            o += al256((size_t)xdn::WG_SPLITS * xdn::WG_PART * 4);
+            o += al256((size_t)xdn::WG_SPLITS * unet_job_pstride(j) * 4);
         }
         bytes = o;
     }

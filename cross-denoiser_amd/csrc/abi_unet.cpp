@@ -17,7 +17,7 @@ using Frames = const UnetFrame *;
 
 // 3x3 convolution of layer l over every image: forward from concat[a, b] (the images when a < 0)
 // into buffer out, or (dgrad) data gradient from buffer a into out, masked by buffer mask's ReLU
-// (FP32 output when mask < 0).
+// (unmasked when mask < 0).
 int conv(Frames F, int n, hipStream_t st, int kind, int l, int a, int b, int out, int mask, bool dgrad,
          const char *what)
 {
@@ -98,8 +98,8 @@ int wgrad(Frames F, int n, hipStream_t st, int kind, int j, int a, int b, int dy
     return hip_check(xdn::launch_uwgrad(kind, n, fr.data(), st), "unet wgrad");
 }
 
-// transposed layer l backward (partial set j): input buffer in, FP32 gradient of its output in
-// buffer dout (its first C channels); the gradient into its input -> din, masked by in's ReLU
+// transposed layer l backward (partial set j): input buffer in, gradient of its output in buffer
+// dout (its first C channels); the gradient into its input -> din, masked by in's ReLU
 int convT_bwd(Frames F, int n, hipStream_t st, int l, int j, int in, int dout, int din)
 {
     const int lv = kUnet[l].level;
@@ -110,7 +110,7 @@ int convT_bwd(Frames F, int n, hipStream_t st, int l, int j, int in, int dout, i
         xdn::CtFrame &f = fr[i];
         memset(&f, 0, sizeof(f));
         f.in = reinterpret_cast<const uint16_t *>(U.buf[in]);
-        f.dout = reinterpret_cast<const float *>(U.buf[dout]);
+        f.dout = reinterpret_cast<const uint16_t *>(U.buf[dout]);
         f.din = reinterpret_cast<uint16_t *>(U.buf[din]);
         f.w = master(U, l);
         f.b = f.w + kUnet[l].ci * kUnet[l].co;
@@ -132,9 +132,10 @@ int pool_bwd(Frames F, int n, hipStream_t st, int C, int dp, int act, int skip, 
     for (int i = 0; i < n; ++i)
     {
         const UnetFrame &U = F[i];
-        fr[i] = xdn::PoolBwFrame{reinterpret_cast<const float *>(U.buf[dp]), reinterpret_cast<const uint16_t *>(U.buf[act]),
-                                 reinterpret_cast<const float *>(U.buf[skip]), reinterpret_cast<uint16_t *>(U.buf[dy]),
-                                 U.h[lv], U.w[lv], kUnetBuf[skip].ch, soff};
+        fr[i] = xdn::PoolBwFrame{reinterpret_cast<const uint16_t *>(U.buf[dp]),
+                                 reinterpret_cast<const uint16_t *>(U.buf[act]),
+                                 reinterpret_cast<const uint16_t *>(U.buf[skip]),
+                                 reinterpret_cast<uint16_t *>(U.buf[dy]), U.h[lv], U.w[lv], kUnetBuf[skip].ch, soff};
     }
     return hip_check(xdn::launch_pool_bwd(C, n, fr.data(), st), "unet max pool backward");
 }
@@ -173,7 +174,7 @@ UnetFrame unet_frame(const UnetWs &ws, int b, const float *const img[5], const v
     for (int i = 0; i < UNET_BUFS; ++i) U.buf[i] = static_cast<char *>(ws.at(i, b));
     for (int j = 0; j < UNET_JOBS; ++j)
     {
-        U.part[j] = ws.part[j] + (size_t)b * ws.nblk[j] * xdn::WG_PART;
+        U.part[j] = ws.part[j] + (size_t)b * ws.nblk[j] * unet_job_pstride(j);
         U.nblk[j] = ws.nblk[j];
     }
     for (int k = 0; k < 3; ++k) U.h[k] = ws.h[k], U.w[k] = ws.w[k];
@@ -250,6 +251,8 @@ xdn::RedJob unet_red_job(int set, float *part, float *stage, float *grad, int nb
     r.transpose = L.type == UL_CONVT;  // Keras kernel [co][ci]; partials hold [ci][co]
     if (set == 7) r.ci_off = 0, r.ci_cnt = 32;               // c4_1, input channels 0-31 (+ bias)
     if (set == 8) r.ci_off = 32, r.ci_cnt = 16, r.bias = 0;  // c4_1, input channels 32-47
+    r.compact = 1;
+    r.pstride = unet_job_pstride(set);
     return r;
 }
 

@@ -515,13 +515,13 @@ __global__ __launch_bounds__(256) void wgrad_reduce1(RedBatch b)
     const RedJob &J = b.j[blockIdx.z];
     const int sp = blockIdx.y;
     const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= WG_PART) return;
+    if (e >= J.pstride) return;
     const int nb = J.nblk;
     const int lo = (int)((long)nb * sp / b.splits), hi = (int)((long)nb * (sp + 1) / b.splits);
     float s = 0.f;
     const float *p = J.part + e;
-    for (int i = lo; i < hi; ++i) s += p[(size_t)i * WG_PART];
-    J.stage[sp * WG_PART + e] = s;
+    for (int i = lo; i < hi; ++i) s += p[(size_t)i * J.pstride];
+    J.stage[sp * J.pstride + e] = s;
 }
 
 __global__ __launch_bounds__(256) void wgrad_reduce2(RedBatch b)
@@ -535,16 +535,16 @@ __global__ __launch_bounds__(256) void wgrad_reduce2(RedBatch b)
     if (e < nw)
     {
         const int t = e / (cc * J.co), r = e - t * cc * J.co, cl = r / J.co, co = r - cl * J.co;
-        src = t * 1024 + cl * 32 + co;
+        src = J.compact ? e : t * 1024 + cl * 32 + co;
         dst = J.transpose ? co * J.ci + J.ci_off + cl : (t * J.ci + J.ci_off + cl) * J.co + co;
     }
     else
     {
-        src = 9 * 1024 + (e - nw);
+        src = J.compact ? e : 9 * 1024 + (e - nw);
         dst = J.taps * J.ci * J.co + (e - nw);
     }
     float s = 0.f;
-    for (int sp = 0; sp < b.splits; ++sp) s += J.stage[sp * WG_PART + src];
+    for (int sp = 0; sp < b.splits; ++sp) s += J.stage[sp * J.pstride + src];
     J.grad[dst] = s;
 }
 
@@ -689,15 +689,16 @@ hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st)
         RedBatch b{};
         b.n = m;
         b.splits = WG_SPLITS;
-        int maxw = 0;
+        int maxw = 0, maxp = 0;
         for (int i = 0; i < m; ++i)
         {
             const RedJob &J = jobs[j0 + i];
             b.j[i] = J;
             const int w = J.taps * (J.ci_cnt < 0 ? J.ci : J.ci_cnt) * J.co + J.co;
             maxw = w > maxw ? w : maxw;
+            maxp = J.pstride > maxp ? J.pstride : maxp;
         }
-        hipLaunchKernelGGL(wgrad_reduce1, dim3(div_up(WG_PART, 256), WG_SPLITS, m), dim3(256), 0, st, b);
+        hipLaunchKernelGGL(wgrad_reduce1, dim3(div_up(maxp, 256), WG_SPLITS, m), dim3(256), 0, st, b);
         hipLaunchKernelGGL(wgrad_reduce2, dim3(div_up(maxw, 256), m), dim3(256), 0, st, b);
         if (hipError_t e = hipGetLastError()) return e;
     }

@@ -4,8 +4,8 @@
 //     v_mfma_f32_32x32x16_bf16 (M = 32 px of a row, N = 32 output channels per N-tile, K = 16
 //     input channels per chunk) over concat[a (CA ch), b (CB ch)] staged in LDS with zero padding
 //     to whole K chunks; 16x32-px tiles, every A fragment feeds up to three output rows. Forward
-//     (bias + ReLU -> bf16) and data gradient (flipped, transposed fragments; ReLU-masked bf16 or
-//     FP32 for concat and pooled inputs). Channel counts are the stored ones (8, 16, 24, 32, 48).
+//     (bias + ReLU -> bf16) and data gradient (flipped, transposed fragments; ReLU-masked, or
+//     plain for concat and pooled inputs). Channel counts are the stored ones (8, 16, 24, 32, 48).
 //   uwgrad<SRC, CA, CB, CO, MT>  weight gradient over 8x32-px strips, operands from pixel-major
 //     LDS tiles through ds_read_b64_tr_b16, one 32-channel input slice (MT) per instantiation,
 //     software-pipelined strip loop (as conv3x3_wgrad of k_cnn.hip).
@@ -74,15 +74,13 @@ __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
     constexpr int NT = (CO + 31) / 32;
     constexpr int LW = UT_W + 2, LH = UT_H + 2;
     constexpr int OS = CO + 8;  // epilogue pixel stride (bf16)
-    constexpr int IN_ELEMS = LH * LW * PS, OUT_ELEMS = EPI == UC_EPI_F32 ? 0 : UT_H * UT_W * OS;
+    constexpr int IN_ELEMS = LH * LW * PS, OUT_ELEMS = UT_H * UT_W * OS;
     __shared__ __attribute__((aligned(16))) uint16_t lds[IN_ELEMS > OUT_ELEMS ? IN_ELEMS : OUT_ELEMS];
-    const int tile = xcd_tile(blockIdx.x, b.map.total);
-    const TileLoc L = locate(b.map, tile);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
+    const TileLoc L = locate(b.map, xcd_tile(blockIdx.x, b.map.total));
     const UcFrame &F = b.f[L.f];
     const int H = F.H, W = F.W;
     const int x0 = L.tx * UT_W, y0 = L.ty * UT_H;
-    const int tid = threadIdx.x;
-
     if constexpr (SRC == UC_SRC_BF16)
     {
         constexpr int QA = CA / 8, QB = CB / 8, QP = CIP / 8;
@@ -130,7 +128,6 @@ __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
             *reinterpret_cast<u32x4 *>(&lds[pix * PS + 8]) = c;
         }
     }
-    const int lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
     u32x4 wf[9][KC][NT];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
@@ -169,29 +166,8 @@ __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
                 }
             }
 
-    // epilogue: lane owns output channel nt*32 + px of 16 pixels x = 8(k/4) + 4h + k%4 per row
-    if constexpr (EPI == UC_EPI_F32)
-    {
-#pragma unroll
-        for (int orow = 0; orow < 4; ++orow)
-        {
-            const int gy = y0 + 4 * wv + orow;
-            if (gy >= H) continue;
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-            {
-                const int n = nt * 32 + px;
-                if (n >= CO) continue;
-#pragma unroll
-                for (int k = 0; k < 16; ++k)
-                {
-                    const int gx = x0 + 8 * (k >> 2) + 4 * h + (k & 3);
-                    if (gx < W) static_cast<float *>(F.out)[((size_t)gy * W + gx) * CO + n] = acc[orow][nt][k];
-                }
-            }
-        }
-    }
-    else
+    // epilogue: lane owns output channel nt*32 + px of 16 pixels x = 8(k/4) + 4h + k%4 per row;
+    // through LDS to 16-byte stores of whole pixels
     {
         __syncthreads();  // the input tile is dead
 #pragma unroll
@@ -261,12 +237,12 @@ hipError_t launch_uconv(int kind, int n, const UcFrame *fr, hipStream_t st)
     case UC_F_48_16: return uconv_t<UC_SRC_BF16, 32, 16, 16, UC_EPI_RELU>(n, fr, st);
     case UC_F_24_8: return uconv_t<UC_SRC_BF16, 16, 8, 8, UC_EPI_RELU>(n, fr, st);
     case UC_D_8_8: return uconv_t<UC_SRC_BF16, 8, 0, 8, UC_EPI_MASK>(n, fr, st);
-    case UC_D_16_8: return uconv_t<UC_SRC_BF16, 16, 0, 8, UC_EPI_F32>(n, fr, st);
+    case UC_D_16_8: return uconv_t<UC_SRC_BF16, 16, 0, 8, UC_EPI_PLAIN>(n, fr, st);
     case UC_D_16_16: return uconv_t<UC_SRC_BF16, 16, 0, 16, UC_EPI_MASK>(n, fr, st);
-    case UC_D_32_16: return uconv_t<UC_SRC_BF16, 32, 0, 16, UC_EPI_F32>(n, fr, st);
+    case UC_D_32_16: return uconv_t<UC_SRC_BF16, 32, 0, 16, UC_EPI_PLAIN>(n, fr, st);
     case UC_D_32_32: return uconv_t<UC_SRC_BF16, 32, 0, 32, UC_EPI_MASK>(n, fr, st);
-    case UC_D_16_48: return uconv_t<UC_SRC_BF16, 16, 0, 48, UC_EPI_F32>(n, fr, st);
-    case UC_D_8_24: return uconv_t<UC_SRC_BF16, 8, 0, 24, UC_EPI_F32>(n, fr, st);
+    case UC_D_16_48: return uconv_t<UC_SRC_BF16, 16, 0, 48, UC_EPI_PLAIN>(n, fr, st);
+    case UC_D_8_24: return uconv_t<UC_SRC_BF16, 8, 0, 24, UC_EPI_PLAIN>(n, fr, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -424,7 +400,9 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
                 }
         }
     }
-    float *out = F.part + (size_t)li * WG_PART;
+    // compact partial [9][CS][CO] + [CO] of this input slice
+    constexpr int CS = SRC == UC_SRC_IMAGES ? 15 : ((CA + CB - 32 * MT) < 32 ? (CA + CB - 32 * MT) : 32);
+    float *out = F.part + (size_t)li * compact_part(9, CS, CO);
     float *red = reinterpret_cast<float *>(lds);  // [4 waves][1024]
     static_assert(sizeof(lds) >= 4 * 1024 * sizeof(float), "reduction buffer");
 #pragma unroll
@@ -438,19 +416,23 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
             red[wv * 1024 + ci * 32 + ch] = acc[t][k];
         }
         __syncthreads();
-        for (int e = tid; e < 1024; e += 256) out[t * 1024 + e] = (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]);
+        for (int e = tid; e < CS * CO; e += 256)
+        {
+            const int ci = e / CO, co = e - ci * CO, r = ci * 32 + co;
+            out[t * CS * CO + e] = (red[r] + red[1024 + r]) + (red[2048 + r] + red[3072 + r]);
+        }
     }
     db += __shfl_xor(db, 32);
     if (h == 0) dbs[wv][ch] = db;
     __syncthreads();
-    if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
+    if (tid < CO) out[9 * CS * CO + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
 }
 
+// up to 512 blocks per image (two per CU): the quarter- and half-resolution layers have few strips
 int ublocks(int H, int W)
 {
     const int strips = udiv_up(W, US_W) * udiv_up(H, US_H);
-    const int nb = udiv_up(strips, 16);
-    return nb < 1 ? 1 : nb;
+    return strips < 1 ? 1 : (strips < 512 ? strips : 512);
 }
 
 template <int SRC, int CA, int CB, int CO, int MT>
@@ -543,7 +525,8 @@ __global__ __launch_bounds__(256) void maxpool2(PoolBatch b)
     *reinterpret_cast<u32x4 *>(F.out + (size_t)p2 * C + q * 8) = u32x4{o[0], o[1], o[2], o[3]};
 }
 
-// one item = one full-resolution pixel x one 8-channel chunk
+// one item = one 2x2 window x one 8-channel chunk: vector loads of the window's four activations,
+// skip gradients and outputs
 template <int C>
 __global__ __launch_bounds__(256) void pool_bwd(PoolBwBatch b)
 {
@@ -552,37 +535,49 @@ __global__ __launch_bounds__(256) void pool_bwd(PoolBwBatch b)
     const int f = item_frame(b.begin, b.n, i);
     const PoolBwFrame &F = b.f[f];
     const int W = F.W, W2 = W / 2;
-    const int li = i - b.begin[f], q = li % (C / 8), p = li / (C / 8);
-    const int y = p / W, x = p - y * W;
-    const int wy = y & ~1, wx = x & ~1, me = ((y & 1) << 1) | (x & 1);
-    const uint16_t *act = F.act;
-    const float *dp = F.dpool + ((size_t)(y >> 1) * W2 + (x >> 1)) * C + q * 8;
-    const float *sk = F.dskip + (size_t)p * F.sstride + F.soff + q * 8;
-    uint32_t o[4];
+    const int li = i - b.begin[f], q = li % (C / 8), p2 = li / (C / 8);
+    const int y2 = p2 / W2, x2 = p2 - y2 * W2;
+    size_t pix[4];
 #pragma unroll
-    for (int j = 0; j < 8; j += 2)
+    for (int k = 0; k < 4; ++k) pix[k] = (size_t)(2 * y2 + (k >> 1)) * W + 2 * x2 + (k & 1);
+    u32x4 a[4], sk[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
     {
-        float r[2];
-#pragma unroll
-        for (int e = 0; e < 2; ++e)
-        {
-            const int c = q * 8 + j + e;
-            // first maximum of the window in row-major order
-            int arg = 0;
-            float best = u_bf(act, ((size_t)wy * W + wx) * C + c);
-#pragma unroll
-            for (int k = 1; k < 4; ++k)
-            {
-                const float v = u_bf(act, ((size_t)(wy + (k >> 1)) * W + wx + (k & 1)) * C + c);
-                if (v > best) best = v, arg = k;
-            }
-            const float g = (arg == me ? dp[j + e] : 0.f) + sk[j + e];
-            const float a = u_bf(act, (size_t)p * C + c);
-            r[e] = a > 0.f ? g : 0.f;
-        }
-        o[j >> 1] = u_pack2(r[0], r[1]);
+        a[k] = *reinterpret_cast<const u32x4 *>(F.act + pix[k] * C + q * 8);
+        sk[k] = *reinterpret_cast<const u32x4 *>(F.dskip + pix[k] * F.sstride + F.soff + q * 8);
     }
-    *reinterpret_cast<u32x4 *>(F.dy + (size_t)p * C + q * 8) = u32x4{o[0], o[1], o[2], o[3]};
+    const u32x4 dpw = *reinterpret_cast<const u32x4 *>(F.dpool + (size_t)p2 * C + q * 8);
+    auto half = [](const u32x4 &w, int c) {
+        const uint32_t x = c < 2 ? w.x : (c < 4 ? w.y : (c < 6 ? w.z : w.w));
+        return (c & 1) ? u_hi(x) : u_lo(x);
+    };
+    float r[4][8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = half(a[k], c);
+        // first maximum of the window in row-major order
+        int arg = 0;
+        float best = v[0];
+#pragma unroll
+        for (int k = 1; k < 4; ++k)
+            if (v[k] > best) best = v[k], arg = k;
+        const float dp = half(dpw, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const float g = (arg == k ? dp : 0.f) + half(sk[k], c);
+            r[k][c] = v[k] > 0.f ? g : 0.f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        *reinterpret_cast<u32x4 *>(F.dy + pix[k] * C + q * 8) =
+            u32x4{u_pack2(r[k][0], r[k][1]), u_pack2(r[k][2], r[k][3]), u_pack2(r[k][4], r[k][5]),
+                  u_pack2(r[k][6], r[k][7])};
 }
 
 template <int C>
@@ -622,7 +617,7 @@ static hipError_t pool_bwd_t(int n, const PoolBwFrame *fr, hipStream_t st)
     {
         b.f[i] = fr[i];
         b.begin[i] = acc;
-        acc += fr[i].H * fr[i].W * (C / 8);
+        acc += (fr[i].H / 2) * (fr[i].W / 2) * (C / 8);
     }
     b.begin[n] = b.total = acc;
     if (acc == 0) return hipSuccess;
@@ -700,114 +695,145 @@ __global__ __launch_bounds__(256) void convT1x1s2(CtBatch b)
     }
 }
 
-// backward: blocks walk coarse pixels in chunks of 256; dIn per pixel, dW = sum in (x) dOut(2y,2x)
-// through LDS, db = sum of dOut over all output positions.
+// backward: blocks walk coarse pixels in chunks of 256. Staging (one thread per coarse pixel):
+// dIn = W dOut(2y,2x) masked by the input's ReLU, and into LDS the input, dOut(2y,2x) and the sum
+// of dOut over the 2x2 output block. Then dW [ci][co] = sum in (x) dOut(2y,2x) in 4x4 register
+// tiles (lane groups split the chunk's pixels) and db = sum of the block sums; the groups are
+// reduced through LDS at the end.
 template <int C>
 __global__ __launch_bounds__(256) void convT_bwd(CtBatch b)
 {
-    __shared__ float ws[C][C + 1];   // [co][ci]
-    __shared__ float xin[256][C + 1];
-    __shared__ float gout[256][C + 1];
-    __shared__ float dbl[256 / 64][C];
+    constexpr int CB = C / 4;          // 4-channel blocks
+    constexpr int LPG = CB * CB;       // lanes per group: one 4x4 (ci, co) tile each
+    constexpr int G = 256 / LPG;       // groups, each taking every G-th pixel of a chunk
+    constexpr int PS = C + 4;          // LDS pixel stride (floats)
+    __shared__ __attribute__((aligned(16))) float xin[256 * PS];
+    __shared__ __attribute__((aligned(16))) float gin[256 * PS];
+    __shared__ __attribute__((aligned(16))) float gsum[256 * PS];
+    __shared__ __attribute__((aligned(16))) float ws[C * C];  // [co][ci]
+    static_assert(G * (C * C + C) <= 256 * PS, "group reduction buffer");
     int f = 0;
     while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
     const CtFrame &F = b.f[f];
     const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
     const int h = F.h, w = F.w_, np = h * w, W2 = 2 * w;
     const int tid = threadIdx.x;
-    for (int e = tid; e < C * C; e += 256) ws[e / C][e % C] = F.w[e];
-    // dW accumulators: thread t owns the (ci, co) pairs e = t + 256 k
-    constexpr int NE = C * C / 256 > 0 ? C * C / 256 : 1;
-    float dw[NE];
+    for (int e = tid; e < C * C; e += 256) ws[e] = F.w[e];
+    const int grp = tid / LPG, l = tid - grp * LPG, cib = l / CB, cob = l - cib * CB;
+    float acc[4][4], bacc[4];
 #pragma unroll
-    for (int k = 0; k < NE; ++k) dw[k] = 0.f;
-    float dbp[C];
+    for (int i = 0; i < 4; ++i)
+    {
+        bacc[i] = 0.f;
 #pragma unroll
-    for (int c = 0; c < C; ++c) dbp[c] = 0.f;
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+    }
     for (int base = li * 256; base < np; base += nblk * 256)
     {
-        __syncthreads();  // ws written / previous chunk's LDS reads done
+        __syncthreads();  // ws written / the previous chunk's LDS reads done
         const int p = base + tid;
+        float x[C], g[C], gs[C];
         if (p < np)
         {
-            const int y = p / w, x = p - y * w;
+            const int y = p / w, xx = p - y * w;
 #pragma unroll
             for (int c = 0; c < C; c += 8)
             {
                 const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * C + c);
-                xin[tid][c] = u_lo(v.x); xin[tid][c + 1] = u_hi(v.x); xin[tid][c + 2] = u_lo(v.y);
-                xin[tid][c + 3] = u_hi(v.y); xin[tid][c + 4] = u_lo(v.z); xin[tid][c + 5] = u_hi(v.z);
-                xin[tid][c + 6] = u_lo(v.w); xin[tid][c + 7] = u_hi(v.w);
+                x[c] = u_lo(v.x); x[c + 1] = u_hi(v.x); x[c + 2] = u_lo(v.y); x[c + 3] = u_hi(v.y);
+                x[c + 4] = u_lo(v.z); x[c + 5] = u_hi(v.z); x[c + 6] = u_lo(v.w); x[c + 7] = u_hi(v.w);
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k)
             {
-                const float *g = F.dout + ((size_t)(2 * y + (k >> 1)) * W2 + 2 * x + (k & 1)) * F.dstride;
+                const uint16_t *gp = F.dout + ((size_t)(2 * y + (k >> 1)) * W2 + 2 * xx + (k & 1)) * F.dstride;
 #pragma unroll
-                for (int c = 0; c < C; ++c)
+                for (int c = 0; c < C; c += 8)
                 {
-                    const float v = g[c];
-                    dbp[c] += v;
-                    if (k == 0) gout[tid][c] = v;
+                    const u32x4 v = *reinterpret_cast<const u32x4 *>(gp + c);
+                    const float e[8] = {u_lo(v.x), u_hi(v.x), u_lo(v.y), u_hi(v.y),
+                                        u_lo(v.z), u_hi(v.z), u_lo(v.w), u_hi(v.w)};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                    {
+                        if (k == 0)
+                            g[c + j] = gs[c + j] = e[j];
+                        else
+                            gs[c + j] += e[j];
+                    }
                 }
             }
-            // dIn = W dOut(2y,2x) (W is [co][ci]), masked by the ReLU of the input
-            uint32_t o[C / 2];
-#pragma unroll
-            for (int ci = 0; ci < C; ci += 2)
-            {
-                float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-                for (int co = 0; co < C; ++co)
-                {
-                    s0 += ws[co][ci] * gout[tid][co];
-                    s1 += ws[co][ci + 1] * gout[tid][co];
-                }
-                o[ci / 2] = u_pack2(xin[tid][ci] > 0.f ? s0 : 0.f, xin[tid][ci + 1] > 0.f ? s1 : 0.f);
-            }
-#pragma unroll
-            for (int c = 0; c < C / 2; c += 4)
-                *reinterpret_cast<u32x4 *>(F.din + (size_t)p * C + 2 * c) = u32x4{o[c], o[c + 1], o[c + 2], o[c + 3]};
         }
         else
         {
 #pragma unroll
-            for (int c = 0; c < C; ++c) xin[tid][c] = gout[tid][c] = 0.f;
+            for (int c = 0; c < C; ++c) x[c] = g[c] = gs[c] = 0.f;
         }
-        __syncthreads();
-        const int cnt = min(256, np - base);
 #pragma unroll
-        for (int k = 0; k < NE; ++k)
+        for (int c = 0; c < C; c += 4)
         {
-            const int e = tid + 256 * k;
-            if (e >= C * C) break;
-            const int ci = e / C, co = e - ci * C;
-            float s = 0.f;
-            for (int q = 0; q < cnt; ++q) s += xin[q][ci] * gout[q][co];
-            dw[k] += s;
+            *reinterpret_cast<float4 *>(&xin[tid * PS + c]) = float4{x[c], x[c + 1], x[c + 2], x[c + 3]};
+            *reinterpret_cast<float4 *>(&gin[tid * PS + c]) = float4{g[c], g[c + 1], g[c + 2], g[c + 3]};
+            *reinterpret_cast<float4 *>(&gsum[tid * PS + c]) = float4{gs[c], gs[c + 1], gs[c + 2], gs[c + 3]};
+        }
+        __syncthreads();  // ws (first chunk) and this chunk's tiles visible
+        if (p < np)
+        {
+            float d[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) d[c] = 0.f;
+#pragma unroll
+            for (int co = 0; co < C; ++co)
+#pragma unroll
+                for (int c = 0; c < C; c += 4)
+                {
+                    const float4 wv = *reinterpret_cast<const float4 *>(&ws[co * C + c]);
+                    d[c] += wv.x * g[co]; d[c + 1] += wv.y * g[co]; d[c + 2] += wv.z * g[co]; d[c + 3] += wv.w * g[co];
+                }
+#pragma unroll
+            for (int c = 0; c < C; c += 8)
+                *reinterpret_cast<u32x4 *>(F.din + (size_t)p * C + c) =
+                    u32x4{u_pack2(x[c] > 0.f ? d[c] : 0.f, x[c + 1] > 0.f ? d[c + 1] : 0.f),
+                          u_pack2(x[c + 2] > 0.f ? d[c + 2] : 0.f, x[c + 3] > 0.f ? d[c + 3] : 0.f),
+                          u_pack2(x[c + 4] > 0.f ? d[c + 4] : 0.f, x[c + 5] > 0.f ? d[c + 5] : 0.f),
+                          u_pack2(x[c + 6] > 0.f ? d[c + 6] : 0.f, x[c + 7] > 0.f ? d[c + 7] : 0.f)};
+        }
+        const int cnt = min(256, np - base);
+        for (int q = grp; q < cnt; q += G)
+        {
+            const float4 xv = *reinterpret_cast<const float4 *>(&xin[q * PS + 4 * cib]);
+            const float4 gv = *reinterpret_cast<const float4 *>(&gin[q * PS + 4 * cob]);
+            const float xa[4] = {xv.x, xv.y, xv.z, xv.w}, ga[4] = {gv.x, gv.y, gv.z, gv.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += xa[i] * ga[j];
+            if (cib == 0)
+            {
+                const float4 sv = *reinterpret_cast<const float4 *>(&gsum[q * PS + 4 * cob]);
+                bacc[0] += sv.x; bacc[1] += sv.y; bacc[2] += sv.z; bacc[3] += sv.w;
+            }
         }
     }
-    // partial: tap 0 rows [ci][co] of the [9][32][32] layout, bias at 9*1024
-    float *out = F.part + (size_t)li * WG_PART;
-#pragma unroll
-    for (int k = 0; k < NE; ++k)
-    {
-        const int e = tid + 256 * k;
-        if (e < C * C) out[(e / C) * 32 + (e % C)] = dw[k];
-    }
-    // bias: reduce dbp over the block (wave shuffles, then the 4 waves)
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-    {
-        float v = dbp[c];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        dbp[c] = v;
-    }
-    if ((tid & 63) == 0)
-#pragma unroll
-        for (int c = 0; c < C; ++c) dbl[tid >> 6][c] = dbp[c];
+    // reduce the G groups: red[grp][ci * C + co], bias at red[grp][C * C + co]
     __syncthreads();
-    if (tid < C) out[9 * 1024 + tid] = (dbl[0][tid] + dbl[1][tid]) + (dbl[2][tid] + dbl[3][tid]);
+    float *red = xin;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[grp * (C * C + C) + (4 * cib + i) * C + 4 * cob + j] = acc[i][j];
+    if (cib == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[grp * (C * C + C) + C * C + 4 * cob + j] = bacc[j];
+    __syncthreads();
+    // compact partial: dW [ci][co], then db
+    float *out = F.part + (size_t)li * compact_part(1, C, C);
+    for (int e = tid; e < C * C + C; e += 256)
+    {
+        float sum = 0.f;
+        for (int k = 0; k < G; ++k) sum += red[k * (C * C + C) + e];
+        out[e] = sum;
+    }
 }
 
 template <int C, bool BWD>
@@ -931,9 +957,8 @@ __global__ __launch_bounds__(256) void head_bwd(HeadBatch b)
 #pragma unroll
         for (int e = 0; e < 54; ++e) red[tid >> 6][e] = g[e];
     __syncthreads();
-    float *out = F.part + (size_t)li * WG_PART;
-    if (tid < 48) out[(tid / 6) * 32 + tid % 6] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
-    else if (tid < 54) out[9 * 1024 + tid - 48] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+    float *out = F.part + (size_t)li * compact_part(1, 8, 6);  // [8][6] then [6]
+    if (tid < 54) out[tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
 }
 
 hipError_t launch_head(int n, const HeadFrame *fr, hipStream_t st)

@@ -47,17 +47,21 @@ struct PackJob {
 hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st);
 hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st);
 // One reduction job: the nblk partials of one layer of one frame -> its flat gradient.
-// The partials hold [taps][32 ci][32 co] + [32 co] of an input-channel slice ci_off .. ci_off +
-// ci_cnt - 1 (ci_cnt < 0: all ci); the gradient is written in the layer's flat layout
+// The partials hold an input-channel slice ci_off .. ci_off + ci_cnt - 1 (ci_cnt < 0: all ci),
+// [taps][32 ci][32 co] + [32 co] in WG_PART floats each, or when compact [taps][ci_cnt][co] +
+// [co] in pstride floats each; the gradient is written in the layer's flat layout
 // [taps][ci][co] (or [co][ci] when transpose, taps 1) followed by [co] when bias.
 struct RedJob {
-    const float *part;  // [nblk][WG_PART]
-    float *stage;       // [WG_SPLITS][WG_PART]
+    const float *part;  // [nblk][pstride]
+    float *stage;       // [WG_SPLITS][pstride]
     float *grad;        // the layer's weights, then its bias
     int nblk, ci, co;
     int taps = 9, ci_off = 0, ci_cnt = -1, bias = 1, transpose = 0;
+    int compact = 0, pstride = WG_PART;
 };
-constexpr int WG_MAX_JOBS = 64;  // per launch; longer job lists are split
+// floats of one compact partial: [taps][cc][co] + [co]
+constexpr int compact_part(int taps, int cc, int co) { return taps * cc * co + co; }
+constexpr int WG_MAX_JOBS = 48;  // per launch; longer job lists are split
 hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st);
 hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st);
 int wgrad_blocks(int H, int W);

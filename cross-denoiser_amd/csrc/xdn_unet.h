@@ -27,14 +27,14 @@
 namespace xdn {
 
 enum { UC_SRC_BF16 = 0, UC_SRC_IMAGES = 1 };
-enum { UC_EPI_RELU = 0, UC_EPI_MASK = 1, UC_EPI_F32 = 2 };
+enum { UC_EPI_RELU = 0, UC_EPI_MASK = 1, UC_EPI_PLAIN = 2 };
 
 // 3x3 'same' convolution over concat[a (CA ch), b (CB ch)] (or the five images), forward or data
 // gradient (the data gradient is the same kernel on dY with the flipped, transposed fragments).
 struct UcFrame {
     const void *a, *b;          // bf16 sources (b may be null when CB == 0)
     const float *img[5];        // FP32 3-channel images (UC_SRC_IMAGES)
-    void *out;                  // bf16 [H][W][CO] (RELU / MASK) or FP32 [H][W][CO] (F32)
+    void *out;                  // bf16 [H][W][CO]
     const uint16_t *mask;       // bf16 [H][W][CO] activation whose ReLU masks the output (MASK)
     const void *wfrag;          // 16-byte B fragments [9][KC][NT][64]
     const float *bias;          // [NT * 32] (RELU)
@@ -52,22 +52,23 @@ enum {
     UC_F_48_16,   // concat 32+16 -> 16 ReLU
     UC_F_24_8,    // concat 16+8 -> 8 ReLU
     UC_D_8_8,     // dgrad dY 8 -> dX 8, masked
-    UC_D_16_8,    // dgrad dY 16 -> dX 8, FP32 (pool input)
+    UC_D_16_8,    // dgrad dY 16 -> dX 8, unmasked (pool input)
     UC_D_16_16,   // dgrad dY 16 -> dX 16, masked
-    UC_D_32_16,   // dgrad dY 32 -> dX 16, FP32 (pool input)
+    UC_D_32_16,   // dgrad dY 32 -> dX 16, unmasked (pool input)
     UC_D_32_32,   // dgrad dY 32 -> dX 32, masked
-    UC_D_16_48,   // dgrad dY 16 -> dX 48, FP32 (concat input)
-    UC_D_8_24,    // dgrad dY 8 -> dX 24, FP32 (concat input)
+    UC_D_16_48,   // dgrad dY 16 -> dX 48, unmasked (concat input)
+    UC_D_8_24,    // dgrad dY 8 -> dX 24, unmasked (concat input)
     UC_KINDS
 };
 
-// Weight gradient of a 3x3 layer over concat[a (CA), b (CB)] (or the images) and dY (CO ch): block
-// partials [9][32 ci][32 co] + [32] of the 32-channel input slice starting at 32 * mtile.
+// Weight gradient of a 3x3 layer over concat[a (CA), b (CB)] (or the images) and dY (CO ch): per
+// block one compact partial [9][cs][CO] + [CO] of the input slice [32 * mtile, 32 * mtile + cs)
+// (cs = min(32, channels - 32 * mtile); compact_part(9, cs, CO) floats).
 struct UwFrame {
     const void *a, *b;
     const float *img[5];
     const uint16_t *dy;         // bf16 [H][W][CO]
-    float *part;                // [nblk][WG_PART]
+    float *part;                // [nblk][compact_part(9, cs, CO)]
     int H, W, nblk;
 };
 enum {
@@ -92,13 +93,13 @@ struct PoolFrame {
 };
 hipError_t launch_maxpool(int C, int n, const PoolFrame *fr, hipStream_t st);
 
-// Gradient into a pooled activation act [H][W][C]: the pooled gradient (FP32 [H/2][W/2][C]) goes to
-// the first maximum of each 2x2 window (row-major), the skip gradient (FP32 [H][W][sstride], channels
-// soff..soff+C-1) is added, and the ReLU of act masks the sum -> bf16 dY [H][W][C].
+// Gradient into a pooled activation act [H][W][C]: the pooled gradient (bf16 [H/2][W/2][C]) goes to
+// the first maximum of each 2x2 window (row-major), the skip gradient (bf16 [H][W][sstride], channels
+// soff..soff+C-1) is added in FP32, and the ReLU of act masks the sum -> bf16 dY [H][W][C].
 struct PoolBwFrame {
-    const float *dpool;
+    const uint16_t *dpool;
     const uint16_t *act;
-    const float *dskip;
+    const uint16_t *dskip;
     uint16_t *dy;
     int H, W, sstride, soff;
 };
@@ -106,22 +107,21 @@ hipError_t launch_pool_bwd(int C, int n, const PoolBwFrame *fr, hipStream_t st);
 
 // Transposed 1x1 convolution, stride 2 ('same'): out(2y,2x) = W^T in(y,x) + b, other positions b.
 // W is the Keras kernel [co][ci] (shape 1x1xCOxCI). Backward: dIn(y,x) = W dOut(2y,2x) masked by
-// the ReLU of in -> bf16; partials of dW [ci][co] (tap 0 of the [9][32][32] layout) and db (all
-// output positions).
+// the ReLU of in -> bf16; per block one compact partial: dW [ci][co] then db (all output positions).
 struct CtFrame {
     const uint16_t *in;         // bf16 [h][w][C]
     uint16_t *out;              // bf16 [2h][2w][C] (forward)
-    const float *dout;          // FP32 [2h][2w][dstride], channels 0..C-1 (backward)
+    const uint16_t *dout;       // bf16 [2h][2w][dstride], channels 0..C-1 (backward)
     uint16_t *din;              // bf16 [h][w][C] (backward)
     const float *w, *b;         // FP32 master
-    float *part;                // [nblk][WG_PART] (backward)
+    float *part;                // [nblk][C * C + C] (backward)
     int h, w_, dstride, nblk;
 };
 hipError_t launch_convT(int C, int n, const CtFrame *fr, hipStream_t st);
 hipError_t launch_convT_bwd(int C, int n, const CtFrame *fr, hipStream_t st);
 
 // Head: 1x1 convolution 8 -> 6, FP32 output; backward: d(in) = W dparam masked by the ReLU of in
-// (bf16), partials of dW [ci][co] and db.
+// (bf16), per block one compact partial [8][6] + [6].
 struct HeadFrame {
     const uint16_t *in;         // bf16 [H][W][8]
     float *param;               // FP32 [H][W][6]
@@ -145,6 +145,6 @@ struct UPackJob {
 };
 hipError_t launch_upack(int n, const UPackJob *jobs, hipStream_t st);
 
-int ublocks(int H, int W);      // weight-gradient (and small-kernel) blocks per image
+int ublocks(int H, int W);      // weight-gradient blocks per image
 
 }  // namespace xdn
