@@ -517,6 +517,7 @@ int xdn_cnn5_forward(const float *ols1, const float *ols2, const float *prev_out
             else
                 f.in[0] = ws.X[l - 1] + o32;
             f.out = l < 4 ? static_cast<void *>(ws.X[l] + o32) : static_cast<void *>(param + (size_t)b * N * 6);
+            f.mask_out = l < 4 ? ws.M[l] + (size_t)b * N : nullptr;
             f.wfrag = pk + kLay.fwd_off[l];
             f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[l]);
             f.H = height;
@@ -574,7 +575,7 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
             const size_t o32 = (size_t)b * N * 32;
             f.in[0] = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6) : static_cast<const void *>(ws.dY[cur] + o32);
             f.out = ws.dY[1 - cur] + o32;
-            f.mask = ws.X[l - 1] + o32;
+            f.mask = ws.M[l - 1] + (size_t)b * N;
             f.wfrag = pk + kLay.dg_off[l];
             f.H = height;
             f.W = width;

@@ -66,11 +66,12 @@ struct Cnn5Layout {
 };
 inline const Cnn5Layout kLay;
 
-// Workspace of cnn5 for B images of H x W: hidden activations, gradient ping-pong, and per layer
-// the weight-gradient partials of every block and the reduction stage (all five layers are
-// reduced together after the backward pass).
+// Workspace of cnn5 for B images of H x W: hidden activations, gradient ping-pong, the hidden
+// activations' ReLU bits, and per layer the weight-gradient partials of every block and the
+// reduction stage (all five layers are reduced together after the backward pass).
 struct Cnn5Ws {
     uint16_t *X[4], *dY[2];
+    uint32_t *M[4];  // ReLU bits of X[l], one word per pixel
     float *part[5], *stage[5];
     int nblk[5];  // weight-gradient blocks per image and layer
     size_t bytes;
@@ -81,6 +82,7 @@ struct Cnn5Ws {
         size_t o = 0;
         for (int i = 0; i < 4; ++i) X[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
         for (int i = 0; i < 2; ++i) dY[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
+        for (int i = 0; i < 4; ++i) M[i] = reinterpret_cast<uint32_t *>(p + o), o += al256((size_t)B * H * W * 4);
         for (int l = 0; l < 5; ++l)
         {
             nblk[l] = xdn::wgrad_blocks(H, W);

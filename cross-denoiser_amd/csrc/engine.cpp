@@ -221,6 +221,7 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
                 else
                     f.in[0] = cw[i].X[l - 1];
                 f.out = l < 4 ? static_cast<void *>(cw[i].X[l]) : static_cast<void *>(sc[i].param);
+                f.mask_out = l < 4 ? cw[i].M[l] : nullptr;
                 f.wfrag = pk + kLay.fwd_off[l];
                 f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[l]);
                 f.H = s.height; f.W = s.width;
@@ -300,7 +301,7 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
                 memset(&f, 0, sizeof(f));
                 f.in[0] = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
                 f.out = cw[i].dY[1 - cur];
-                f.mask = cw[i].X[l - 1];
+                f.mask = cw[i].M[l - 1];
                 f.wfrag = static_cast<const char *>(s.packed) + kLay.dg_off[l];
                 f.H = s.height; f.W = s.width;
             }

@@ -8,16 +8,18 @@
 //     tile plus a 1-px halo staged in LDS as NHWC bf16 with a padded pixel stride (conflict-free
 //     ds_read_b128). Each wave owns 4 output rows; every A fragment loaded from LDS feeds the
 //     MFMAs of up to three output rows (the three kernel rows), so LDS traffic is ~0.5 fragments
-//     per MFMA. Kernel fragments live in VGPRs for the whole tile.
+//     per MFMA. Persistent, software-pipelined blocks; the kernel fragments sit in LDS.
 //     Inputs: bf16 activations, or the five FP32 3-channel images of the network input (the
 //     concatenation and the bf16 rounding are fused into the staging), or the FP32 6-channel
-//     gradient of the network output. Epilogues: bias + ReLU -> bf16, ReLU-mask of the layer
-//     input (backward) -> bf16, or bias -> FP32 6 channels (network output).
+//     gradient of the network output. Epilogues: bias + ReLU -> bf16 (plus one 32-bit word per
+//     pixel of ReLU bits, bit c = channel c > 0), the backward's ReLU mask of the layer input
+//     (read as those bits: 4 B/px instead of the 64 B/px activation) -> bf16, or bias -> FP32 6
+//     channels (network output).
 //   conv3x3_wgrad<X, CI, DY>  weight gradient: M = input channel, N = output channel, K = 16
-//     pixels (2 rows x 8 px). Both operands are staged channel-major in LDS, X three times
-//     (shifted by -1, 0, +1 px) so every operand is one aligned ds_read_b128. A wave keeps all 9
-//     taps' 32x32 FP32 accumulators (AGPRs) and loops over many 8x32-px strips; per-block partial
-//     sums are reduced by wgrad_reduce in two deterministic stages.
+//     pixels (2 rows x 8 px). Both operands are staged pixel-major in LDS as they lie in HBM and
+//     read channel-major with the transposing ds_read_b64_tr_b16. A wave keeps all 9 taps' 32x32
+//     FP32 accumulators (AGPRs) and loops over many 8x32-px strips; per-block partial sums are
+//     reduced by wgrad_reduce in two deterministic stages.
 //   pack_weights  FP32 master weights -> bf16 MFMA B fragments for the forward and the data
 //     gradient (flipped/transposed) convolutions.
 #include "xdn_common.h"
@@ -70,6 +72,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
     constexpr int IN_ELEMS = LH * LW * PS, OUT_ELEMS = CT_H * CT_W * 40;  // input tile / epilogue tile
     __shared__ __attribute__((aligned(16))) uint16_t lds[IN_ELEMS > OUT_ELEMS ? IN_ELEMS : OUT_ELEMS];
     __shared__ u32x4 wl[9 * KC * 64];  // kernel fragments of the current frame
+    __shared__ uint32_t mlds[EPI == CONV_EPI_MASK ? CT_H * CT_W : 1];  // ReLU bits of the tile (backward)
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
     const int total = b.map.total;
@@ -174,6 +177,17 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
     commit();
     __syncthreads();
     if (it + (int)gridDim.x < total) fetch(it + gridDim.x);
+    // the backward's ReLU bits of the tile's pixels tid and tid + 256, loaded now so their latency
+    // hides behind the MFMAs; they go through LDS to the lanes that store those pixels
+    uint32_t mpre[2] = {0u, 0u};
+    if constexpr (EPI == CONV_EPI_MASK)
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+        {
+            const int pix = tid + 256 * k;
+            const int gy = y0 + pix / CT_W, gx = x0 + (pix & (CT_W - 1));
+            if (gy < H && gx < W) mpre[k] = F.mask[(size_t)gy * W + gx];
+        }
 
     f32x16 acc[4];
 #pragma unroll
@@ -249,25 +263,41 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
                 lds[((4 * wv + orow) * CT_W + x) * OS + px] =
                     (uint16_t)f2bf(EPI == CONV_EPI_RELU ? fmaxf(v + bias, 0.f) : v);
             }
+        if constexpr (EPI == CONV_EPI_MASK)
+        {
+            mlds[tid] = mpre[0];
+            mlds[tid + 256] = mpre[1];
+        }
         __syncthreads();
+        // four consecutive lanes (one DPP quad) hold the four 8-channel quarters of one pixel
         for (int i = tid; i < CT_H * CT_W * 4; i += 256)
         {
             const int pix = i >> 2, q = i & 3;
             const int gy = y0 + pix / CT_W, gx = x0 + (pix & (CT_W - 1));
-            if (gy >= H || gx >= W) continue;
+            const bool in = gy < H && gx < W;
             u32x4 v = *reinterpret_cast<const u32x4 *>(&lds[pix * OS + q * 8]);
-            const size_t o = ((size_t)gy * W + gx) * 32 + q * 8;
+            const size_t p = (size_t)gy * W + gx, o = p * 32 + q * 8;
+            if constexpr (EPI == CONV_EPI_RELU)
+            {
+                // ReLU bits of the pixel, bit 8q + e = (bf16 channel 8q + e > 0): its bits lie in
+                // [1, 0x7FFF] (a -0 from the max counts as zero); the quad's bytes are ORed with
+                // two DPP quad permutes (xor 1: [1,0,3,2] = 0xB1, xor 2: [2,3,0,1] = 0x4E)
+                auto pos = [](uint32_t hb) { return (hb - 1u) < 0x7FFFu ? 1u : 0u; };
+                auto two = [&](uint32_t w) { return pos(w & 0xFFFFu) | (pos(w >> 16) << 1); };
+                uint32_t m = (two(v.x) | (two(v.y) << 2) | (two(v.z) << 4) | (two(v.w) << 6)) << (8 * q);
+                m |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0xB1, 0xF, 0xF, false);
+                m |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x4E, 0xF, 0xF, false);
+                if (F.mask_out && in && q == 0) F.mask_out[p] = m;
+            }
+            if (!in) continue;
             if constexpr (EPI == CONV_EPI_MASK)
             {
                 // keep the gradient where the layer's bf16 activation is > 0 (ReLU derivative)
-                const u32x4 m = *reinterpret_cast<const u32x4 *>(F.mask + o);
-                auto keep = [](uint32_t mw, uint32_t vw) {
-                    const uint32_t lo = mw & 0xFFFFu, hi = mw >> 16;
-                    const uint32_t klo = ((lo & 0x8000u) == 0 && lo != 0) ? 0xFFFFu : 0u;
-                    const uint32_t khi = ((hi & 0x8000u) == 0 && hi != 0) ? 0xFFFF0000u : 0u;
-                    return vw & (klo | khi);
+                const uint32_t mb = mlds[pix] >> (8 * q);
+                auto keep = [](uint32_t bits, uint32_t vw) {
+                    return vw & (((bits & 1u) ? 0xFFFFu : 0u) | ((bits & 2u) ? 0xFFFF0000u : 0u));
                 };
-                v.x = keep(m.x, v.x); v.y = keep(m.y, v.y); v.z = keep(m.z, v.z); v.w = keep(m.w, v.w);
+                v.x = keep(mb, v.x); v.y = keep(mb >> 2, v.y); v.z = keep(mb >> 4, v.z); v.w = keep(mb >> 6, v.w);
             }
             *reinterpret_cast<u32x4 *>(static_cast<uint16_t *>(F.out) + o) = v;
         }

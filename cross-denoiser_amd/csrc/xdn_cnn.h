@@ -24,7 +24,8 @@ constexpr int WG_SPLITS = 32;
 struct ConvFrame {
     const void *in[5];       // see the kernel kinds
     void *out;
-    const uint16_t *mask;    // bf16 [H][W][32] (mask epilogue)
+    const uint32_t *mask;    // [H][W] ReLU bits of the layer input, bit c = channel c > 0 (mask epilogue)
+    uint32_t *mask_out;      // [H][W] ReLU bits of the output (bias + ReLU epilogue; may be null)
     const void *wfrag;       // 16-byte B fragments [9][KC][64]
     const float *bias;       // [32]
     int H, W;
