@@ -436,6 +436,49 @@ int xdn_filter_train(const float *ols1, const float *ols2, const float *albedo, 
     return XDN_OK;
 }
 
+size_t xdn_frame_error_workspace_size(int batch, int height, int width)
+{
+    if (batch <= 0 || height <= 0 || width <= 0) return 0;
+    return (size_t)batch * xdn::frame_error_blocks(height, width) * sizeof(float);
+}
+
+int xdn_frame_error(const float *out, const float *emissive, const float *env_light, const float *ref,
+                    const float *ref_emissive, const float *ref_env_light, float *out_composed, float *err,
+                    void *workspace, size_t workspace_bytes, int batch, int height, int width, void *stream)
+{
+    if (int e = check_shape(batch, height, width)) return e;
+    if (!out || !ref || !err) return fail(XDN_EINVAL, "frame error: out, ref and err are required");
+    if (workspace_bytes < xdn_frame_error_workspace_size(batch, height, width))
+        return fail(XDN_EINVAL, "frame error: workspace too small");
+    const size_t N3 = px(height, width) * 3;
+    const int nb = xdn::frame_error_blocks(height, width);
+    float *parts = static_cast<float *>(workspace);
+    auto at = [&](const float *p, int b) { return p ? p + (size_t)b * N3 : nullptr; };
+    for (int b0 = 0; b0 < batch; b0 += XDN_MAX_FRAMES)
+    {
+        const int n = batch - b0 < XDN_MAX_FRAMES ? batch - b0 : XDN_MAX_FRAMES;
+        xdn::ErrFrame fr[XDN_MAX_FRAMES];
+        const float *pp[XDN_MAX_FRAMES];
+        int cnt[XDN_MAX_FRAMES];
+        float scale[XDN_MAX_FRAMES];
+        float *dst[XDN_MAX_FRAMES];
+        for (int i = 0; i < n; ++i)
+        {
+            const int b = b0 + i;
+            fr[i] = xdn::ErrFrame{at(out, b), at(emissive, b), at(env_light, b), at(ref, b), at(ref_emissive, b),
+                                  at(ref_env_light, b), out_composed ? out_composed + (size_t)b * N3 : nullptr,
+                                  parts + (size_t)b * nb, height, width};
+            pp[i] = parts + (size_t)b * nb;
+            cnt[i] = nb;
+            scale[i] = (float)(1.0 / (double)N3);
+            dst[i] = err + b;
+        }
+        if (int e = hip_check(xdn::launch_frame_error(n, fr, S(stream)), "frame error")) return e;
+        if (int e = hip_check(xdn::launch_sums(n, pp, cnt, scale, dst, S(stream)), "frame error sum")) return e;
+    }
+    return XDN_OK;
+}
+
 int xdn_adam(float *param, const float *grad, float *m, float *v, int count, int step, float lr, float beta1,
              float beta2, float eps, void *stream)
 {

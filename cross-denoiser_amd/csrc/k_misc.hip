@@ -1,5 +1,6 @@
-// Small kernels of the per-frame training step: the Adam update of the bandwidth network and the
-// deterministic reduction of the per-tile loss partials.
+// Small kernels of the per-frame training step: the Adam update of the bandwidth network, the
+// deterministic reduction of the per-tile loss partials, and the frame driver's report (the
+// displayed frame and its RelL2 against the reference).
 #include "xdn_common.h"
 #include "xdn_misc.h"
 
@@ -123,6 +124,81 @@ hipError_t launch_sums(int n, const float *const *parts, const int *count, const
     }
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(sum_kernel, dim3(n), dim3(256), 0, st, b);
+    return hipGetLastError();
+}
+
+// Frame report of scripts/main.py:344-346 with RelL2 of :191-196, one pass: the displayed frame
+// out + emissive + envLight (written when `composed` is set), the reference ref + ref_emissive +
+// ref_envLight, and per block the sum over its pixels and channels of
+// (ref_c - out_c)^2 / (mean over channels of ref_c)^2 + 1e-2). Absent (null) terms are zero.
+struct ErrBatch {
+    int n;
+    int blk_begin[XDN_MAX_FRAMES + 1];
+    ErrFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(ErrBatch) <= 4096, "kernel argument too large");
+
+__global__ __launch_bounds__(256) void frame_error_kernel(ErrBatch b)
+{
+    __shared__ float red[4];
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const ErrFrame &F = b.f[f];
+    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
+    const size_t np = (size_t)F.H * F.W;
+    float s = 0.f;
+    for (size_t p = (size_t)li * 256 + threadIdx.x; p < np; p += (size_t)nblk * 256)
+    {
+        float3 o = ld3(F.out, p), r = ld3(F.ref, p);
+        const float *add[2] = {F.emissive, F.env_light}, *radd[2] = {F.ref_emissive, F.ref_env_light};
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+        {
+            if (add[k])
+            {
+                const float3 e = ld3(add[k], p);
+                o.x += e.x; o.y += e.y; o.z += e.z;
+            }
+            if (radd[k])
+            {
+                const float3 e = ld3(radd[k], p);
+                r.x += e.x; r.y += e.y; r.z += e.z;
+            }
+        }
+        if (F.composed) st3(F.composed, p, o);
+        const float m = (r.x + r.y + r.z) / 3.f;
+        const float den = m * m + 1e-2f;
+        const float dx = r.x - o.x, dy = r.y - o.y, dz = r.z - o.z;
+        s += (dx * dx) / den + (dy * dy) / den + (dz * dz) / den;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) F.part[li] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+int frame_error_blocks(int H, int W)
+{
+    const long np = (long)H * W;
+    long nb = (np + 4095) / 4096;  // ~16 px per lane
+    return (int)(nb < 1 ? 1 : (nb > 1024 ? 1024 : nb));
+}
+
+hipError_t launch_frame_error(int n, const ErrFrame *fr, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
+    ErrBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.blk_begin[i] = acc;
+        acc += frame_error_blocks(fr[i].H, fr[i].W);
+    }
+    b.blk_begin[n] = acc;
+    hipLaunchKernelGGL(frame_error_kernel, dim3(acc), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 

@@ -19,4 +19,15 @@ hipError_t launch_adam_tick(int n, int *const *counter, float *const *alpha, flo
 hipError_t launch_sums(int n, const float *const *parts, const int *count, const float *scale, float *const *out,
                        hipStream_t st);
 
+// Frame report (displayed frame and RelL2 partial sums) of one image; see frame_error_kernel.
+struct ErrFrame {
+    const float *out, *emissive, *env_light;          // [H][W][3]; the last two may be null
+    const float *ref, *ref_emissive, *ref_env_light;  // [H][W][3]; the last two may be null
+    float *composed;                                  // [H][W][3] or null
+    float *part;                                      // [frame_error_blocks(H, W)]
+    int H, W;
+};
+int frame_error_blocks(int H, int W);
+hipError_t launch_frame_error(int n, const ErrFrame *fr, hipStream_t st);
+
 }  // namespace xdn

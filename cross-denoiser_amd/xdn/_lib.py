@@ -45,6 +45,8 @@ PROTOTYPES = {
     "xdn_filter_train_workspace_size": (_Z, [_I, _I, _I]),
     "xdn_filter_train": (_I, [_P] * 16 + [_P, _Z] + [_I] * 4 + [_P]),
     "xdn_adam": (_I, [_P] * 4 + [_I, _I, _F, _F, _F, _F, _P]),
+    "xdn_frame_error_workspace_size": (_Z, [_I, _I, _I]),
+    "xdn_frame_error": (_I, [_P] * 8 + [_P, _Z] + [_I] * 3 + [_P]),
     "xdn_cnn5_param_count": (_Z, []),
     "xdn_cnn5_packed_bytes": (_Z, []),
     "xdn_cnn5_workspace_size": (_Z, [_I, _I, _I]),

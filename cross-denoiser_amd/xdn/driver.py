@@ -1,0 +1,92 @@
+"""The reference's frame-driver loop (scripts/main.py:318-359) on the device.
+
+Per frame: the online training step (xdn.pipeline.Denoiser: frame-0 zero history of
+main.py:323-330, the history handoff of :338-341), then the report of :344-347 -- the displayed
+frame out + emissive + envLight, the reference ref + ref_emissive + ref_envLight and their RelL2
+(main.py:191-196) -- in one fused pass (xdn_frame_error), and the reference's log line
+``[Frame %3d] relL2: %.4f (...)``. Writing EXR images is not part of this module.
+
+    python -m xdn.driver --height 1080 --width 1920 --frames 16 [--network unet]
+"""
+from __future__ import annotations
+
+import time
+
+import torch
+
+from . import pipeline
+from ._lib import check, lib, ptr
+
+
+def frame_error(out, ref, emissive=None, env_light=None, ref_emissive=None, ref_env_light=None, composed=None):
+    """RelL2 per image of out + emissive + env_light against ref + ref_emissive + ref_env_light
+    ([B,H,W,3] float32 device tensors; absent terms are zero). Returns err [B] (device) and writes
+    the displayed frame into `composed` when given."""
+    ts = [out, emissive, env_light, ref, ref_emissive, ref_env_light, composed]
+    for t in ts:
+        if t is not None and (t.dtype != torch.float32 or not t.is_contiguous() or t.shape != out.shape):
+            raise ValueError("frame_error: contiguous float32 tensors of the same [B,H,W,3] shape expected")
+    B, H, W, _ = out.shape
+    err = torch.empty(B, device=out.device, dtype=torch.float32)
+    ws_bytes = int(lib.xdn_frame_error_workspace_size(B, H, W))
+    ws = torch.empty(max(ws_bytes, 4), device=out.device, dtype=torch.uint8)
+    check(lib.xdn_frame_error(*[ptr(t) for t in ts], ptr(err), ptr(ws), ws_bytes, B, H, W,
+                              torch.cuda.current_stream(out.device).cuda_stream), "frame_error")
+    return err
+
+
+class FrameDriver:
+    """One stream through the reference's main loop: train step, history handoff, displayed frame
+    and its RelL2 against the reference."""
+
+    def __init__(self, H: int, W: int, device="cuda", network: str = "cnn5", seed: int = 1234,
+                 lr: float = pipeline.LEARNING_RATE, use_graphs: bool = False):
+        self.den = pipeline.Denoiser(H, W, device=device, seed=seed, lr=lr, use_graphs=use_graphs, network=network)
+        self.shown = torch.empty((1, H, W, 3), device=self.den.device, dtype=torch.float32)
+        self.err = torch.zeros(1, device=self.den.device, dtype=torch.float32)
+
+    def step(self, frame: dict) -> dict:
+        """One frame: the step's outputs plus the displayed frame ("shown") and its RelL2 ("relL2",
+        a device tensor [1]). Frames without emissive / envLight terms show the filtered output."""
+        out = self.den.step(frame)
+        g = frame.get
+        e = frame_error(out["out"], frame["ref"], g("emissive"), g("envLight"), g("ref_emissive"), g("ref_envLight"),
+                        composed=self.shown)
+        self.err.copy_(e)
+        return {**out, "shown": self.shown, "relL2": self.err}
+
+    def run(self, frames, log=print, first_index: int = 0):
+        """Runs the frames in order, logging the reference's per-frame line; returns the errors."""
+        errs = []
+        for i, f in enumerate(frames):
+            t0 = time.perf_counter()
+            r = self.step(f)
+            e = float(r["relL2"][0])  # synchronizes: the line reports the frame's wall time
+            ms = (time.perf_counter() - t0) * 1e3
+            errs.append(e)
+            if log is not None:
+                log(f"[Frame {first_index + i:3d}] relL2: {e:.4f} ({ms:.2f} ms)")
+        return errs
+
+
+def main(argv=None):
+    import argparse
+
+    from . import synth
+    ap = argparse.ArgumentParser(description="per-frame online denoising of a synthetic stream (main.py loop)")
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--frames", type=int, default=16)
+    ap.add_argument("--network", choices=("cnn5", "unet"), default="cnn5")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--graphs", action="store_true", help="replay one captured step graph per frame")
+    a = ap.parse_args(argv)
+    dev = torch.device("cuda:0")
+    spec = synth.StreamSpec(a.height, a.width, seed=a.seed, pan=None, pan_max=3.0, disocclusion=0.10)
+    drv = FrameDriver(a.height, a.width, device=dev, network=a.network, use_graphs=a.graphs)
+    frames = (synth.make_frame(spec, t, device=dev, composition=True) for t in range(a.frames))
+    drv.run(frames)
+
+
+if __name__ == "__main__":
+    main()

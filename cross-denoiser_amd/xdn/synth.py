@@ -12,6 +12,10 @@ generator instead (shapes and channel meanings of scripts/main.py:60-66 and util
     linearZ  (Z, fwidth Z, 0) with Z in [1,50] from a smooth field moving with the pan;
              a random `disocclusion` fraction of the pixels gets Z x 2 in each frame
     pnFwidth U[0,0.05]^3;  mvec = (-pan_x/W, -pan_y/H, 0);  opacity = 1
+    with composition=True also the terms the driver adds to the displayed frame and to the
+    reference (main.py:344-345): emissive = one noise-free disc light (radiance 4, radius W/32)
+    moving with the pan, envLight = a smooth sky term where the depth field is far (Z > 45);
+    ref_emissive / ref_envLight equal them (both are noise-free)
 
 Seeds are (seed, stream, frame): scene constants depend on (seed, stream) only, per-frame noise
 on all three. Generation runs on any torch device; CPU generation feeds the oracle tests.
@@ -70,8 +74,9 @@ def scene_pan(spec: StreamSpec) -> tuple:
     return tuple(_scene(spec, "cpu")["pan"])
 
 
-def make_frame(spec: StreamSpec, t: int, device="cpu") -> dict:
-    """Frame t of the stream as a dict of [1,H,W,C] float32 tensors on `device`."""
+def make_frame(spec: StreamSpec, t: int, device="cpu", composition: bool = False) -> dict:
+    """Frame t of the stream as a dict of [1,H,W,C] float32 tensors on `device` (with the
+    emissive / envLight terms of the displayed frame when `composition` is set)."""
     sc = _scene(spec, device)
     H, W = spec.height, spec.width
     dev = torch.device(device)
@@ -113,7 +118,16 @@ def make_frame(spec: StreamSpec, t: int, device="cpu") -> dict:
     mvec[..., 0] = -px / W
     mvec[..., 1] = -py / H
     b = lambda a: a.unsqueeze(0).contiguous()
+    extra = {}
+    if composition:
+        r2 = (X - float(sc["cx"][0])) ** 2 + (Y - float(sc["cy"][0])) ** 2
+        emissive = (r2 <= (W / 32.0) ** 2).to(torch.float64).unsqueeze(-1).expand(H, W, 3) * 4.0
+        sky = (Z > 45.0).to(torch.float64) * (0.5 + 0.25 * torch.sin(two_pi * X / W))
+        env = sky.unsqueeze(-1) * torch.tensor([0.6, 0.8, 1.0], **f64).view(1, 1, 3)
+        extra = {"emissive": b(emissive.float()), "envLight": b(env.float()),
+                 "ref_emissive": b(emissive.float()), "ref_envLight": b(env.float())}
     return {
+        **extra,
         "ref": b(ref32),
         "color": b(ref32 * noise1),
         "color2": b(ref32 * noise2),

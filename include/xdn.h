@@ -160,6 +160,17 @@ int xdn_filter_train(const float *ols1, const float *ols2, const float *albedo, 
 int xdn_adam(float *param, const float *grad, float *m, float *v, int count, int step, float lr, float beta1,
              float beta2, float eps, void *stream);
 
+/* Frame report of the reference's driver loop (scripts/main.py:344-346, RelL2 of :191-196), in
+ * one pass: out_composed = out + emissive + env_light (the displayed frame; may be NULL), the
+ * reference ref + ref_emissive + ref_env_light, and err[b] = mean over pixels and channels of
+ * (ref_c - out_c)^2 / (mean over channels of ref_c)^2 + 1e-2). All images [B,H,W,3]; emissive,
+ * env_light, ref_emissive, ref_env_light may be NULL (zero). The workspace holds per-block
+ * partial sums, xdn_frame_error_workspace_size(B,H,W) bytes. */
+size_t xdn_frame_error_workspace_size(int batch, int height, int width);
+int xdn_frame_error(const float *out, const float *emissive, const float *env_light, const float *ref,
+                    const float *ref_emissive, const float *ref_env_light, float *out_composed, float *err,
+                    void *workspace, size_t workspace_bytes, int batch, int height, int width, void *stream);
+
 /* ---------------------------------------------------------------------------------------------
  * cnn5: the bandwidth network of the benchmark configuration (the reference's online network is
  * the Keras U-Net of scripts/main.py:83-118, trained per frame at :289-303): five 3x3 'same'

@@ -337,3 +337,16 @@ def band_alpha_grad(param, gband, galpha):
     gx = np.asarray(gband, np.float64) * (-2.0 * x * band * band)
     ga = np.asarray(galpha, np.float64) * alpha * (1 - alpha)
     return np.concatenate([gx, ga], axis=-1)
+
+
+def frame_report(out, ref, emissive=None, env_light=None, ref_emissive=None, ref_env_light=None):
+    """Displayed frame and its RelL2 error of the driver loop (scripts/main.py:344-346), with RelL2
+    as defined at main.py:191-196: per element (y_true - y_pred)^2 / (mean_c(y_true)^2 + 1e-2),
+    averaged over the image. Returns (out + emissive + env_light [B,H,W,3], err [B]) in float64;
+    absent terms are zero."""
+    d = lambda a: 0.0 if a is None else np.asarray(a, np.float64)
+    o = np.asarray(out, np.float64) + d(emissive) + d(env_light)
+    r = np.asarray(ref, np.float64) + d(ref_emissive) + d(ref_env_light)
+    m = r.mean(axis=-1, keepdims=True)
+    e = np.square(r - o) / (np.square(m) + 1e-2)
+    return o, e.reshape(e.shape[0], -1).mean(axis=1)

@@ -555,3 +555,26 @@ def test_band_alpha_head_grad():
         xm[0, i, j, k] -= 1e-6
         L = lambda z: float((O.band_alpha(z)[0] * gb).sum() + (O.band_alpha(z)[1] * ga).sum())
         assert g[0, i, j, k] == pytest.approx((L(xp) - L(xm)) / 2e-6, rel=1e-5)
+
+
+def test_frame_report_known_answers():
+    """RelL2 of the driver report (scripts/main.py:191-196, 344-346) against closed forms."""
+    rng = np.random.default_rng(5)
+    ref = rng.uniform(0, 2, (2, 5, 7, 3)).astype(np.float32)
+    em, env = rng.uniform(0, 1, ref.shape).astype(np.float32), rng.uniform(0, 1, ref.shape).astype(np.float32)
+    # prediction equal to the composed reference: zero error, and the displayed frame is out + em + env
+    comp, err = O.frame_report(ref, ref, em, env, em, env)
+    assert np.array_equal(err, np.zeros(2))
+    assert np.allclose(comp, ref.astype(np.float64) + em + env, rtol=0, atol=0)
+    # zero prediction against a constant grey c: c^2 / (c^2 + 1e-2) per element
+    c = 0.3
+    _, err = O.frame_report(np.zeros((1, 4, 4, 3)), np.full((1, 4, 4, 3), c))
+    assert np.allclose(err, c * c / (c * c + 1e-2), rtol=1e-15)
+    # the normaliser is the channel mean of the reference at that pixel: ref (1, 2, 3) -> mean 2
+    ref1 = np.array([1.0, 2.0, 3.0]).reshape(1, 1, 1, 3)
+    _, err = O.frame_report(np.zeros_like(ref1), ref1)
+    assert np.isclose(err[0], (1 + 4 + 9) / 3 / (4 + 1e-2), rtol=1e-15)
+    # a per-image mean: two images with different errors do not mix
+    out2 = np.stack([np.zeros((3, 3, 3)), np.full((3, 3, 3), c)])[..., :]
+    _, err = O.frame_report(out2, np.full((2, 3, 3, 3), c))
+    assert np.allclose(err, [c * c / (c * c + 1e-2), 0.0], rtol=1e-15)

@@ -205,3 +205,43 @@ def test_unet_step_uses_the_unet_ops(dev):
                 frames[1]["normal"])
     torch.cuda.synchronize()
     assert torch.equal(net.out, den.buffer("param", 6))
+
+
+def test_frame_error_matches_oracle(dev):
+    """xdn_frame_error (the driver's displayed frame and RelL2, main.py:191-196,344-346) against the
+    float64 oracle, with and without the emissive / envLight terms, batch 2, a ragged size."""
+    from xdn import driver
+    rng = np.random.default_rng(17)
+    B, H, W = 2, 37, 45
+    mk = lambda lo, hi: rng.uniform(lo, hi, (B, H, W, 3)).astype(np.float32)
+    out, ref, em, env, rem, renv = mk(0, 4), mk(0, 4), mk(0, 1), mk(0, 1), mk(0, 1), mk(0, 1)
+    g = lambda a: torch.from_numpy(a).to(dev)
+    shown = torch.empty_like(g(out))
+    err = driver.frame_error(g(out), g(ref), g(em), g(env), g(rem), g(renv), composed=shown)
+    comp_ref, err_ref = O.frame_report(out, ref, em, env, rem, renv)
+    assert rel_l2(err.cpu().numpy(), err_ref) <= 1e-5
+    assert rel_l2(shown.cpu().numpy(), comp_ref) <= 1e-6
+    err2 = driver.frame_error(g(out), g(ref))
+    assert rel_l2(err2.cpu().numpy(), O.frame_report(out, ref)[1]) <= 1e-5
+
+
+def test_driver_reports_each_frame(dev):
+    """The driver loop on a synthetic stream with emissive and sky terms: one log line per frame in
+    the reference's format, each error equal to the oracle's on the frame's own outputs."""
+    from xdn import driver, synth
+    H, W = 64, 96
+    spec = synth.StreamSpec(H, W, seed=8, pan=(1.0, 0.5), disocclusion=0.1)
+    frames = [synth.make_frame(spec, t, device=dev, composition=True) for t in range(4)]
+    assert float(frames[0]["emissive"].max()) == 4.0 and float(frames[0]["envLight"].max()) > 0
+    drv = driver.FrameDriver(H, W, device=dev)
+    lines = []
+    for t, f in enumerate(frames):
+        r = drv.step(f)
+        torch.cuda.synchronize()
+        n = lambda k: f[k].cpu().numpy()
+        comp, err = O.frame_report(r["out"].cpu().numpy(), n("ref"), n("emissive"), n("envLight"),
+                                   n("ref_emissive"), n("ref_envLight"))
+        assert abs(float(r["relL2"][0]) - err[0]) <= 1e-5 * err[0]
+        assert rel_l2(r["shown"].cpu().numpy(), comp) <= 1e-6
+    errs = driver.FrameDriver(H, W, device=dev).run(frames, log=lines.append)
+    assert len(lines) == 4 and lines[0].startswith("[Frame   0] relL2: ") and len(errs) == 4
