@@ -7,6 +7,7 @@ frame out + emissive + envLight, the reference ref + ref_emissive + ref_envLight
 ``[Frame %3d] relL2: %.4f (...)``. Writing EXR images is not part of this module.
 
     python -m xdn.driver --height 1080 --width 1920 --frames 16 [--network unet]
+    python -m xdn.driver --scene <dir of {type}_{frame:04d}.npy> [--frames 101]
 """
 from __future__ import annotations
 
@@ -80,8 +81,19 @@ def main(argv=None):
     ap.add_argument("--network", choices=("cnn5", "unet"), default="cnn5")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--graphs", action="store_true", help="replay one captured step graph per frame")
+    ap.add_argument("--scene", type=str, default=None,
+                    help="directory of {type}_{frame:04d}.npy buffers (the reference's converted dataset) "
+                         "instead of the synthetic stream")
     a = ap.parse_args(argv)
     dev = torch.device("cuda:0")
+    if a.scene:
+        from . import data
+        scene = data.NpyScene(a.scene, max_num_frames=a.frames, device=dev)
+        first_id, first = scene[0]
+        H, W = first["color"].shape[1:3]
+        drv = FrameDriver(H, W, device=dev, network=a.network, use_graphs=a.graphs)
+        drv.run((f for _, f in scene), first_index=first_id)
+        return
     spec = synth.StreamSpec(a.height, a.width, seed=a.seed, pan=None, pan_max=3.0, disocclusion=0.10)
     drv = FrameDriver(a.height, a.width, device=dev, network=a.network, use_graphs=a.graphs)
     frames = (synth.make_frame(spec, t, device=dev, composition=True) for t in range(a.frames))
