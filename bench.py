@@ -51,7 +51,23 @@ def parse():
                     help="overlap frame t+1's cross-regression with frame t on a second HIP stream")
     ap.add_argument("--no-unet", action="store_true",
                     help="skip the secondary measurement of the step with the reference's U-Net")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three plain 1 GiB device copies (the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profile of this script) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(torch):
+    """Three device copies of 256 Mi float32 (1 GiB each way): what FETCH_SIZE / WRITE_SIZE report
+    for them gives the factors that turn those counters into bytes."""
+    n = 256 << 20
+    a = torch.empty(n, device="cuda", dtype=torch.float32).fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied 1 GiB x3", flush=True)
 
 
 def unet_leg(frames, H, W, steps, warmup, reps, torch):
@@ -240,6 +256,9 @@ def main():
     import numpy as np
     import torch
 
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     from xdn import pipeline, synth
 
     dev = torch.device("cuda:0")
