@@ -201,8 +201,10 @@ def profiled_traffic(kernel: str, H: int, W: int):
     return None
 
 
-def cpu_baseline(torch, H, W):
-    """Oracle (FP64, all host threads) + torch-CPU CNN step on a 256x256 crop, scaled to 1080p."""
+def cpu_baseline(torch, H, W, repeats=5):
+    """The FP64 oracle's filtering path + a torch-CPU FP32 CNN step on a 256x256 crop of the same
+    workload, per stage, median of `repeats` runs, on all host threads and on one; scaled by pixel
+    count to H x W."""
     import numpy as np
 
     from oracle import oracle as O
@@ -212,39 +214,68 @@ def cpu_baseline(torch, H, W):
     spec = synth.StreamSpec(h, w, seed=1, pan=(1.0, -1.0), disocclusion=0.1)
     f0, f1 = synth.make_frame(spec, 0), synth.make_frame(spec, 1)
     n = lambda t: t.numpy()
-    t0 = time.perf_counter()
-    y1, y2 = O.prepare_input(n(f1["color"])).astype(np.float32), O.prepare_input(n(f1["color2"])).astype(np.float32)
-    o1, o2, _, _ = O.cross_regression(y1, y2, n(f1["albedo"]), n(f1["normal"]))
-    o1, o2 = o1.astype(np.float32), o2.astype(np.float32)
-    s, rp = O.reproject(n(f1["mvec"]), [o1, o1, o2], n(f1["linearZ"]), n(f0["linearZ"]), n(f1["normal"]),
-                        n(f0["normal"]), n(f1["pnFwidth"]), n(f1["opacity"]), n(f1["opacity"]))
-    se = O.erosion(s)
-    pv, p1, p2, sp = O.temporal_screening(o1, o2, *[r.astype(np.float32) for r in rp], se)
     band = np.ones((1, h, w, 5), np.float32)
-    acc = O.spatial_filter(o1, o2, n(f1["albedo"]), n(f1["normal"]), band)
-    O.spatial_filter_grad(acc[0].astype(np.float32), acc[1].astype(np.float32), acc[2].astype(np.float32),
-                          acc[3].astype(np.float32), o1, o2, n(f1["albedo"]), n(f1["normal"]), band)
-    t_filter = time.perf_counter() - t0
-    # CNN forward + backward in FP32 on the CPU
     params = [p.clone().requires_grad_(True) for wb in cnn_mod.init_params() for p in
               (wb[0].permute(3, 2, 0, 1).contiguous(), wb[1])]
     x = torch.randn(1, 15, h, w)
-    t0 = time.perf_counter()
-    hcur = x
-    for i in range(5):
-        hcur = torch.nn.functional.conv2d(hcur, params[2 * i], params[2 * i + 1], padding=1)
-        if i < 4:
-            hcur = torch.relu(hcur)
-    hcur.sum().backward()
-    t_cnn = time.perf_counter() - t0
+    st = {}
+
+    def cr():
+        y1 = O.prepare_input(n(f1["color"])).astype(np.float32)
+        y2 = O.prepare_input(n(f1["color2"])).astype(np.float32)
+        o1, o2, _, _ = O.cross_regression(y1, y2, n(f1["albedo"]), n(f1["normal"]))
+        st["o1"], st["o2"] = o1.astype(np.float32), o2.astype(np.float32)
+
+    def rp():
+        s, r = O.reproject(n(f1["mvec"]), [st["o1"], st["o1"], st["o2"]], n(f1["linearZ"]), n(f0["linearZ"]),
+                           n(f1["normal"]), n(f0["normal"]), n(f1["pnFwidth"]), n(f1["opacity"]), n(f1["opacity"]))
+        st["s"], st["rp"] = s, [a.astype(np.float32) for a in r]
+
+    def ts():
+        O.temporal_screening(st["o1"], st["o2"], *st["rp"], O.erosion(st["s"]))
+
+    def sf():
+        st["acc"] = [a.astype(np.float32) for a in O.spatial_filter(st["o1"], st["o2"], n(f1["albedo"]),
+                                                                     n(f1["normal"]), band)]
+
+    def sfg():
+        O.spatial_filter_grad(*st["acc"], st["o1"], st["o2"], n(f1["albedo"]), n(f1["normal"]), band)
+
+    def cnn():
+        hcur = x
+        for i in range(5):
+            hcur = torch.nn.functional.conv2d(hcur, params[2 * i], params[2 * i + 1], padding=1)
+            if i < 4:
+                hcur = torch.relu(hcur)
+        hcur.sum().backward()
+
+    stages = (("cross_regression", cr), ("reproject", rp), ("erosion_screening", ts), ("spatial_filter", sf),
+              ("spatial_filter_grad", sfg), ("cnn_fwd_bwd", cnn))
     scale = (H * W) / (h * w)
-    per_frame = (t_filter + t_cnn) * scale
-    return {"value": round(1.0 / per_frame, 4), "unit": "frames/s", "cores": int(torch.get_num_threads()),
-            "kind": "port",
-            "sample": f"one {h}x{w} frame: FP64 oracle filtering path (cross-regression, reprojection, erosion, "
-                      f"screening, spatial filter + band gradient; {os.cpu_count()} host threads) + torch-CPU FP32 "
-                      f"CNN fwd+bwd; time scaled by pixel count to {W}x{H}",
-            "filter_s": round(t_filter, 3), "cnn_s": round(t_cnn, 3)}
+    all_threads = O.threads()
+    torch_threads = torch.get_num_threads()
+    res = {}
+    for label, nt in (("all", all_threads), ("one", 1)):
+        O.set_threads(nt)
+        torch.set_num_threads(nt)
+        times = {k: [] for k, _ in stages}
+        for _ in range(repeats):
+            for k, fn in stages:
+                t0 = time.perf_counter()
+                fn()
+                times[k].append(time.perf_counter() - t0)
+        res[label] = {k: float(np.median(v)) * scale for k, v in times.items()}
+    O.set_threads(all_threads)
+    torch.set_num_threads(torch_threads)
+    per_frame = sum(res["all"].values())
+    return {"value": round(1.0 / per_frame, 4), "unit": "frames/s", "cores": int(all_threads), "kind": "port",
+            "sample": f"one {h}x{w} frame of the same workload, each stage the median of {repeats} runs: FP64 oracle "
+                      f"filtering path (cross-regression, reprojection, erosion + screening, spatial filter, band "
+                      f"gradient; OpenMP) + torch-CPU FP32 cnn5 forward+backward; times scaled by pixel count to "
+                      f"{W}x{H}",
+            "stage_ms_all_threads": {k: round(v * 1e3, 1) for k, v in res["all"].items()},
+            "single_thread": {"value": round(1.0 / sum(res["one"].values()), 5), "unit": "frames/s", "cores": 1,
+                              "stage_ms": {k: round(v * 1e3, 1) for k, v in res["one"].items()}}}
 
 
 def main():

@@ -57,8 +57,19 @@ def lib():
             ctypes.POINTER(ctypes.c_void_p), _i, _i]
         L.orc_erosion.argtypes = [_u8p, ctypes.c_void_p, _i, _i]
         L.orc_screening.argtypes = [_f32p] * 6 + [ctypes.c_void_p] * 4 + [_i] * 4
+        L.orc_set_threads.argtypes = [_i]
+        L.orc_get_threads.restype = _i
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Host threads of the C oracle's OpenMP loops."""
+    lib().orc_set_threads(int(n))
+
+
+def threads() -> int:
+    return int(lib().orc_get_threads())
 
 
 def _f32(a):

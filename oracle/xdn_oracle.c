@@ -24,6 +24,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <limits.h>
+#include <omp.h>
 
 #define P6 6 /* COEFFI_DIM_PILOT, regression.cu.cc:30 */
 
@@ -837,3 +838,7 @@ void orc_screening(const float *c1, const float *c2, const float *pv, const floa
             osucc[c] = keep[0] ? 1.f : 0.f;
         }
 }
+
+/* Host threads the OpenMP loops above use (for timing the oracle as a CPU baseline). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
