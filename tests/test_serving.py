@@ -1,6 +1,7 @@
 """Serving loop: admission by HBM budget with idle eviction (host logic, CPU) and an end-to-end run
 of the continuous-batching loop on the GPU."""
 import numpy as np
+import torch
 import pytest
 
 from xdn.serving import Admission, StreamReq, config3, config4, onoff_arrivals, poisson_arrivals
@@ -61,3 +62,20 @@ def test_serving_loop_runs_all_frames(dev):
     srv2 = Server(dev, hbm_budget=b2, pool_frames=2, idle_after=0.0)
     res2 = srv2.run(reqs)
     assert res2["frames"] == 24 and res2["evictions"] > 0
+
+
+@pytest.mark.gpu
+def test_serving_loop_with_unet_streams_under_eviction(dev):
+    """U-Net streams: batched serving with a two-slot budget evicts and readmits (network and Adam
+    state go to the host and back) and serves every frame with finite outputs."""
+    from xdn import pipeline
+    from xdn.serving import Server
+    rng = np.random.default_rng(2)
+    reqs = [StreamReq(i, h, w, 5, poisson_arrivals(150.0, 5, rng=rng), seed=i)
+            for i, (h, w) in enumerate([(64, 64), (48, 96), (64, 64)])]
+    b2 = 2 * pipeline.slot_bytes(64, 64, "unet") + 1
+    srv = Server(dev, hbm_budget=b2, pool_frames=2, idle_after=0.0, network="unet")
+    res = srv.run(reqs)
+    assert res["frames"] == 15 and res["evictions"] > 0 and np.isfinite(res["p99_ms"])
+    for d in srv.denoisers.values():
+        assert d.network == "unet" and torch.isfinite(d.params).all()
