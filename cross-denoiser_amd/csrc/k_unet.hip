@@ -263,9 +263,14 @@ constexpr int UX_C = US_W + 2, UX_R = US_H + 2;   // staged X: 10 rows x 34 px (
 template <int SRC, int CA, int CB, int CO, int MT>
 __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
 {
-    constexpr int XS = 32, DS = 32;  // staged channels (zero-padded)
+    // input-slice channels of this instantiation; staged channel counts (pixel strides) 16 or 32,
+    // zero-padded: with 16 the lane groups 0 and 1 read the same channels, so the accumulator rows
+    // (or columns) 16-31 repeat 0-15 and are not written out
+    constexpr int CS = SRC == UC_SRC_IMAGES ? 15 : ((CA + CB - 32 * MT) < 32 ? (CA + CB - 32 * MT) : 32);
+    constexpr int XS = CS <= 16 ? 16 : 32, DS = CO <= 16 ? 16 : 32;
     constexpr int X_ELEMS = UX_R * UX_C * XS, D_ELEMS = US_H * US_W * DS;
-    __shared__ __attribute__((aligned(16))) uint16_t lds[X_ELEMS + D_ELEMS];
+    constexpr int LDS_ELEMS = (X_ELEMS + D_ELEMS) > 8192 ? (X_ELEMS + D_ELEMS) : 8192;  // >= 4 x 1024 floats
+    __shared__ __attribute__((aligned(16))) uint16_t lds[LDS_ELEMS];
     __shared__ float dbs[4][32];
     uint16_t *xs = lds, *ds = lds + X_ELEMS;
     int f = 0;
@@ -277,10 +282,11 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, h = lane >> 5, ch = lane & 31;
     const int rr = 2 * wv + h;
     const int grp = (lane >> 4) & 1, s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3;
-    const int cb = 16 * grp + 4 * qq;
+    const int cbx = (XS > 16 ? 16 * grp : 0) + 4 * qq, cbd = (DS > 16 ? 16 * grp : 0) + 4 * qq;
 
-    constexpr int NX = SRC == UC_SRC_BF16 ? (UX_R * UX_C * 4 + 255) / 256 : (UX_R * UX_C + 255) / 256;
-    constexpr int ND = US_H * US_W * 4 / 256;
+    constexpr int QX = XS / 8, QD = DS / 8;  // 16-byte chunks per staged pixel
+    constexpr int NX = SRC == UC_SRC_BF16 ? (UX_R * UX_C * QX + 255) / 256 : (UX_R * UX_C + 255) / 256;
+    constexpr int ND = US_H * US_W * QD / 256;
     u32x4 rx[SRC == UC_SRC_BF16 ? NX : 1], rd[ND];
     float fx[SRC == UC_SRC_BF16 ? 1 : NX][15];
     auto fetch = [&](int s) {
@@ -290,12 +296,12 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
         {
             if constexpr (SRC == UC_SRC_BF16)
             {
-                const int i = tid + 256 * u, pix = i >> 2, q = i & 3;
+                const int i = tid + 256 * u, pix = i / QX, q = i % QX;
                 const int r = pix / UX_C, col = pix - r * UX_C;
                 const int gy = y0 - 1 + r, gx = x0 - 1 + col;
                 const int g = 32 * MT + 8 * q;  // first concat channel of this chunk
                 rx[u] = u32x4{0u, 0u, 0u, 0u};
-                if (i < UX_R * UX_C * 4 && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                if (i < UX_R * UX_C * QX && gy >= 0 && gy < H && gx >= 0 && gx < W)
                 {
                     const size_t p = (size_t)gy * W + gx;
                     if (g < CA)
@@ -326,7 +332,7 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
 #pragma unroll
         for (int u = 0; u < ND; ++u)
         {
-            const int i = tid + 256 * u, pix = i >> 2, q = i & 3;
+            const int i = tid + 256 * u, pix = i / QD, q = i % QD;
             const int r = pix / US_W, col = pix - r * US_W;
             const int gy = y0 + r, gx = x0 + col;
             rd[u] = u32x4{0u, 0u, 0u, 0u};
@@ -341,7 +347,7 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
             if constexpr (SRC == UC_SRC_BF16)
             {
                 const int i = tid + 256 * u;
-                if (i < UX_R * UX_C * 4) *reinterpret_cast<u32x4 *>(&xs[(i >> 2) * XS + (i & 3) * 8]) = rx[u];
+                if (i < UX_R * UX_C * QX) *reinterpret_cast<u32x4 *>(&xs[(i / QX) * XS + (i % QX) * 8]) = rx[u];
             }
             else
             {
@@ -355,8 +361,11 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
                     c.z = u_pack2(fx[u][12], fx[u][13]); c.w = u_pack2(fx[u][14], 0.f);
                     *reinterpret_cast<u32x4 *>(&xs[pix * XS]) = a;
                     *reinterpret_cast<u32x4 *>(&xs[pix * XS + 8]) = c;
-                    *reinterpret_cast<u32x4 *>(&xs[pix * XS + 16]) = z;
-                    *reinterpret_cast<u32x4 *>(&xs[pix * XS + 24]) = z;
+                    if constexpr (XS > 16)
+                    {
+                        *reinterpret_cast<u32x4 *>(&xs[pix * XS + 16]) = z;
+                        *reinterpret_cast<u32x4 *>(&xs[pix * XS + 24]) = z;
+                    }
                 }
             }
         }
@@ -364,7 +373,7 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
         for (int u = 0; u < ND; ++u)
         {
             const int i = tid + 256 * u;
-            *reinterpret_cast<u32x4 *>(&ds[(i >> 2) * DS + (i & 3) * 8]) = rd[u];
+            *reinterpret_cast<u32x4 *>(&ds[(i / QD) * DS + (i % QD) * 8]) = rd[u];
         }
     };
 
@@ -385,7 +394,7 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {
-            const uint16_t *dp = &ds[(rr * US_W + 8 * c + jq) * DS + cb];
+            const uint16_t *dp = &ds[(rr * US_W + 8 * c + jq) * DS + cbd];
             const uint2 b0 = u_ld_tr(dp), b1 = u_ld_tr(dp + 4 * DS);
             const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
             db += (u_lo(B.x) + u_hi(B.x)) + (u_lo(B.y) + u_hi(B.y)) + (u_lo(B.z) + u_hi(B.z)) + (u_lo(B.w) + u_hi(B.w));
@@ -394,14 +403,13 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
 #pragma unroll
                 for (int sh = 0; sh < 3; ++sh)
                 {
-                    const uint16_t *xp = &xs[((rr + dy) * UX_C + 8 * c + sh + jq) * XS + cb];
+                    const uint16_t *xp = &xs[((rr + dy) * UX_C + 8 * c + sh + jq) * XS + cbx];
                     const uint2 a0 = u_ld_tr(xp), a1 = u_ld_tr(xp + 4 * XS);
                     acc[dy * 3 + sh] = u_mfma(u32x4{a0.x, a0.y, a1.x, a1.y}, B, acc[dy * 3 + sh]);
                 }
         }
     }
     // compact partial [9][CS][CO] + [CO] of this input slice
-    constexpr int CS = SRC == UC_SRC_IMAGES ? 15 : ((CA + CB - 32 * MT) < 32 ? (CA + CB - 32 * MT) : 32);
     float *out = F.part + (size_t)li * compact_part(9, CS, CO);
     float *red = reinterpret_cast<float *>(lds);  // [4 waves][1024]
     static_assert(sizeof(lds) >= 4 * 1024 * sizeof(float), "reduction buffer");
