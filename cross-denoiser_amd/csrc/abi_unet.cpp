@@ -37,6 +37,11 @@ int conv(Frames F, int n, hipStream_t st, int kind, int l, int a, int b, int out
         f.mask = mask >= 0 ? reinterpret_cast<const uint16_t *>(U.buf[mask]) : nullptr;
         f.wfrag = U.pk + (dgrad ? kULay.dg_off[l] : kULay.fwd_off[l]);
         f.bias = reinterpret_cast<const float *>(U.pk + kULay.bias_off[l]);
+        if (kind == xdn::UC_F_24_8_HEAD)
+        {
+            f.head_w = reinterpret_cast<const float *>(U.pk + kULay.master_off[11]);
+            f.head_out = U.param;
+        }
         f.H = U.h[lv];
         f.W = U.w[lv];
     }
@@ -140,7 +145,8 @@ int pool_bwd(Frames F, int n, hipStream_t st, int C, int dp, int act, int skip, 
     return hip_check(xdn::launch_pool_bwd(C, n, fr.data(), st), "unet max pool backward");
 }
 
-int head(Frames F, int n, hipStream_t st, bool bwd)
+// head backward: dY10 = ReLU'(A10) * W dparam, partial set 12 (the forward runs in c5_1's epilogue)
+int head_bwd(Frames F, int n, hipStream_t st)
 {
     std::vector<xdn::HeadFrame> fr(n);
     for (int i = 0; i < n; ++i)
@@ -149,7 +155,6 @@ int head(Frames F, int n, hipStream_t st, bool bwd)
         xdn::HeadFrame &f = fr[i];
         memset(&f, 0, sizeof(f));
         f.in = reinterpret_cast<const uint16_t *>(U.buf[UB_A10]);
-        f.param = U.param;
         f.dparam = U.dparam;
         f.din = reinterpret_cast<uint16_t *>(U.buf[UB_DY10]);
         f.w = master(U, 11);
@@ -159,8 +164,7 @@ int head(Frames F, int n, hipStream_t st, bool bwd)
         f.W = U.w[0];
         f.nblk = U.nblk[12];
     }
-    return bwd ? hip_check(xdn::launch_head_bwd(n, fr.data(), st), "unet head backward")
-               : hip_check(xdn::launch_head(n, fr.data(), st), "unet head");
+    return hip_check(xdn::launch_head_bwd(n, fr.data(), st), "unet head backward");
 }
 
 }  // namespace
@@ -201,15 +205,15 @@ int unet_forward_frames(const UnetFrame *F, int n, hipStream_t st)
     if ((e = conv(F, n, st, UC_F_48_16, 7, UB_A6, UB_A3, UB_A7, -1, false, "unet c4_1"))) return e;  // [c3_3, c2_2]
     if ((e = conv(F, n, st, UC_F_16_16, 8, UB_A7, -1, UB_A8, -1, false, "unet c4_2"))) return e;
     if ((e = convT(F, n, st, 9, UB_A8, UB_A9))) return e;                                       // c4_3
-    if ((e = conv(F, n, st, UC_F_24_8, 10, UB_A9, UB_A1, UB_A10, -1, false, "unet c5_1"))) return e;  // [c4_3, c1_2]
-    return head(F, n, st, false);
+    // c5_1 over [c4_3, c1_2], with the 1x1 head applied in its epilogue
+    return conv(F, n, st, UC_F_24_8_HEAD, 10, UB_A9, UB_A1, UB_A10, -1, false, "unet c5_1 + head");
 }
 
 int unet_backward_frames(const UnetFrame *F, int n, hipStream_t st)
 {
     using namespace xdn;
     int e = 0;
-    if ((e = head(F, n, st, true))) return e;                                    // dY10, set 12
+    if ((e = head_bwd(F, n, st))) return e;                                      // dY10, set 12
     if ((e = wgrad(F, n, st, UW_24_8, 11, UB_A9, UB_A1, UB_DY10))) return e;     // c5_1
     if ((e = conv(F, n, st, UC_D_8_24, 10, UB_DY10, -1, UB_DC5, -1, true, "unet c5_1 dgrad"))) return e;
     if ((e = convT_bwd(F, n, st, 9, 10, UB_A8, UB_DC5, UB_DY8))) return e;       // c4_3 <- dC5[:, 0:16]

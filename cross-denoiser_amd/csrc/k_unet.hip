@@ -12,7 +12,8 @@
 //   maxpool2 / pool_bwd          2x2 max pool and its gradient (to the first maximum, plus the
 //     skip-connection gradient, ReLU-masked).
 //   convT1x1s2 / convT_bwd       transposed 1x1 stride-2 convolution and its gradients.
-//   head1x1 / head_bwd           the 1x1 8->6 output convolution and its gradients.
+//   head_bwd                     gradients of the 1x1 8->6 output convolution (its forward runs in
+//                                the epilogue of the last 3x3 layer, uconv3x3<..., HEAD>).
 //   upack                        FP32 master weights -> bf16 fragments.
 #include "xdn_common.h"
 #include "xdn_unet.h"
@@ -66,9 +67,10 @@ static_assert(sizeof(UcBatch) <= 4096, "kernel argument too large");
 
 constexpr int UT_W = 32, UT_H = 16;
 
-template <int SRC, int CA, int CB, int CO, int EPI>
+template <int SRC, int CA, int CB, int CO, int EPI, bool HEAD = false>
 __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
 {
+    static_assert(!HEAD || (CO == 8 && EPI == UC_EPI_RELU), "the head follows the 8-channel ReLU layer");
     constexpr int CI = SRC == UC_SRC_IMAGES ? 15 : CA + CB;
     constexpr int KC = (CI + 15) / 16, CIP = KC * 16, PS = CIP + 8;
     constexpr int NT = (CO + 31) / 32;
@@ -201,11 +203,26 @@ __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
                 v.x = u_keep(m.x, v.x); v.y = u_keep(m.y, v.y); v.z = u_keep(m.z, v.z); v.w = u_keep(m.w, v.w);
             }
             *reinterpret_cast<u32x4 *>(static_cast<uint16_t *>(F.out) + o) = v;
+            if constexpr (HEAD)
+            {
+                // the head on the stored bf16 activation: bias, then the 8 inputs in order
+                const float in[8] = {u_lo(v.x), u_hi(v.x), u_lo(v.y), u_hi(v.y),
+                                     u_lo(v.z), u_hi(v.z), u_lo(v.w), u_hi(v.w)};
+                float *po = F.head_out + ((size_t)gy * W + gx) * 6;
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                {
+                    float sum = F.head_w[48 + k];
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) sum += in[c] * F.head_w[c * 6 + k];
+                    po[k] = sum;
+                }
+            }
         }
     }
 }
 
-template <int SRC, int CA, int CB, int CO, int EPI>
+template <int SRC, int CA, int CB, int CO, int EPI, bool HEAD = false>
 static hipError_t uconv_t(int n, const UcFrame *fr, hipStream_t st)
 {
     UcBatch b{};
@@ -218,7 +235,7 @@ static hipError_t uconv_t(int n, const UcFrame *fr, hipStream_t st)
     }
     b.map = make_tilemap(n, tx, ty);
     if (b.map.total == 0) return hipSuccess;
-    hipLaunchKernelGGL((uconv3x3<SRC, CA, CB, CO, EPI>), dim3(b.map.total), dim3(256), 0, st, b);
+    hipLaunchKernelGGL((uconv3x3<SRC, CA, CB, CO, EPI, HEAD>), dim3(b.map.total), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 
@@ -243,6 +260,7 @@ hipError_t launch_uconv(int kind, int n, const UcFrame *fr, hipStream_t st)
     case UC_D_32_32: return uconv_t<UC_SRC_BF16, 32, 0, 32, UC_EPI_MASK>(n, fr, st);
     case UC_D_16_48: return uconv_t<UC_SRC_BF16, 16, 0, 48, UC_EPI_PLAIN>(n, fr, st);
     case UC_D_8_24: return uconv_t<UC_SRC_BF16, 8, 0, 24, UC_EPI_PLAIN>(n, fr, st);
+    case UC_F_24_8_HEAD: return uconv_t<UC_SRC_BF16, 16, 8, 8, UC_EPI_RELU, true>(n, fr, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -884,7 +902,7 @@ hipError_t launch_convT_bwd(int C, int n, const CtFrame *fr, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Head: 1x1 convolution 8 -> 6
+// Head backward: 1x1 convolution 8 -> 6
 // ---------------------------------------------------------------------------------------------
 struct HeadBatch {
     int n;
@@ -892,25 +910,6 @@ struct HeadBatch {
     HeadFrame f[XDN_MAX_FRAMES];
 };
 static_assert(sizeof(HeadBatch) <= 4096, "kernel argument too large");
-
-__global__ __launch_bounds__(256) void head1x1(HeadBatch b)
-{
-    int f = 0;
-    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
-    const HeadFrame &F = b.f[f];
-    const int p = (blockIdx.x - b.blk_begin[f]) * 256 + threadIdx.x;
-    if (p >= F.H * F.W) return;
-    const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * 8);
-    const float in[8] = {u_lo(v.x), u_hi(v.x), u_lo(v.y), u_hi(v.y), u_lo(v.z), u_hi(v.z), u_lo(v.w), u_hi(v.w)};
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-    {
-        float s = F.b[k];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) s += in[c] * F.w[c * 6 + k];
-        F.param[(size_t)p * 6 + k] = s;
-    }
-}
 
 __global__ __launch_bounds__(256) void head_bwd(HeadBatch b)
 {
@@ -967,24 +966,6 @@ __global__ __launch_bounds__(256) void head_bwd(HeadBatch b)
     __syncthreads();
     float *out = F.part + (size_t)li * compact_part(1, 8, 6);  // [8][6] then [6]
     if (tid < 54) out[tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
-}
-
-hipError_t launch_head(int n, const HeadFrame *fr, hipStream_t st)
-{
-    if (n <= 0) return hipSuccess;
-    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
-    HeadBatch b{};
-    b.n = n;
-    int acc = 0;
-    for (int i = 0; i < n; ++i)
-    {
-        b.f[i] = fr[i];
-        b.blk_begin[i] = acc;
-        acc += udiv_up(fr[i].H * fr[i].W, 256);
-    }
-    b.blk_begin[n] = acc;
-    hipLaunchKernelGGL(head1x1, dim3(acc), dim3(256), 0, st, b);
-    return hipGetLastError();
 }
 
 hipError_t launch_head_bwd(int n, const HeadFrame *fr, hipStream_t st)

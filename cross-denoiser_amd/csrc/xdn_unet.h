@@ -38,6 +38,8 @@ struct UcFrame {
     const uint16_t *mask;       // bf16 [H][W][CO] activation whose ReLU masks the output (MASK)
     const void *wfrag;          // 16-byte B fragments [9][KC][NT][64]
     const float *bias;          // [NT * 32] (RELU)
+    const float *head_w;        // UC_F_24_8_HEAD: the 1x1 head, FP32 [8][6] then [6]
+    float *head_out;            // UC_F_24_8_HEAD: its output, FP32 [H][W][6]
     int H, W;
 };
 // kind = index into the instantiated (SRC, CA, CB, CO, EPI) table of k_unet.hip
@@ -58,6 +60,7 @@ enum {
     UC_D_32_32,   // dgrad dY 32 -> dX 32, masked
     UC_D_16_48,   // dgrad dY 16 -> dX 48, unmasked (concat input)
     UC_D_8_24,    // dgrad dY 8 -> dX 24, unmasked (concat input)
+    UC_F_24_8_HEAD,  // UC_F_24_8 with the 1x1 head 8 -> 6 (FP32) applied to its bf16 output
     UC_KINDS
 };
 
@@ -120,18 +123,16 @@ struct CtFrame {
 hipError_t launch_convT(int C, int n, const CtFrame *fr, hipStream_t st);
 hipError_t launch_convT_bwd(int C, int n, const CtFrame *fr, hipStream_t st);
 
-// Head: 1x1 convolution 8 -> 6, FP32 output; backward: d(in) = W dparam masked by the ReLU of in
-// (bf16), per block one compact partial [8][6] + [6].
+// Head (1x1 convolution 8 -> 6, FP32 output; its forward is UC_F_24_8_HEAD) backward: d(in) =
+// W dparam masked by the ReLU of in (bf16), per block one compact partial [8][6] + [6].
 struct HeadFrame {
     const uint16_t *in;         // bf16 [H][W][8]
-    float *param;               // FP32 [H][W][6]
     const float *dparam;        // FP32 [H][W][6]
     uint16_t *din;              // bf16 [H][W][8]
     const float *w, *b;         // FP32 master [8][6], [6]
     float *part;
     int H, W, nblk;
 };
-hipError_t launch_head(int n, const HeadFrame *fr, hipStream_t st);
 hipError_t launch_head_bwd(int n, const HeadFrame *fr, hipStream_t st);
 
 // Weight packing of one 3x3 layer: forward fragments [9][KC][NT][64] (K = ci, N = co) and, when
