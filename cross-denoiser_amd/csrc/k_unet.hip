@@ -402,13 +402,14 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
         for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
     float db = 0.f;
     const int nstrip = sx * sy;
-    if (li < nstrip) fetch(li);
-    for (int s = li; s < nstrip; s += nblk)
+    const StripWalk sw = xcd_strips(li, nblk, nstrip);
+    if (sw.first < sw.end) fetch(sw.first);
+    for (int s = sw.first; s < sw.end; s += sw.step)
     {
         __syncthreads();
         commit();
         __syncthreads();
-        if (s + nblk < nstrip) fetch(s + nblk);
+        if (s + sw.step < sw.end) fetch(s + sw.step);
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {

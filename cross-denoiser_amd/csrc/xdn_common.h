@@ -42,6 +42,24 @@ __device__ __forceinline__ int xcd_tile(int bid, int total)
     return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
 
+// XCD-aware strip walk of a weight-gradient block: the frame's blocks are li = 0 .. nblk - 1 and
+// blocks dealt round-robin to the 8 XCDs, so the blocks with one li mod 8 share an XCD wherever the
+// frame's blocks start in the launch. Each such group takes a contiguous band of the frame's nstrip
+// strips (row-major, so vertically adjacent strips, which share halo rows, meet in one L2) and its
+// blocks take the band's strips round-robin: first, first + step, ... below end. The walk depends
+// on li only, so a frame's partial sums are the same alone and in a batch. With fewer than 8
+// blocks the plain round-robin over all blocks is used.
+struct StripWalk {
+    int first, step, end;
+};
+__device__ __forceinline__ StripWalk xcd_strips(int li, int nblk, int nstrip)
+{
+    if (nblk < XDN_NUM_XCD) return StripWalk{li, nblk, nstrip};
+    const int g = li % XDN_NUM_XCD, lb = li / XDN_NUM_XCD, nb = (nblk - g + XDN_NUM_XCD - 1) / XDN_NUM_XCD;
+    const int s0 = (int)((long)nstrip * g / XDN_NUM_XCD), s1 = (int)((long)nstrip * (g + 1) / XDN_NUM_XCD);
+    return StripWalk{s0 + lb, nb, s1};
+}
+
 struct TileLoc {
     int f, tx, ty;
 };
