@@ -541,6 +541,8 @@ int xdn_cnn5_forward(const float *ols1, const float *ols2, const float *prev_out
                      int batch, int height, int width, void *stream)
 {
     if (int e = cnn5_check(batch, height, width, workspace_bytes)) return e;
+    if (!ols1 || !ols2 || !prev_out || !albedo || !normal || !packed || !param || !workspace)
+        return fail(XDN_EINVAL, "cnn5 forward: null buffer");
     Cnn5Ws ws(workspace, batch, height, width);
     const size_t N = px(height, width);
     const char *pk = static_cast<const char *>(packed);
@@ -577,6 +579,8 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
                       size_t workspace_bytes, int batch, int height, int width, void *stream)
 {
     if (int e = cnn5_check(batch, height, width, workspace_bytes)) return e;
+    if (!ols1 || !ols2 || !prev_out || !albedo || !normal || !dparam || !packed || !grad || !workspace)
+        return fail(XDN_EINVAL, "cnn5 backward: null buffer");
     Cnn5Ws ws(workspace, batch, height, width);
     const size_t N = px(height, width);
     const char *pk = static_cast<const char *>(packed);

@@ -312,6 +312,7 @@ int xdn_unet_workspace_layout(int batch, int height, int width, int index, size_
 
 int xdn_unet_pack(const float *params, void *packed, void *stream)
 {
+    if (!params || !packed) return fail(XDN_EINVAL, "unet pack: null buffer");
     std::vector<xdn::UPackJob> jobs;
     unet_pack_jobs(params, packed, jobs);
     return hip_check(xdn::launch_upack((int)jobs.size(), jobs.data(), S(stream)), "unet pack");
@@ -347,6 +348,8 @@ int xdn_unet_forward(const float *ols1, const float *ols2, const float *prev_out
                      int batch, int height, int width, void *stream)
 {
     if (int e = unet_check(batch, height, width, workspace_bytes)) return e;
+    if (!ols1 || !ols2 || !prev_out || !albedo || !normal || !packed || !param || !workspace)
+        return fail(XDN_EINVAL, "unet forward: null buffer");
     const UnetWs ws(workspace, batch, height, width);
     const float *in[5] = {ols1, ols2, prev_out, albedo, normal};
     const std::vector<UnetFrame> F = batch_frames(ws, batch, in, packed, param, nullptr);
@@ -358,6 +361,8 @@ int xdn_unet_backward(const float *ols1, const float *ols2, const float *prev_ou
                       size_t workspace_bytes, int batch, int height, int width, void *stream)
 {
     if (int e = unet_check(batch, height, width, workspace_bytes)) return e;
+    if (!ols1 || !ols2 || !prev_out || !albedo || !normal || !dparam || !packed || !grad || !workspace)
+        return fail(XDN_EINVAL, "unet backward: null buffer");
     const UnetWs ws(workspace, batch, height, width);
     const float *in[5] = {ols1, ols2, prev_out, albedo, normal};
     const std::vector<UnetFrame> F = batch_frames(ws, batch, in, packed, nullptr, dparam);

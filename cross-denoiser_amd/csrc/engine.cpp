@@ -74,6 +74,30 @@ int check_slot(const xdn_slot &s)
     return XDN_OK;
 }
 
+// The buffers the requested stages read or write must be set (a null one would fault the device).
+int check_slot_ptrs(const xdn_slot &s, int stages)
+{
+    struct Need {
+        const void *p;
+        int stages;
+        const char *name;
+    };
+    constexpr int R = XDN_STAGE_REGRESS, P = XDN_STAGE_REPROJECT, F = XDN_STAGE_FINISH;
+    const Need need[] = {
+        {s.scratch, R | P | F, "scratch"},   {s.color, R, "color"},           {s.color2, R, "color2"},
+        {s.albedo, R | F, "albedo"},         {s.normal, R | P | F, "normal"}, {s.ols1, R | F, "ols1"},
+        {s.ols2, R | F, "ols2"},             {s.mvec, P, "mvec"},             {s.linear_z, P, "linear_z"},
+        {s.prev_linear_z, P, "prev_linear_z"}, {s.prev_normal, P, "prev_normal"}, {s.pn_fwidth, P, "pn_fwidth"},
+        {s.prev_out_log, P, "prev_out_log"}, {s.prev_ols1, P, "prev_ols1"}, {s.prev_ols2, P, "prev_ols2"},
+        {s.opacity, F, "opacity"},           {s.out_log, F, "out_log"},       {s.out, F, "out"},
+        {s.params, F, "params"},             {s.grad, F, "grad"},             {s.adam_m, F, "adam_m"},
+        {s.adam_v, F, "adam_v"},             {s.packed, F, "packed"},         {s.avg_loss, F, "avg_loss"},
+    };
+    for (const Need &n : need)
+        if ((n.stages & stages) && !n.p) return fail(XDN_EINVAL, "slot buffer %s is null", n.name);
+    return XDN_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -99,6 +123,8 @@ int xdn_slot_scratch_offsets(int height, int width, size_t *offsets)
 int xdn_slot_init(const xdn_slot *slot, void *stream)
 {
     if (int e = check_slot(*slot)) return e;
+    if (!slot->scratch || !slot->params || !slot->packed)
+        return fail(XDN_EINVAL, "slot init needs scratch, params and packed");
     SlotScratch sc(slot->scratch, slot->height, slot->width);
     const size_t n = px(slot->height, slot->width);
     hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(256), 0, S(stream), sc.ones, n, 1.0f);
@@ -121,6 +147,8 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
         if (int e = check_slot(slots[i])) return e;
         if (slots[i].network != slots[0].network) return fail(XDN_EINVAL, "one call takes slots of one network");
     }
+    for (int i = 0; i < n_slots; ++i)
+        if (int e = check_slot_ptrs(slots[i], stages)) return e;
     if (n_slots == 0) return XDN_OK;
     const bool unet = slots[0].network == XDN_NET_UNET;
     hipStream_t st = S(stream);

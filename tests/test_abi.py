@@ -94,3 +94,20 @@ def test_slot_network_checks_without_gpu():
     assert lib.xdn_denoise_step(ctypes.addressof(pair), 2, 1e-3, None) == -1
     assert "one network" in lib.xdn_last_error().decode()
     assert pipeline.slot_bytes(1080, 1920, "unet") > 0 and pipeline.slot_bytes(1080, 1920, "cnn5") > 0
+
+
+def test_null_buffers_are_rejected_without_gpu():
+    """Entry points reject null buffers before anything is launched."""
+    from xdn import pipeline
+    from xdn._lib import lib
+    ws = int(lib.xdn_unet_workspace_size(1, 32, 32))
+    assert lib.xdn_unet_forward(None, None, None, None, None, None, None, None, ws, 1, 32, 32, None) == -1
+    assert "null" in lib.xdn_last_error().decode()
+    ws5 = int(lib.xdn_cnn5_workspace_size(1, 32, 32))
+    assert lib.xdn_cnn5_backward(*([None] * 9), ws5, 1, 32, 32, None) == -1
+    assert "null" in lib.xdn_last_error().decode()
+    s = pipeline.XdnSlot()
+    s.height, s.width = 64, 64
+    assert lib.xdn_denoise_step(ctypes.addressof(s), 1, 1e-3, None) == -1
+    assert "is null" in lib.xdn_last_error().decode()
+    assert lib.xdn_slot_init(ctypes.addressof(s), None) == -1
