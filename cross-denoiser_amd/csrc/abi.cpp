@@ -75,7 +75,9 @@ static int cross_regression_impl(const float *color, const float *color2, const 
     if (H % rs != 0 || W % rs != 0) return fail(XDN_EINVAL, "Image height and width must be divisible by recon_scale");
     if (H % s != 0 || W % s != 0) return fail(XDN_EINVAL, "Image height and width must be divisible by sparsity");
     if (H <= win / 2 || W <= win / 2) return fail(XDN_EINVAL, "Image height and width must exceed win_size/2");
-    if (ws_bytes < xdn_cross_regression_workspace_size(B, H, W, s))
+    if ((!color && (!imga || !imgb)) || !albedo || !normal || !outa || !outb || !vara || !varb)
+        return fail(XDN_EINVAL, "cross_regression: null tensor");
+    if (!ws || ws_bytes < xdn_cross_regression_workspace_size(B, H, W, s))
         return fail(XDN_EINVAL, "workspace too small: %zu < %zu", ws_bytes, xdn_cross_regression_workspace_size(B, H, W, s));
     const bool fused = color != nullptr;
     const size_t N = px(H, W), Nr = px(H / rs, W / rs), Nb = (size_t)(H / s) * (W / s) * 36;
@@ -233,6 +235,8 @@ int xdn_spatial_filter(const float *imga, const float *imgb, const float *albedo
                        int batch, int height, int width, int win_size, void *stream)
 {
     if (int e = sf_common_check(batch, height, width, win_size)) return e;
+    if (!imga || !imgb || !albedo || !normal || !band || !out_acca || !out_accb || !out_wgtsuma || !out_wgtsumb)
+        return fail(XDN_EINVAL, "spatial_filter: null tensor");
     const size_t N = px(height, width);
     return sf_run(
         batch, height, width, win_size,
@@ -252,6 +256,9 @@ int xdn_spatial_filter_grad(const float *grad_acca, const float *grad_accb, cons
                             int width, int win_size, void *stream)
 {
     if (int e = sf_common_check(batch, height, width, win_size)) return e;
+    if (!grad_acca || !grad_accb || !grad_wgtsuma || !grad_wgtsumb || !imga || !imgb || !img_albedo || !img_normal ||
+        !band || !out_grad_band)
+        return fail(XDN_EINVAL, "spatial_filter_grad: null tensor");
     const size_t N = px(height, width);
     return sf_run(
         batch, height, width, win_size,
@@ -274,6 +281,9 @@ int xdn_spatial_filter_grad_inputs(const float *grad_acca, const float *grad_acc
 {
     if (int e = sf_common_check(batch, height, width, win_size)) return e;
     if (win_size / 2 > 7) return fail(XDN_EINVAL, "input gradients support win_size <= 15");
+    if (!grad_acca || !grad_accb || !grad_wgtsuma || !grad_wgtsumb || !imga || !imgb || !img_albedo || !img_normal ||
+        !band)
+        return fail(XDN_EINVAL, "spatial_filter_grad_inputs: null tensor");
     const size_t N = px(height, width);
     auto off = [](float *p, size_t o) { return p ? p + o : nullptr; };
     return sf_run(
@@ -297,8 +307,14 @@ int xdn_reproject(const float *mvec, int n_list, const float *const *inputs, con
 {
     if (int e = check_shape(batch, height, width)) return e;
     if (n_list < 0) return fail(XDN_EINVAL, "n_list must be >= 0");
+    if (n_list > 0 && !dims) return fail(XDN_EINVAL, "reproject: null dims");
     for (int l = 0; l < n_list; ++l)
         if (dims[l] < 1 || dims[l] > 4) return fail(XDN_EINVAL, "input_list[%d] has %d channels (1..4 supported)", l, dims[l]);
+    if (!mvec || !linear_z || !prev_linear_z || !normal || !prev_normal || !pn_fwidth || !opacity || !prev_opacity ||
+        !success || (n_list > 0 && (!inputs || !outputs)))
+        return fail(XDN_EINVAL, "reproject: null tensor");
+    for (int l = 0; l < n_list; ++l)
+        if (!inputs[l] || !outputs[l]) return fail(XDN_EINVAL, "reproject: input_list[%d] is null", l);
     const size_t N = px(height, width);
     // buffers beyond XDN_MAX_LIST go to further launches (the mask is recomputed, identically)
     int l0 = 0;
@@ -338,6 +354,8 @@ static int screening_impl(const float *c1, const float *c2, const float *prev, c
 {
     if (int e = check_shape(B, H, W)) return e;
     if (win_curr < 1 || win_prev < 1) return fail(XDN_EINVAL, "win_curr and win_prev must be >= 1");
+    if (!c1 || !c2 || !prev || !prev1 || !prev2 || !o || !o1 || !o2 || !osucc)
+        return fail(XDN_EINVAL, "temporal_screening: null tensor");
     const size_t N = px(H, W);
     for (int b0 = 0; b0 < B; b0 += XDN_MAX_FRAMES)
     {
@@ -397,7 +415,10 @@ int xdn_filter_train(const float *ols1, const float *ols2, const float *albedo, 
 {
     if (int e = sf_common_check(batch, height, width, win_size)) return e;
     if (win_size / 2 != 5) return fail(XDN_EINVAL, "the fused training kernel supports win_size 11 (half 5) only");
-    if (workspace_bytes < xdn_filter_train_workspace_size(batch, height, width))
+    if (!ols1 || !ols2 || !albedo || !normal || !param || !success || !success_prev || !prev_out || !opacity ||
+        !prev_ols1 || !prev_ols2 || !out_log || !out || !dparam)
+        return fail(XDN_EINVAL, "filter_train: null tensor");
+    if (!workspace || workspace_bytes < xdn_filter_train_workspace_size(batch, height, width))
         return fail(XDN_EINVAL, "workspace too small");
     const size_t N = px(height, width);
     const int T = xdn::filter_train_tiles(height, width);
@@ -448,7 +469,7 @@ int xdn_frame_error(const float *out, const float *emissive, const float *env_li
 {
     if (int e = check_shape(batch, height, width)) return e;
     if (!out || !ref || !err) return fail(XDN_EINVAL, "frame error: out, ref and err are required");
-    if (workspace_bytes < xdn_frame_error_workspace_size(batch, height, width))
+    if (!workspace || workspace_bytes < xdn_frame_error_workspace_size(batch, height, width))
         return fail(XDN_EINVAL, "frame error: workspace too small");
     const size_t N3 = px(height, width) * 3;
     const int nb = xdn::frame_error_blocks(height, width);
@@ -483,6 +504,7 @@ int xdn_adam(float *param, const float *grad, float *m, float *v, int count, int
              float beta2, float eps, void *stream)
 {
     if (count < 0 || step < 1) return fail(XDN_EINVAL, "count must be >= 0 and step >= 1");
+    if (count > 0 && (!param || !grad || !m || !v)) return fail(XDN_EINVAL, "adam: null tensor");
     xdn::AdamJob j{};
     j.param = param;
     j.grad = const_cast<float *>(grad);
@@ -513,6 +535,7 @@ size_t xdn_cnn5_workspace_size(int batch, int height, int width)
 
 int xdn_cnn5_pack(const float *params, void *packed, void *stream)
 {
+    if (!params || !packed) return fail(XDN_EINVAL, "cnn5 pack: null buffer");
     xdn::PackJob j[5];
     char *pk = static_cast<char *>(packed);
     for (int l = 0; l < 5; ++l)

@@ -111,3 +111,26 @@ def test_null_buffers_are_rejected_without_gpu():
     assert lib.xdn_denoise_step(ctypes.addressof(s), 1, 1e-3, None) == -1
     assert "is null" in lib.xdn_last_error().decode()
     assert lib.xdn_slot_init(ctypes.addressof(s), None) == -1
+
+
+def test_single_op_null_buffers_are_rejected_without_gpu():
+    """The single-op entry points reject a missing required tensor before anything is launched; the
+    documented optional ones (variance gradients, unrequested input gradients, the per-pixel loss) stay
+    optional."""
+    from xdn._lib import lib
+
+    def rejected(rc):
+        return rc == -1 and "null" in lib.xdn_last_error().decode()
+
+    ws = 1 << 20
+    assert rejected(lib.xdn_cross_regression(*([None] * 8), ws, ws, 1, 32, 32, 7, 4, 1, None))
+    assert rejected(lib.xdn_spatial_filter(*([None] * 9), 1, 32, 32, 11, None))
+    assert rejected(lib.xdn_spatial_filter_grad(*([None] * 10), 1, 32, 32, 11, None))
+    assert rejected(lib.xdn_spatial_filter_grad_inputs(*([None] * 13), 1, 32, 32, 11, None))
+    assert rejected(lib.xdn_reproject(None, 0, None, None, *([None] * 8), None, 1, 32, 32, None))
+    assert rejected(lib.xdn_temporal_screening(*([None] * 5), 1, *([None] * 4), 1, 32, 32, 7, 7, None))
+    assert rejected(lib.xdn_filter_train(*([None] * 16), ws, ws, 1, 32, 32, 11, None))
+    assert rejected(lib.xdn_adam(None, None, None, None, 16, 1, 1e-3, 0.9, 0.999, 1e-7, None))
+    assert rejected(lib.xdn_cnn5_pack(None, None, None))
+    # nothing to update: no buffers needed
+    assert lib.xdn_adam(None, None, None, None, 0, 1, 1e-3, 0.9, 0.999, 1e-7, None) in (0, -1)
