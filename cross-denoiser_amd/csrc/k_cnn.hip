@@ -31,13 +31,14 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
 
-__device__ __forceinline__ uint32_t f2bf(float f)
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+
+// Two floats -> packed bf16 {a, b}, round to nearest even (one v_cvt_pk_bf16_f32; finite inputs).
+__device__ __forceinline__ uint32_t pack2(float a, float b)
 {
-    // round to nearest even (inputs are finite)
-    const uint32_t u = __float_as_uint(f);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
-__device__ __forceinline__ uint32_t pack2(float a, float b) { return f2bf(a) | (f2bf(b) << 16); }
 __device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
@@ -58,21 +59,27 @@ static_assert(sizeof(ConvBatch) <= 4096, "kernel argument too large");
 
 constexpr int CT_W = 32, CT_H = 16;
 
+// Smallest float whose bf16 rounding (nearest even) is nonzero is just above 2^-134: a ReLU output
+// r >= 0 is stored as a positive bf16 exactly when r > 2^-134.
+constexpr float kBf16Tiny = 0x1p-134f;
+
 // Persistent and software-pipelined: a block walks the tiles it, it + gridDim.x, ... (gridDim.x a
 // multiple of 8, so a block stays on one XCD and xcd_tile keeps each XCD on a contiguous run of
 // tiles). The global loads of the next tile are issued into registers before the MFMAs of the
 // current one and land in LDS at the top of the next iteration. The kernel fragments of the tile's
-// frame sit in LDS (reloaded when the frame changes), which keeps two blocks per CU.
+// frame sit in LDS (reloaded when the frame changes) and are the MFMA A operands; the B operands
+// are 32 pixels x 16 input channels read from the LDS tile, so the accumulators hold C[channel][pixel]
+// and a lane owns 16 channels c(k) = 8(k/4) + 4h + k%4 of one pixel: the epilogue works in
+// registers, and a permlane32 swap between the two half-waves gives each lane 2 x 8 consecutive
+// channels for 16-B stores.
 template <int IN, int CIN, int EPI>
 __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
 {
     constexpr int PS = CIN + 8;  // LDS pixel stride in bf16 (20 or 12 dwords: conflict-free b128 reads)
     constexpr int LW = CT_W + 2, LH = CT_H + 2;
     constexpr int KC = CIN / 16;
-    constexpr int IN_ELEMS = LH * LW * PS, OUT_ELEMS = CT_H * CT_W * 40;  // input tile / epilogue tile
-    __shared__ __attribute__((aligned(16))) uint16_t lds[IN_ELEMS > OUT_ELEMS ? IN_ELEMS : OUT_ELEMS];
-    __shared__ u32x4 wl[9 * KC * 64];  // kernel fragments of the current frame
-    __shared__ uint32_t mlds[EPI == CONV_EPI_MASK ? CT_H * CT_W : 1];  // ReLU bits of the tile (backward)
+    __shared__ __attribute__((aligned(16))) uint16_t lds[LH * LW * PS];
+    __shared__ u32x4 wl[9 * KC * 64];  // kernel fragments of the current frame: [tap][kc][lane]
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
     const int total = b.map.total;
@@ -158,7 +165,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
         }
     };
 
-    float bias = 0.f;
+    float bias[16];  // bias of the lane's channels c(k)
     int cur_f = -1;
     if ((int)blockIdx.x < total) fetch(blockIdx.x);
     for (int it = blockIdx.x; it < total; it += gridDim.x)
@@ -167,26 +174,26 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
     const ConvFrame &F = b.f[L.f];
     const int H = F.H, W = F.W;
     const int x0 = L.tx * CT_W, y0 = L.ty * CT_H;
-    __syncthreads();  // the previous tile's epilogue and MFMAs are done with LDS
+    __syncthreads();  // every wave is done reading the previous tile from LDS
     if (L.f != cur_f)
     {
         for (int e = tid; e < 9 * KC * 64; e += 256) wl[e] = static_cast<const u32x4 *>(F.wfrag)[e];
-        bias = (EPI == CONV_EPI_MASK) ? 0.f : F.bias[px];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) bias[k] = (EPI == CONV_EPI_MASK) ? 0.f : F.bias[8 * (k >> 2) + 4 * h + (k & 3)];
         cur_f = L.f;
     }
     commit();
     __syncthreads();
     if (it + (int)gridDim.x < total) fetch(it + gridDim.x);
-    // the backward's ReLU bits of the tile's pixels tid and tid + 256, loaded now so their latency
-    // hides behind the MFMAs; they go through LDS to the lanes that store those pixels
-    uint32_t mpre[2] = {0u, 0u};
+    // the backward's ReLU bits of the lane's pixel in each of its 4 output rows, loaded now so their
+    // latency hides behind the MFMAs
+    uint32_t mrow[4] = {0u, 0u, 0u, 0u};
     if constexpr (EPI == CONV_EPI_MASK)
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
+        for (int orow = 0; orow < 4; ++orow)
         {
-            const int pix = tid + 256 * k;
-            const int gy = y0 + pix / CT_W, gx = x0 + (pix & (CT_W - 1));
-            if (gy < H && gx < W) mpre[k] = F.mask[(size_t)gy * W + gx];
+            const int gy = y0 + 4 * wv + orow, gx = x0 + px;
+            if (gy < H && gx < W) mrow[orow] = F.mask[(size_t)gy * W + gx];
         }
 
     f32x16 acc[4];
@@ -194,112 +201,105 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc[r][k] = 0.f;
-    // one (tap, channel chunk) per step; steps are not unrolled so only a step's five fragments are
-    // live, which keeps two blocks per CU
-#pragma unroll 2
-    for (int t = 0; t < 9 * KC; ++t)
-    {
-        const int tap = t / KC, kc = t - tap * KC, dy = tap / 3, dx = tap - dy * 3;
-        const u32x4 bw = wl[t * 64 + lane];
-        const uint16_t *ap = &lds[((4 * wv + dy) * LW + px + dx) * PS + kc * 16 + 8 * h];
+    // wave wv owns output rows 4wv..4wv+3, which read LDS rows 4wv..4wv+5: each B fragment (LDS row
+    // lr, column shift dx, channel chunk kc) is read once and feeds the kernel rows dy with
+    // output row lr - dy in 0..3
 #pragma unroll
-        for (int orow = 0; orow < 4; ++orow)
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
         {
-            // output row 4wv + orow reads LDS row 4wv + orow + dy (kernel row dy)
-            const u32x4 a = *reinterpret_cast<const u32x4 *>(ap + orow * LW * PS);
-            acc[orow] = mfma(a, bw, acc[orow]);
-        }
-    }
-
-    // ---- epilogue: lane owns output channel px of 16 pixels x = 8(k/4) + 4h + k%4 per row. The
-    // tile is transposed through LDS (the input tile is dead after the barrier) so every global
-    // store -- and the ReLU-mask load of the backward -- moves 16 contiguous bytes.
-    __syncthreads();
-    if constexpr (EPI == CONV_EPI_F32_6)
-    {
-        float *o = reinterpret_cast<float *>(lds);  // [16 rows][32 px][6]
-        static_assert(sizeof(lds) >= CT_H * CT_W * 6 * sizeof(float), "epilogue buffer");
-        if (px < 6)
+            u32x4 wk[3], bx[6];
 #pragma unroll
-            for (int orow = 0; orow < 4; ++orow)
+            for (int dy = 0; dy < 3; ++dy) wk[dy] = wl[((dy * 3 + dx) * KC + kc) * 64 + lane];
 #pragma unroll
-                for (int k = 0; k < 16; ++k)
+            for (int lr = 0; lr < 6; ++lr)
+                bx[lr] = *reinterpret_cast<const u32x4 *>(&lds[((4 * wv + lr) * LW + px + dx) * PS + kc * 16 + 8 * h]);
+#pragma unroll
+            for (int lr = 0; lr < 6; ++lr)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
                 {
-                    const int x = 8 * (k >> 2) + 4 * h + (k & 3);
-                    o[((4 * wv + orow) * CT_W + x) * 6 + px] = acc[orow][k] + bias;
+                    const int orow = lr - dy;
+                    if (orow >= 0 && orow < 4) acc[orow] = mfma(wk[dy], bx[lr], acc[orow]);
                 }
-        __syncthreads();
-        // each row of the tile is 32 px x 6 floats = 48 float4, contiguous in the image
-        for (int i = tid; i < CT_H * 48; i += 256)
-        {
-            const int r = i / 48, q = i - r * 48;
-            const int gy = y0 + r, x = (q * 4) / 6;
-            if (gy >= H || x0 + x >= W) continue;
-            const float4 v = reinterpret_cast<const float4 *>(o)[r * 48 + q];
-            float *dst = static_cast<float *>(F.out) + ((size_t)gy * W + x0) * 6 + q * 4;
-            if (x0 + CT_W <= W && (W & 1) == 0)  // whole row of the tile, 16-B aligned
-                *reinterpret_cast<float4 *>(dst) = v;
-            else
-            {
-                // partial tile: element e of this float4 is pixel (4q + e) / 6
-                const float vv[4] = {v.x, v.y, v.z, v.w};
+        }
+
+    // ---- epilogue, per output row: lane (px, h) holds channels c(k) of pixel (row, x0 + px) ----
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (x0 + (4 * q + e) / 6 < W) dst[e] = vv[e];
+    for (int orow = 0; orow < 4; ++orow)
+    {
+        const int gy = y0 + 4 * wv + orow, gx = x0 + px;
+        const bool in = gy < H && gx < W;
+        const size_t p = (size_t)gy * W + gx;
+        if constexpr (EPI == CONV_EPI_F32_6)
+        {
+            // network output: channels 0..3 on the h = 0 lane, 4..5 on the h = 1 lane
+            float *o = static_cast<float *>(F.out) + p * 6;
+            const float v0 = acc[orow][0] + bias[0], v1 = acc[orow][1] + bias[1];
+            const float v2 = acc[orow][2] + bias[2], v3 = acc[orow][3] + bias[3];
+            if (in)
+            {
+                if (h == 0)
+                {
+                    *reinterpret_cast<float2 *>(o) = make_float2(v0, v1);
+                    *reinterpret_cast<float2 *>(o + 2) = make_float2(v2, v3);
+                }
+                else
+                    *reinterpret_cast<float2 *>(o + 4) = make_float2(v0, v1);
             }
         }
-    }
-    else
-    {
-        constexpr int OS = 40;  // bf16 pixel stride of the output tile in LDS
-        static_assert(sizeof(lds) >= CT_H * CT_W * OS * 2, "epilogue buffer");
-#pragma unroll
-        for (int orow = 0; orow < 4; ++orow)
+        else
+        {
+            float r[16];
 #pragma unroll
             for (int k = 0; k < 16; ++k)
-            {
-                const int x = 8 * (k >> 2) + 4 * h + (k & 3);
-                const float v = acc[orow][k];
-                lds[((4 * wv + orow) * CT_W + x) * OS + px] =
-                    (uint16_t)f2bf(EPI == CONV_EPI_RELU ? fmaxf(v + bias, 0.f) : v);
-            }
-        if constexpr (EPI == CONV_EPI_MASK)
-        {
-            mlds[tid] = mpre[0];
-            mlds[tid + 256] = mpre[1];
-        }
-        __syncthreads();
-        // four consecutive lanes (one DPP quad) hold the four 8-channel quarters of one pixel
-        for (int i = tid; i < CT_H * CT_W * 4; i += 256)
-        {
-            const int pix = i >> 2, q = i & 3;
-            const int gy = y0 + pix / CT_W, gx = x0 + (pix & (CT_W - 1));
-            const bool in = gy < H && gx < W;
-            u32x4 v = *reinterpret_cast<const u32x4 *>(&lds[pix * OS + q * 8]);
-            const size_t p = (size_t)gy * W + gx, o = p * 32 + q * 8;
+                r[k] = EPI == CONV_EPI_RELU ? fmaxf(acc[orow][k] + bias[k], 0.f) : acc[orow][k];
+            uint32_t d[8];  // d[2j + e] = channels 8j + 4h + 2e, +1
+#pragma unroll
+            for (int q = 0; q < 8; ++q) d[q] = pack2(r[2 * q], r[2 * q + 1]);
             if constexpr (EPI == CONV_EPI_RELU)
             {
-                // ReLU bits of the pixel, bit 8q + e = (bf16 channel 8q + e > 0): its bits lie in
-                // [1, 0x7FFF] (a -0 from the max counts as zero); the quad's bytes are ORed with
-                // two DPP quad permutes (xor 1: [1,0,3,2] = 0xB1, xor 2: [2,3,0,1] = 0x4E)
-                auto pos = [](uint32_t hb) { return (hb - 1u) < 0x7FFFu ? 1u : 0u; };
-                auto two = [&](uint32_t w) { return pos(w & 0xFFFFu) | (pos(w >> 16) << 1); };
-                uint32_t m = (two(v.x) | (two(v.y) << 2) | (two(v.z) << 4) | (two(v.w) << 6)) << (8 * q);
-                m |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0xB1, 0xF, 0xF, false);
-                m |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x4E, 0xF, 0xF, false);
-                if (F.mask_out && in && q == 0) F.mask_out[p] = m;
+                // ReLU bits of the pixel, bit c = (bf16 channel c > 0); this lane's 16 channels, then
+                // the other half-wave's 16 through a permlane32 swap
+                uint32_t bits = 0u;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) bits |= (r[k] > kBf16Tiny ? 1u : 0u) << (8 * (k >> 2) + (k & 3));
+                bits <<= 4 * h;
+                const auto pb = __builtin_amdgcn_permlane32_swap(bits, bits, false, false);
+                if (F.mask_out && in && h == 0) F.mask_out[p] = bits | (uint32_t)pb[1];
             }
-            if (!in) continue;
-            if constexpr (EPI == CONV_EPI_MASK)
+            else
             {
                 // keep the gradient where the layer's bf16 activation is > 0 (ReLU derivative)
-                const uint32_t mb = mlds[pix] >> (8 * q);
-                auto keep = [](uint32_t bits, uint32_t vw) {
-                    return vw & (((bits & 1u) ? 0xFFFFu : 0u) | ((bits & 2u) ? 0xFFFF0000u : 0u));
-                };
-                v.x = keep(mb, v.x); v.y = keep(mb >> 2, v.y); v.z = keep(mb >> 4, v.z); v.w = keep(mb >> 6, v.w);
+                const uint32_t m = mrow[orow] >> (4 * h);
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                {
+                    const int pos = 8 * (q >> 1) + 2 * (q & 1);
+                    const uint32_t lo = (m >> pos) & 1u ? 0x0000FFFFu : 0u;
+                    const uint32_t hi = (m >> (pos + 1)) & 1u ? 0xFFFF0000u : 0u;
+                    d[q] &= lo | hi;
+                }
             }
-            *reinterpret_cast<u32x4 *>(static_cast<uint16_t *>(F.out) + o) = v;
+            // lane h gathers channels 16g + 8h .. 16g + 8h + 7 (g = 0, 1): the low half-wave keeps its
+            // group 2g and takes the high half's; the high half keeps group 2g + 1 and takes the low's
+            u32x4 v[2];
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                {
+                    const auto sw = __builtin_amdgcn_permlane32_swap(d[4 * g + e], d[4 * g + 2 + e], false, false);
+                    v[g][e] = (uint32_t)sw[0];
+                    v[g][2 + e] = (uint32_t)sw[1];
+                }
+            if (in)
+            {
+                uint16_t *o = static_cast<uint16_t *>(F.out) + p * 32 + 8 * h;
+                *reinterpret_cast<u32x4 *>(o) = v[0];
+                *reinterpret_cast<u32x4 *>(o + 16) = v[1];
+            }
         }
     }
     }  // tile loop

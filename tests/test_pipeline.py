@@ -65,14 +65,16 @@ def test_denoiser_stages_match_oracle(dev):
 
 def test_denoiser_trains(dev):
     """Repeated steps on one frame with the history reset before each step (every step is a
-    first frame, so the loss depends on the network alone): the online training lowers the loss."""
+    first frame, so the loss depends on the network alone): the online training lowers the loss.
+    At this learning rate the loss sits on a plateau for ~20 steps, swings for ~10 after leaving
+    it (where rounding-level differences change the path), then settles ~25% lower by step 40."""
     from xdn import pipeline, synth
     H, W = 64, 64
     spec = synth.StreamSpec(H, W, seed=5, pan=(0.0, 0.0))
     frame = {k: v.to(dev) for k, v in synth.make_frame(spec, 0).items()}
     den = pipeline.Denoiser(H, W, device=dev, lr=1e-2)
     losses = []
-    for _ in range(30):
+    for _ in range(60):
         den.prev_frame = None
         for bufs in den.hist:
             for t in bufs:
