@@ -56,27 +56,40 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
 #pragma unroll
         for (int k = 0; k < 4; ++k) acc[l][k] = 0.f;
 
+    // Three round trips to memory: the pixel's own inputs; every bilinear tap's validity inputs,
+    // loaded at coordinates clamped into the image and used only where the tap is in the image;
+    // the list values of the valid taps.
+    const float op = F.opac[c];
+    const float mx = F.mvec[3 * c], my = F.mvec[3 * c + 1];
+    const float fwN = F.pnFw[3 * c + 1];
+    const float Z = F.linZ[3 * c], fwZ = F.linZ[3 * c + 1];
+    const float3 n = ld3(F.nrm, c);
     bool valid = false;
-    if (F.opac[c] != 0.f)
+    if (op != 0.f)
     {
-        const float mx = F.mvec[3 * c], my = F.mvec[3 * c + 1];
-        const float fwN = F.pnFw[3 * c + 1];
         const float ppx = __builtin_fmaf(mx, (float)W, (float)cx);
         const float ppy = __builtin_fmaf(my, (float)H, (float)cy);
         const int ipx = (int)(ppx + 0.5f), ipy = (int)(ppy + 0.5f);
-        const float Z = F.linZ[3 * c], fwZ = F.linZ[3 * c + 1];
-        const float3 n = ld3(F.nrm, c);
         const int bx = (int)ppx, by = (int)ppy;
-        bool v[4];
+        bool inb[4];
+        size_t li[4];
+        float po[4], pz[4];
+        float3 pn[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t)
         {
             const int lx = bx + (t & 1), ly = by + (t >> 1);
-            v[t] = false;
-            if (lx >= W || lx < 0 || ly >= H || ly < 0) continue;
-            const size_t li = (size_t)ly * W + lx;
-            if (F.prevOpac[li] == 0.f) continue;
-            v[t] = reproj_valid(W, H, ipx, ipy, Z, F.prevLinZ[3 * li], fwZ, n, ld3(F.prevNrm, li), fwN);
+            inb[t] = !(lx >= W || lx < 0 || ly >= H || ly < 0);
+            li[t] = (size_t)min(max(ly, 0), H - 1) * W + min(max(lx, 0), W - 1);
+            po[t] = F.prevOpac[li[t]];
+            pz[t] = F.prevLinZ[3 * li[t]];
+            pn[t] = ld3(F.prevNrm, li[t]);
+        }
+        bool v[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+        {
+            v[t] = inb[t] && po[t] != 0.f && reproj_valid(W, H, ipx, ipy, Z, pz[t], fwZ, n, pn[t], fwN);
             valid = valid || v[t];
         }
         if (valid)
@@ -88,13 +101,12 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
             for (int t = 0; t < 4; ++t)
             {
                 if (!v[t]) continue;
-                const size_t li = (size_t)(by + (t >> 1)) * W + (bx + (t & 1));
 #pragma unroll
                 for (int l = 0; l < XDN_MAX_LIST; ++l)
                 {
                     if (l >= nl) break;
                     const int d = FNL > 0 ? FD : b.dims[l];
-                    const float *src = F.in[l] + li * d;
+                    const float *src = F.in[l] + li[t] * d;
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
                         if (k < d) acc[l][k] = __builtin_fmaf(w[t], src[k], acc[l][k]);

@@ -8,8 +8,9 @@
 // exact given identical inputs; the set average is computed as (c1 + c2) * 0.5f in FP32.
 //
 // screening_tile (win_curr 7, win_prev 1): 32x8 pixels per workgroup; the tile plus a 3-px halo of
-// (c1, c2, avg) goes to LDS, then a separable min/max (7-tap row pass into LDS, 7-tap column pass)
-// gives the 18 extrema per pixel with 14 instead of 49 taps. HBM-bound.
+// (c1, c2, avg) goes to LDS, then a separable min/max (7-tap row pass into LDS, two outputs per
+// thread; 7-tap column pass) gives the 18 extrema per pixel with 14 instead of 49 taps, unrolled
+// into v_min3/v_max3 chains.
 // screening_generic: any window sizes, direct loads.
 #include "xdn_common.h"
 #include "xdn_kernels.h"
@@ -84,13 +85,15 @@ __device__ __forceinline__ void emit_values(const TsFrame &F, size_t c, float sc
 }
 
 // Every global load of the block is issued up front (the pixel's history and the success input with
-// the tile staging), so a block makes one round trip to memory instead of three.
+// the tile staging), so a block makes one round trip to memory instead of three. Staging clamps the
+// coordinates into the image: a clamped tap repeats a pixel that is already inside the window, so
+// the extrema over the fixed 7-tap windows equal those over the in-image taps the reference visits.
 __global__ __launch_bounds__(256) void screening_tile(TsBatch b)
 {
     constexpr int EX = TS_TX + 2 * TS_R, EY = TS_TY + 2 * TS_R;
-    __shared__ float v[9][EY][EX];          // avg.xyz, c1.xyz, c2.xyz
-    __shared__ float hmn[9][EY][TS_TX];     // row-pass minima
-    __shared__ float hmx[9][EY][TS_TX];     // row-pass maxima
+    constexpr int EXP = EX + 2;             // row pitch (even: 8-B aligned pairs)
+    __shared__ float v[9][EY][EXP];         // avg.xyz, c1.xyz, c2.xyz
+    __shared__ float2 hm[9][EY][TS_TX];     // row-pass (min, max)
     __shared__ uint8_t ms[EY][EX];          // success mask (fused erosion), 1 outside the image
     const int tile = xcd_tile(blockIdx.x, b.map.total);
     const TileLoc L = locate(b.map, tile);
@@ -117,39 +120,34 @@ __global__ __launch_bounds__(256) void screening_tile(TsBatch b)
         v[6][ly][lx] = bb.x; v[7][ly][lx] = bb.y; v[8][ly][lx] = bb.z;
     }
     __syncthreads();
-    // row pass over the in-bounds columns of each window row
-    for (int i = threadIdx.x; i < EY * TS_TX; i += 256)
+    // row pass: a thread takes two neighbouring outputs (window columns x..x+6 and x+1..x+7 of the
+    // staged row), sharing the six common taps
+    if (threadIdx.x < EY * TS_TX / 2)
     {
-        const int ly = i / TS_TX, lx = i - ly * TS_TX;
-        const int gx = x0 + lx;
-        const int lo = max(gx - TS_R, 0) - (x0 - TS_R), hi = min(gx + TS_R, W - 1) - (x0 - TS_R);
-        float mn[9], mx[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) mn[k] = SCR_BIG, mx[k] = -SCR_BIG;
-        for (int xx = lo; xx <= hi; ++xx)
-#pragma unroll
-            for (int k = 0; k < 9; ++k)
-            {
-                const float t = v[k][ly][xx];
-                mn[k] = fminf(mn[k], t);
-                mx[k] = fmaxf(mx[k], t);
-            }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) hmn[k][ly][lx] = mn[k], hmx[k][ly][lx] = mx[k];
-    }
-    __syncthreads();
-    if (!live) return;
-    const int lo = max(cy - TS_R, 0) - (y0 - TS_R), hi = min(cy + TS_R, H - 1) - (y0 - TS_R);
-    MinMax9 cur;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) cur.mn[k] = SCR_BIG, cur.mx[k] = -SCR_BIG;
-    for (int yy = lo; yy <= hi; ++yy)
+        const int ly = threadIdx.x / (TS_TX / 2), lx = 2 * (threadIdx.x % (TS_TX / 2));
 #pragma unroll
         for (int k = 0; k < 9; ++k)
         {
-            cur.mn[k] = fminf(cur.mn[k], hmn[k][yy][olx]);
-            cur.mx[k] = fmaxf(cur.mx[k], hmx[k][yy][olx]);
+            const float2 *r = reinterpret_cast<const float2 *>(&v[k][ly][lx]);
+            const float2 t0 = r[0], t1 = r[1], t2 = r[2], t3 = r[3];
+            const float cmn = fminf(fminf(fminf(t0.y, t1.x), t1.y), fminf(fminf(t2.x, t2.y), t3.x));
+            const float cmx = fmaxf(fmaxf(fmaxf(t0.y, t1.x), t1.y), fmaxf(fmaxf(t2.x, t2.y), t3.x));
+            *reinterpret_cast<float4 *>(&hm[k][ly][lx]) =
+                make_float4(fminf(cmn, t0.x), fmaxf(cmx, t0.x), fminf(cmn, t3.y), fmaxf(cmx, t3.y));
         }
+    }
+    __syncthreads();
+    if (!live) return;
+    MinMax9 cur;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+    {
+        float2 r[2 * TS_R + 1];
+#pragma unroll
+        for (int yy = 0; yy <= 2 * TS_R; ++yy) r[yy] = hm[k][oly + yy][olx];
+        cur.mn[k] = fminf(fminf(fminf(r[0].x, r[1].x), fminf(r[2].x, r[3].x)), fminf(fminf(r[4].x, r[5].x), r[6].x));
+        cur.mx[k] = fmaxf(fmaxf(fmaxf(r[0].y, r[1].y), fmaxf(r[2].y, r[3].y)), fmaxf(fmaxf(r[4].y, r[5].y), r[6].y));
+    }
     float sc = rs;
     if (b.fuse)
     {
