@@ -549,9 +549,18 @@ __global__ __launch_bounds__(256) void wgrad_reduce1(RedBatch b)
     if (e >= J.pstride) return;
     const int nb = J.nblk;
     const int lo = (int)((long)nb * sp / b.splits), hi = (int)((long)nb * (sp + 1) / b.splits);
+    // the loads of eight partials are issued together; the sum keeps the order lo, lo + 1, ...
     float s = 0.f;
     const float *p = J.part + e;
-    for (int i = lo; i < hi; ++i) s += p[(size_t)i * J.pstride];
+    for (int i0 = lo; i0 < hi; i0 += 8)
+    {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = i0 + k < hi ? p[(size_t)(i0 + k) * J.pstride] : 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (i0 + k < hi) s += v[k];
+    }
     J.stage[sp * J.pstride + e] = s;
 }
 
@@ -574,8 +583,13 @@ __global__ __launch_bounds__(256) void wgrad_reduce2(RedBatch b)
         src = J.compact ? e : 9 * 1024 + (e - nw);
         dst = J.taps * J.ci * J.co + (e - nw);
     }
+    float v[WG_SPLITS];
+#pragma unroll
+    for (int sp = 0; sp < WG_SPLITS; ++sp) v[sp] = sp < b.splits ? J.stage[sp * J.pstride + src] : 0.f;
     float s = 0.f;
-    for (int sp = 0; sp < b.splits; ++sp) s += J.stage[sp * J.pstride + src];
+#pragma unroll
+    for (int sp = 0; sp < WG_SPLITS; ++sp)
+        if (sp < b.splits) s += v[sp];
     J.grad[dst] = s;
 }
 

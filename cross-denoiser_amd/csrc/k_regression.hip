@@ -40,7 +40,7 @@ struct VarBatch {
     int log_in;
 };
 
-constexpr int VT_X = 32, VT_Y = 8;
+constexpr int VT_X = 32, VT_Y = 16;
 
 // log1p(x) = ln(u) * x / (u - 1) with u = fl(1 + x) (Goldberg 1991, "What every computer scientist
 // should know about floating-point arithmetic", Theorem 4): the rounding error of 1 + x cancels in
@@ -54,54 +54,75 @@ __device__ __forceinline__ float log1p_acc(float x)
     return (__builtin_amdgcn_logf(u) * 0.693147180559945309f) * (x * fast_rcp(u - 1.f));
 }
 
+// 32x16 pixels per workgroup, two per lane; the tile and its 1-px halo (zero outside the image)
+// are loaded with every load of a lane in flight at once. The neighbour sum runs over all eight
+// neighbours in the reference's order: the zeros of out-of-image taps leave it unchanged, and the
+// count of in-image neighbours follows from the position.
 __global__ __launch_bounds__(256) void variance_kernel(VarBatch b)
 {
-    __shared__ float s[3][VT_Y + 2][VT_X + 2];
+    constexpr int EX = VT_X + 2, EY = VT_Y + 2, NS = (EX * EY + 255) / 256;
+    __shared__ float s[3][EY][EX];
     const int tile = xcd_tile(blockIdx.x, b.map.total);
     const TileLoc L = locate(b.map, tile);
     const VarFrame &F = b.f[L.f];
     const int H = F.H, W = F.W;
     const int x0 = L.tx * VT_X, y0 = L.ty * VT_Y;
-    for (int i = threadIdx.x; i < (VT_X + 2) * (VT_Y + 2); i += 256)
+    float3 v[NS];
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
     {
-        const int ly = i / (VT_X + 2), lx = i - ly * (VT_X + 2);
+        const int i = threadIdx.x + 256 * u;
+        const int ly = i / EX, lx = i - ly * EX;
         const int gx = x0 + lx - 1, gy = y0 + ly - 1;
-        float3 v = make_float3(0.f, 0.f, 0.f);
-        if (gx >= 0 && gy >= 0 && gx < W && gy < H)
+        v[u] = make_float3(0.f, 0.f, 0.f);
+        if (i < EX * EY && gx >= 0 && gy >= 0 && gx < W && gy < H) v[u] = ld3(F.in, (size_t)gy * W + gx);
+    }
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
+    {
+        const int i = threadIdx.x + 256 * u;
+        if (i >= EX * EY) continue;
+        const int ly = i / EX, lx = i - ly * EX;
+        const int gx = x0 + lx - 1, gy = y0 + ly - 1;
+        float3 t = v[u];
+        if (b.log_in && gx >= 0 && gy >= 0 && gx < W && gy < H)
         {
-            v = ld3(F.in, (size_t)gy * W + gx);
-            if (b.log_in)
-            {
-                v = make_float3(log1p_acc(v.x), log1p_acc(v.y), log1p_acc(v.z));
-                if (lx >= 1 && ly >= 1 && lx <= VT_X && ly <= VT_Y) st3(F.y_out, (size_t)gy * W + gx, v);
-            }
+            t = make_float3(log1p_acc(t.x), log1p_acc(t.y), log1p_acc(t.z));
+            if (lx >= 1 && ly >= 1 && lx <= VT_X && ly <= VT_Y) st3(F.y_out, (size_t)gy * W + gx, t);
         }
-        s[0][ly][lx] = v.x;
-        s[1][ly][lx] = v.y;
-        s[2][ly][lx] = v.z;
+        s[0][ly][lx] = t.x;
+        s[1][ly][lx] = t.y;
+        s[2][ly][lx] = t.z;
     }
     __syncthreads();
-    const int lx = threadIdx.x % VT_X, ly = threadIdx.x / VT_X;
-    const int cx = x0 + lx, cy = y0 + ly;
-    if (cx >= W || cy >= H) return;
-    float acc[3] = {0.f, 0.f, 0.f};
-    int n = 0;
-    for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx)
-        {
-            const int gx = cx + dx, gy = cy + dy;
-            if ((dx | dy) == 0 || gx < 0 || gy < 0 || gx >= W || gy >= H) continue;
-            for (int k = 0; k < 3; ++k) acc[k] += s[k][ly + 1 + dy][lx + 1 + dx];
-            ++n;
-        }
-    const float inv = 1.f / (float)n;
-    float v = 0.f;
-    for (int k = 0; k < 3; ++k)
+    const int lx = threadIdx.x % VT_X;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
     {
-        const float d = acc[k] * inv - s[k][ly + 1][lx + 1];
-        v += d * d;
+        const int ly = threadIdx.x / VT_X + 8 * r;
+        const int cx = x0 + lx, cy = y0 + ly;
+        if (cx >= W || cy >= H) continue;
+        const int n = (1 + (cx > 0) + (cx < W - 1)) * (1 + (cy > 0) + (cy < H - 1)) - 1;
+        float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx)
+            {
+                if ((dx | dy) == 0) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[k] += s[k][ly + 1 + dy][lx + 1 + dx];
+            }
+        const float inv = 1.f / (float)n;
+        float var = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            const float d = acc[k] * inv - s[k][ly + 1][lx + 1];
+            var += d * d;
+        }
+        F.var[(size_t)cy * W + cx] = var;
     }
-    F.var[(size_t)cy * W + cx] = v;
 }
 
 // ---------------------------------------------------------------------------------------------
