@@ -83,6 +83,18 @@ __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
     const UcFrame &F = b.f[L.f];
     const int H = F.H, W = F.W;
     const int x0 = L.tx * UT_W, y0 = L.ty * UT_H;
+    // kernel fragments and bias first: their loads are in flight together with the tile's
+    u32x4 wf[9][KC][NT];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                wf[t][kc][nt] = static_cast<const u32x4 *>(F.wfrag)[((t * KC + kc) * NT + nt) * 64 + lane];
+    float bias[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) bias[nt] = EPI == UC_EPI_RELU ? F.bias[nt * 32 + px] : 0.f;
     if constexpr (SRC == UC_SRC_BF16)
     {
         constexpr int QA = CA / 8, QB = CB / 8, QP = CIP / 8;
@@ -130,17 +142,6 @@ __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
             *reinterpret_cast<u32x4 *>(&lds[pix * PS + 8]) = c;
         }
     }
-    u32x4 wf[9][KC][NT];
-#pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-        for (int kc = 0; kc < KC; ++kc)
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                wf[t][kc][nt] = static_cast<const u32x4 *>(F.wfrag)[((t * KC + kc) * NT + nt) * 64 + lane];
-    float bias[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) bias[nt] = EPI == UC_EPI_RELU ? F.bias[nt * 32 + px] : 0.f;
     __syncthreads();
 
     f32x16 acc[4][NT];
