@@ -190,6 +190,11 @@ __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
                 }
             }
         __syncthreads();
+        // head weights [8][6] + bias [6], read once (the stores below could alias them)
+        float hw[HEAD ? 54 : 1];
+        if constexpr (HEAD)
+#pragma unroll
+            for (int j = 0; j < 54; ++j) hw[j] = F.head_w[j];
         constexpr int QO = CO / 8;
         for (int i = tid; i < UT_H * UT_W * QO; i += 256)
         {
@@ -213,9 +218,9 @@ __global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
 #pragma unroll
                 for (int k = 0; k < 6; ++k)
                 {
-                    float sum = F.head_w[48 + k];
+                    float sum = hw[48 + k];
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) sum += in[c] * F.head_w[c * 6 + k];
+                    for (int c = 0; c < 8; ++c) sum += in[c] * hw[c * 6 + k];
                     po[k] = sum;
                 }
             }
