@@ -677,11 +677,14 @@ struct CtBatch {
 };
 static_assert(sizeof(CtBatch) <= 4096, "kernel argument too large");
 
-// forward: one thread per coarse pixel writes its 2x2 output block
+// forward: one thread per 8-channel (16-B) chunk of the output, consecutive threads on consecutive
+// chunks (coalesced stores). Chunks at even (row, column) hold W^T in + b of their coarse pixel
+// (bias first, then the inputs in order), every other chunk the bias.
 template <int C>
 __global__ __launch_bounds__(256) void convT1x1s2(CtBatch b)
 {
-    __shared__ float ws[C][C + 1];  // [co][ci]
+    constexpr int Q = C / 8;        // chunks per pixel
+    __shared__ float ws[C][C + 4];  // [co][ci]
     __shared__ float bs[C];
     int f = 0;
     while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
@@ -690,128 +693,133 @@ __global__ __launch_bounds__(256) void convT1x1s2(CtBatch b)
     if (threadIdx.x < C) bs[threadIdx.x] = F.b[threadIdx.x];
     __syncthreads();
     const int li = blockIdx.x - b.blk_begin[f];
-    const int p = li * 256 + threadIdx.x, h = F.h, w = F.w_;
-    if (p >= h * w) return;
-    const int y = p / w, x = p - y * w, W2 = 2 * w;
-    float in[C];
+    const int W2 = 2 * F.w_;
+    const long total = (long)4 * F.h * F.w_ * Q;
+    const long i = (long)li * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int q = (int)(i % Q);
+    const long pix = i / Q;
+    const int R = (int)(pix / W2), X = (int)(pix - (long)R * W2);
+    float o[8];
 #pragma unroll
-    for (int c = 0; c < C; c += 8)
+    for (int j = 0; j < 8; ++j) o[j] = bs[8 * q + j];
+    if (((R | X) & 1) == 0)
     {
-        const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * C + c);
-        in[c] = u_lo(v.x); in[c + 1] = u_hi(v.x); in[c + 2] = u_lo(v.y); in[c + 3] = u_hi(v.y);
-        in[c + 4] = u_lo(v.z); in[c + 5] = u_hi(v.z); in[c + 6] = u_lo(v.w); in[c + 7] = u_hi(v.w);
-    }
-    uint32_t outw[C / 2], biasw[C / 2];
+        const size_t p = (size_t)(R >> 1) * F.w_ + (X >> 1);
+        float in[C];
 #pragma unroll
-    for (int co = 0; co < C; co += 2)
-    {
-        float s0 = bs[co], s1 = bs[co + 1];
-#pragma unroll
-        for (int ci = 0; ci < C; ++ci)
+        for (int c = 0; c < C; c += 8)
         {
-            s0 += ws[co][ci] * in[ci];
-            s1 += ws[co + 1][ci] * in[ci];
+            const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + p * C + c);
+            in[c] = u_lo(v.x); in[c + 1] = u_hi(v.x); in[c + 2] = u_lo(v.y); in[c + 3] = u_hi(v.y);
+            in[c + 4] = u_lo(v.z); in[c + 5] = u_hi(v.z); in[c + 6] = u_lo(v.w); in[c + 7] = u_hi(v.w);
         }
-        outw[co / 2] = u_pack2(s0, s1);
-        biasw[co / 2] = u_pack2(bs[co], bs[co + 1]);
-    }
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-    {
-        uint16_t *o = F.out + ((size_t)(2 * y + (k >> 1)) * W2 + 2 * x + (k & 1)) * C;
+        for (int j = 0; j < 8; ++j)
 #pragma unroll
-        for (int c = 0; c < C / 2; c += 4)
-        {
-            const uint32_t *src = k == 0 ? outw : biasw;
-            *reinterpret_cast<u32x4 *>(o + 2 * c) = u32x4{src[c], src[c + 1], src[c + 2], src[c + 3]};
-        }
+            for (int ci = 0; ci < C; ci += 4)
+            {
+                const float4 wv = *reinterpret_cast<const float4 *>(&ws[8 * q + j][ci]);
+                o[j] += wv.x * in[ci];
+                o[j] += wv.y * in[ci + 1];
+                o[j] += wv.z * in[ci + 2];
+                o[j] += wv.w * in[ci + 3];
+            }
     }
+    *reinterpret_cast<u32x4 *>(F.out + pix * C + 8 * q) =
+        u32x4{u_pack2(o[0], o[1]), u_pack2(o[2], o[3]), u_pack2(o[4], o[5]), u_pack2(o[6], o[7])};
 }
 
-// backward: blocks walk coarse pixels in chunks of 256. Staging (one thread per coarse pixel):
-// dIn = W dOut(2y,2x) masked by the input's ReLU, and into LDS the input, dOut(2y,2x) and the sum
-// of dOut over the 2x2 output block. Then dW [ci][co] = sum in (x) dOut(2y,2x) in 4x4 register
-// tiles (lane groups split the chunk's pixels) and db = sum of the block sums; the groups are
-// reduced through LDS at the end.
+// backward: blocks walk coarse pixels in chunks of 256, one per thread. Each thread computes
+// dIn = W^T dOut(2y,2x) masked by the input's ReLU (FP32, output channels in order) and adds the
+// sum of dOut over its 2x2 output block to its bias-gradient sums; the chunk's inputs and
+// dOut(2y,2x) go to LDS as they are (bf16, pixel-major) and dW[ci][co] = sum in (x) dOut(2y,2x)
+// runs on MFMA (M = ci, N = co, K = pixels; operands through ds_read_b64_tr_b16), each wave taking
+// 64 pixels of the chunk. The waves' accumulators and the threads' bias sums are reduced through
+// LDS in a fixed order at the end.
 template <int C>
 __global__ __launch_bounds__(256) void convT_bwd(CtBatch b)
 {
-    constexpr int CB = C / 4;          // 4-channel blocks
-    constexpr int LPG = CB * CB;       // lanes per group: one 4x4 (ci, co) tile each
-    constexpr int G = 256 / LPG;       // groups, each taking every G-th pixel of a chunk
-    constexpr int PS = C + 4;          // LDS pixel stride (floats)
-    __shared__ __attribute__((aligned(16))) float xin[256 * PS];
-    __shared__ __attribute__((aligned(16))) float gin[256 * PS];
-    __shared__ __attribute__((aligned(16))) float gsum[256 * PS];
-    __shared__ __attribute__((aligned(16))) float ws[C * C];  // [co][ci]
-    static_assert(G * (C * C + C) <= 256 * PS, "group reduction buffer");
+    static_assert(C == 16 || C == 32, "channel count");
+    constexpr int Q = C / 8;
+    __shared__ __attribute__((aligned(16))) uint16_t tile[2][256 * C];  // inputs, dOut(2y,2x)
+    __shared__ __attribute__((aligned(16))) float ws[C * C];          // [co][ci]
+    __shared__ float red[4][C * C + C];
     int f = 0;
     while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
     const CtFrame &F = b.f[f];
     const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
     const int h = F.h, w = F.w_, np = h * w, W2 = 2 * w;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int e = tid; e < C * C; e += 256) ws[e] = F.w[e];
-    const int grp = tid / LPG, l = tid - grp * LPG, cib = l / CB, cob = l - cib * CB;
-    float acc[4][4], bacc[4];
+    float bacc[C];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-    {
-        bacc[i] = 0.f;
+    for (int c = 0; c < C; ++c) bacc[c] = 0.f;
+    typedef __attribute__((ext_vector_type(C == 32 ? 16 : 4))) float accv;
+    accv acc;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
-    }
+    for (int k = 0; k < (C == 32 ? 16 : 4); ++k) acc[k] = 0.f;
+    // transposing-read addressing (see ld_tr of k_cnn.hip): lane s of a 16-lane group addresses
+    // pixel +s/4, channels +4(s%4)
+    const int s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3;
     for (int base = li * 256; base < np; base += nblk * 256)
     {
-        __syncthreads();  // ws written / the previous chunk's LDS reads done
+        __syncthreads();  // ws written / the previous chunk's tiles read
         const int p = base + tid;
-        float x[C], g[C], gs[C];
+        u32x4 xv[Q], gv[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) xv[k] = gv[k] = u32x4{0u, 0u, 0u, 0u};
+        float gs[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) gs[c] = 0.f;
         if (p < np)
         {
             const int y = p / w, xx = p - y * w;
 #pragma unroll
-            for (int c = 0; c < C; c += 8)
-            {
-                const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * C + c);
-                x[c] = u_lo(v.x); x[c + 1] = u_hi(v.x); x[c + 2] = u_lo(v.y); x[c + 3] = u_hi(v.y);
-                x[c + 4] = u_lo(v.z); x[c + 5] = u_hi(v.z); x[c + 6] = u_lo(v.w); x[c + 7] = u_hi(v.w);
-            }
+            for (int k = 0; k < Q; ++k) xv[k] = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * C + 8 * k);
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+            for (int t = 0; t < 4; ++t)
             {
-                const uint16_t *gp = F.dout + ((size_t)(2 * y + (k >> 1)) * W2 + 2 * xx + (k & 1)) * F.dstride;
+                const uint16_t *gp = F.dout + ((size_t)(2 * y + (t >> 1)) * W2 + 2 * xx + (t & 1)) * F.dstride;
 #pragma unroll
-                for (int c = 0; c < C; c += 8)
+                for (int k = 0; k < Q; ++k)
                 {
-                    const u32x4 v = *reinterpret_cast<const u32x4 *>(gp + c);
-                    const float e[8] = {u_lo(v.x), u_hi(v.x), u_lo(v.y), u_hi(v.y),
-                                        u_lo(v.z), u_hi(v.z), u_lo(v.w), u_hi(v.w)};
+                    const u32x4 v = *reinterpret_cast<const u32x4 *>(gp + 8 * k);
+                    if (t == 0) gv[k] = v;
+                    const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                    for (int j = 0; j < 8; ++j)
+                    for (int j = 0; j < 4; ++j)
                     {
-                        if (k == 0)
-                            g[c + j] = gs[c + j] = e[j];
-                        else
-                            gs[c + j] += e[j];
+                        gs[8 * k + 2 * j] += u_lo(wd[j]);
+                        gs[8 * k + 2 * j + 1] += u_hi(wd[j]);
                     }
                 }
             }
         }
-        else
-        {
 #pragma unroll
-            for (int c = 0; c < C; ++c) x[c] = g[c] = gs[c] = 0.f;
+        for (int k = 0; k < Q; ++k)
+        {
+            *reinterpret_cast<u32x4 *>(&tile[0][tid * C + 8 * k]) = xv[k];
+            *reinterpret_cast<u32x4 *>(&tile[1][tid * C + 8 * k]) = gv[k];
         }
 #pragma unroll
-        for (int c = 0; c < C; c += 4)
-        {
-            *reinterpret_cast<float4 *>(&xin[tid * PS + c]) = float4{x[c], x[c + 1], x[c + 2], x[c + 3]};
-            *reinterpret_cast<float4 *>(&gin[tid * PS + c]) = float4{g[c], g[c + 1], g[c + 2], g[c + 3]};
-            *reinterpret_cast<float4 *>(&gsum[tid * PS + c]) = float4{gs[c], gs[c + 1], gs[c + 2], gs[c + 3]};
-        }
+        for (int c = 0; c < C; ++c) bacc[c] += gs[c];
         __syncthreads();  // ws (first chunk) and this chunk's tiles visible
         if (p < np)
         {
+            float x[C], g[C];
+#pragma unroll
+            for (int k = 0; k < Q; ++k)
+            {
+                const uint32_t xw[4] = {xv[k].x, xv[k].y, xv[k].z, xv[k].w};
+                const uint32_t gw[4] = {gv[k].x, gv[k].y, gv[k].z, gv[k].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    x[8 * k + 2 * j] = u_lo(xw[j]); x[8 * k + 2 * j + 1] = u_hi(xw[j]);
+                    g[8 * k + 2 * j] = u_lo(gw[j]); g[8 * k + 2 * j + 1] = u_hi(gw[j]);
+                }
+            }
             float d[C];
 #pragma unroll
             for (int c = 0; c < C; ++c) d[c] = 0.f;
@@ -820,8 +828,8 @@ __global__ __launch_bounds__(256) void convT_bwd(CtBatch b)
 #pragma unroll
                 for (int c = 0; c < C; c += 4)
                 {
-                    const float4 wv = *reinterpret_cast<const float4 *>(&ws[co * C + c]);
-                    d[c] += wv.x * g[co]; d[c + 1] += wv.y * g[co]; d[c + 2] += wv.z * g[co]; d[c + 3] += wv.w * g[co];
+                    const float4 wq = *reinterpret_cast<const float4 *>(&ws[co * C + c]);
+                    d[c] += wq.x * g[co]; d[c + 1] += wq.y * g[co]; d[c + 2] += wq.z * g[co]; d[c + 3] += wq.w * g[co];
                 }
 #pragma unroll
             for (int c = 0; c < C; c += 8)
@@ -831,42 +839,59 @@ __global__ __launch_bounds__(256) void convT_bwd(CtBatch b)
                           u_pack2(x[c + 4] > 0.f ? d[c + 4] : 0.f, x[c + 5] > 0.f ? d[c + 5] : 0.f),
                           u_pack2(x[c + 6] > 0.f ? d[c + 6] : 0.f, x[c + 7] > 0.f ? d[c + 7] : 0.f)};
         }
-        const int cnt = min(256, np - base);
-        for (int q = grp; q < cnt; q += G)
+        // dW: pixels past the end of the image were staged as zeros
+        if constexpr (C == 32)
         {
-            const float4 xv = *reinterpret_cast<const float4 *>(&xin[q * PS + 4 * cib]);
-            const float4 gv = *reinterpret_cast<const float4 *>(&gin[q * PS + 4 * cob]);
-            const float xa[4] = {xv.x, xv.y, xv.z, xv.w}, ga[4] = {gv.x, gv.y, gv.z, gv.w};
+            // 32x32x16: lane l holds row ci = l%32, k = pixels 8(l/32) + 0..7 of the K step
+            const int grp = (lane >> 4) & 1, hh = lane >> 5;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += xa[i] * ga[j];
-            if (cib == 0)
+            for (int ks = 0; ks < 4; ++ks)
             {
-                const float4 sv = *reinterpret_cast<const float4 *>(&gsum[q * PS + 4 * cob]);
-                bacc[0] += sv.x; bacc[1] += sv.y; bacc[2] += sv.z; bacc[3] += sv.w;
+                const int pb = 64 * wv + 16 * ks + 8 * hh + jq;
+                const uint16_t *xp = &tile[0][pb * C + 16 * grp + 4 * qq];
+                const uint16_t *gp = &tile[1][pb * C + 16 * grp + 4 * qq];
+                const uint2 a0 = u_ld_tr(xp), a1 = u_ld_tr(xp + 4 * C);
+                const uint2 b0 = u_ld_tr(gp), b1 = u_ld_tr(gp + 4 * C);
+                acc = u_mfma(u32x4{a0.x, a0.y, a1.x, a1.y}, u32x4{b0.x, b0.y, b1.x, b1.y}, acc);
+            }
+        }
+        else
+        {
+            // 16x16x32: lane l holds row ci = l%16, k = pixels 8(l/16) + 0..7 of the K step
+            const int kg = lane >> 4;
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks)
+            {
+                const int pb = 64 * wv + 32 * ks + 8 * kg + jq;
+                const uint16_t *xp = &tile[0][pb * C + 4 * qq];
+                const uint16_t *gp = &tile[1][pb * C + 4 * qq];
+                const uint2 a0 = u_ld_tr(xp), a1 = u_ld_tr(xp + 4 * C);
+                const uint2 b0 = u_ld_tr(gp), b1 = u_ld_tr(gp + 4 * C);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y}),
+                    __builtin_bit_cast(bf16x8, u32x4{b0.x, b0.y, b1.x, b1.y}), acc, 0, 0, 0);
             }
         }
     }
-    // reduce the G groups: red[grp][ci * C + co], bias at red[grp][C * C + co]
-    __syncthreads();
-    float *red = xin;
+    // bias sums: wave reduction in a fixed order, then the four waves
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int c = 0; c < C; ++c)
+        for (int off = 32; off > 0; off >>= 1) bacc[c] += __shfl_down(bacc[c], off);
+    // accumulators: C[row = ci][col = co]
 #pragma unroll
-        for (int j = 0; j < 4; ++j) red[grp * (C * C + C) + (4 * cib + i) * C + 4 * cob + j] = acc[i][j];
-    if (cib == 0)
+    for (int k = 0; k < (C == 32 ? 16 : 4); ++k)
+    {
+        const int ci = C == 32 ? 8 * (k >> 2) + 4 * (lane >> 5) + (k & 3) : 4 * (lane >> 4) + k;
+        const int co = C == 32 ? (lane & 31) : (lane & 15);
+        red[wv][ci * C + co] = acc[k];
+    }
+    if (lane == 0)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) red[grp * (C * C + C) + C * C + 4 * cob + j] = bacc[j];
+        for (int c = 0; c < C; ++c) red[wv][C * C + c] = bacc[c];
     __syncthreads();
     // compact partial: dW [ci][co], then db
     float *out = F.part + (size_t)li * compact_part(1, C, C);
-    for (int e = tid; e < C * C + C; e += 256)
-    {
-        float sum = 0.f;
-        for (int k = 0; k < G; ++k) sum += red[k * (C * C + C) + e];
-        out[e] = sum;
-    }
+    for (int e = tid; e < C * C + C; e += 256) out[e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
 }
 
 template <int C, bool BWD>
@@ -879,7 +904,7 @@ static hipError_t convT_t(int n, const CtFrame *fr, hipStream_t st)
     {
         b.f[i] = fr[i];
         b.blk_begin[i] = acc;
-        acc += BWD ? fr[i].nblk : udiv_up(fr[i].h * fr[i].w_, 256);
+        acc += BWD ? fr[i].nblk : udiv_up(4 * fr[i].h * fr[i].w_ * (C / 8), 256);
     }
     b.blk_begin[n] = acc;
     if (acc == 0) return hipSuccess;
