@@ -98,8 +98,19 @@ __global__ __launch_bounds__(256) void sum_kernel(SumBatch b)
 {
     __shared__ float red[256];
     const int f = blockIdx.x;
+    // eight loads in flight per thread; the sum keeps the order i, i + 256, ...
     float s = 0.f;
-    for (int i = threadIdx.x; i < b.count[f]; i += 256) s += b.parts[f][i];
+    const int cnt = b.count[f];
+    const float *pp = b.parts[f];
+    for (int i0 = threadIdx.x; i0 < cnt; i0 += 8 * 256)
+    {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = i0 + 256 * k < cnt ? pp[i0 + 256 * k] : 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (i0 + 256 * k < cnt) s += v[k];
+    }
     red[threadIdx.x] = s;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1)

@@ -677,57 +677,86 @@ struct CtBatch {
 };
 static_assert(sizeof(CtBatch) <= 4096, "kernel argument too large");
 
-// forward: one thread per 8-channel (16-B) chunk of the output, consecutive threads on consecutive
-// chunks (coalesced stores). Chunks at even (row, column) hold W^T in + b of their coarse pixel
-// (bias first, then the inputs in order), every other chunk the bias.
+// forward: out(2y,2x) = W in(y,x) + b, the other three outputs of the 2x2 block = b. FP32 MFMA
+// (v_mfma_f32_32x32x2_f32: M = output channels, N = 32 coarse pixels, K = 2 input channels per
+// step; a wave takes four such pixel groups with the weights in registers) on the FP32 master
+// weights, so the products are those of FP32 arithmetic;
+// a lane ends with 16 (C = 32) or 8 (C = 16) channels c(i) = 8(i/4) + 4h + i%4 of its pixel, and a
+// permlane32 swap gives it 8 consecutive channels per 16-B store.
 template <int C>
 __global__ __launch_bounds__(256) void convT1x1s2(CtBatch b)
 {
-    constexpr int Q = C / 8;        // chunks per pixel
-    __shared__ float ws[C][C + 4];  // [co][ci]
-    __shared__ float bs[C];
+    constexpr int NI = C == 32 ? 16 : 8;  // accumulator rows holding output channels
+    constexpr int NG = NI / 8;            // 16-B stores per output pixel and lane
+    constexpr int GPW = 4;                // 32-pixel groups per wave
     int f = 0;
     while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
     const CtFrame &F = b.f[f];
-    for (int e = threadIdx.x; e < C * C; e += 256) ws[e / C][e % C] = F.w[e];
-    if (threadIdx.x < C) bs[threadIdx.x] = F.b[threadIdx.x];
-    __syncthreads();
     const int li = blockIdx.x - b.blk_begin[f];
-    const int W2 = 2 * F.w_;
-    const long total = (long)4 * F.h * F.w_ * Q;
-    const long i = (long)li * 256 + threadIdx.x;
-    if (i >= total) return;
-    const int q = (int)(i % Q);
-    const long pix = i / Q;
-    const int R = (int)(pix / W2), X = (int)(pix - (long)R * W2);
-    float o[8];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, px = lane & 31, h = lane >> 5;
+    const int w = F.w_, np = F.h * w, W2 = 2 * w;
+    // A operand of step s: W[co = px][ci = 2s + h]; the bias of the lane's outputs
+    float wa[C / 2], bo[NI];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) o[j] = bs[8 * q + j];
-    if (((R | X) & 1) == 0)
+    for (int s2 = 0; s2 < C / 2; ++s2) wa[s2] = px < C ? F.w[px * C + 2 * s2 + h] : 0.f;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) bo[i] = F.b[8 * (i >> 2) + 4 * h + (i & 3)];
+    u32x4 vb[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
     {
-        const size_t p = (size_t)(R >> 1) * F.w_ + (X >> 1);
-        float in[C];
-#pragma unroll
-        for (int c = 0; c < C; c += 8)
-        {
-            const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + p * C + c);
-            in[c] = u_lo(v.x); in[c + 1] = u_hi(v.x); in[c + 2] = u_lo(v.y); in[c + 3] = u_hi(v.y);
-            in[c + 4] = u_lo(v.z); in[c + 5] = u_hi(v.z); in[c + 6] = u_lo(v.w); in[c + 7] = u_hi(v.w);
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-#pragma unroll
-            for (int ci = 0; ci < C; ci += 4)
-            {
-                const float4 wv = *reinterpret_cast<const float4 *>(&ws[8 * q + j][ci]);
-                o[j] += wv.x * in[ci];
-                o[j] += wv.y * in[ci + 1];
-                o[j] += wv.z * in[ci + 2];
-                o[j] += wv.w * in[ci + 3];
-            }
+        const int cb = 16 * g + 8 * h;
+        vb[g] = u32x4{u_pack2(F.b[cb], F.b[cb + 1]), u_pack2(F.b[cb + 2], F.b[cb + 3]),
+                      u_pack2(F.b[cb + 4], F.b[cb + 5]), u_pack2(F.b[cb + 6], F.b[cb + 7])};
     }
-    *reinterpret_cast<u32x4 *>(F.out + pix * C + 8 * q) =
-        u32x4{u_pack2(o[0], o[1]), u_pack2(o[2], o[3]), u_pack2(o[4], o[5]), u_pack2(o[6], o[7])};
+    for (int gi = 0; gi < GPW; ++gi)
+    {
+        const int p = ((li * 4 + wv) * GPW + gi) * 32 + px;
+        if (p - px >= np) break;  // uniform across the wave
+        // B operand: in[p][ci = 2s + h]
+        float xb[C / 2];
+#pragma unroll
+        for (int s2 = 0; s2 < C / 2; ++s2) xb[s2] = 0.f;
+        if (p < np)
+#pragma unroll
+            for (int c = 0; c < C; c += 8)
+            {
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(F.in + (size_t)p * C + c);
+                const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xb[c / 2 + j] = h ? u_hi(wd[j]) : u_lo(wd[j]);
+            }
+        f32x16 acc;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[k] = 0.f;
+#pragma unroll
+        for (int s2 = 0; s2 < C / 2; ++s2) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[s2], xb[s2], acc, 0, 0, 0);
+        uint32_t d[NI / 2];
+#pragma unroll
+        for (int q = 0; q < NI / 2; ++q) d[q] = u_pack2(acc[2 * q] + bo[2 * q], acc[2 * q + 1] + bo[2 * q + 1]);
+        // lane h gathers channels 16g + 8h .. +7 (see conv3x3 of k_cnn.hip)
+        u32x4 v[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+            {
+                const auto sw = __builtin_amdgcn_permlane32_swap(d[4 * g + e], d[4 * g + 2 + e], false, false);
+                v[g][e] = (uint32_t)sw[0];
+                v[g][2 + e] = (uint32_t)sw[1];
+            }
+        if (p < np)
+        {
+            const int y = p / w, x = p - y * w;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+            {
+                uint16_t *o = F.out + ((size_t)(2 * y + (k >> 1)) * W2 + 2 * x + (k & 1)) * C + 8 * h;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) *reinterpret_cast<u32x4 *>(o + 16 * g) = k == 0 ? v[g] : vb[g];
+            }
+        }
+    }
 }
 
 // backward: blocks walk coarse pixels in chunks of 256, one per thread. Each thread computes
@@ -904,7 +933,7 @@ static hipError_t convT_t(int n, const CtFrame *fr, hipStream_t st)
     {
         b.f[i] = fr[i];
         b.blk_begin[i] = acc;
-        acc += BWD ? fr[i].nblk : udiv_up(4 * fr[i].h * fr[i].w_ * (C / 8), 256);
+        acc += BWD ? fr[i].nblk : udiv_up(fr[i].h * fr[i].w_, 4 * 4 * 32);
     }
     b.blk_begin[n] = acc;
     if (acc == 0) return hipSuccess;
