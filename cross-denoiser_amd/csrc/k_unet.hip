@@ -759,28 +759,35 @@ __global__ __launch_bounds__(256) void convT1x1s2(CtBatch b)
     }
 }
 
-// backward: blocks walk coarse pixels in chunks of 256, one per thread. Each thread computes
-// dIn = W^T dOut(2y,2x) masked by the input's ReLU (FP32, output channels in order) and adds the
-// sum of dOut over its 2x2 output block to its bias-gradient sums; the chunk's inputs and
-// dOut(2y,2x) go to LDS as they are (bf16, pixel-major) and dW[ci][co] = sum in (x) dOut(2y,2x)
-// runs on MFMA (M = ci, N = co, K = pixels; operands through ds_read_b64_tr_b16), each wave taking
-// 64 pixels of the chunk. The waves' accumulators and the threads' bias sums are reduced through
-// LDS in a fixed order at the end.
+// backward: blocks walk coarse pixels in chunks of 256, one per thread for the loads: the chunk's
+// inputs and dOut(2y,2x) go to LDS as they are (bf16, pixel-major) and each thread adds the sum of
+// dOut over its 2x2 output block to its bias-gradient sums. Each wave then takes 64 pixels of the
+// chunk: dIn = W^T dOut(2y,2x) on FP32 MFMA for 32 channels (FP32 VALU for 16), FP32 master
+// weights, masked by the input's ReLU; dW[ci][co] = sum in (x) dOut(2y,2x) on bf16 MFMA (M = ci,
+// N = co, K = pixels; operands through ds_read_b64_tr_b16). The waves' accumulators and the
+// threads' bias sums are reduced through LDS in a fixed order at the end.
 template <int C>
 __global__ __launch_bounds__(256) void convT_bwd(CtBatch b)
 {
     static_assert(C == 16 || C == 32, "channel count");
     constexpr int Q = C / 8;
     __shared__ __attribute__((aligned(16))) uint16_t tile[2][256 * C];  // inputs, dOut(2y,2x)
-    __shared__ __attribute__((aligned(16))) float ws[C * C];          // [co][ci]
     __shared__ float red[4][C * C + C];
+    __shared__ __attribute__((aligned(16))) float ws[C == 16 ? C * C : 4];  // [co][ci] (VALU dIn)
     int f = 0;
     while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
     const CtFrame &F = b.f[f];
     const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
     const int h = F.h, w = F.w_, np = h * w, W2 = 2 * w;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int e = tid; e < C * C; e += 256) ws[e] = F.w[e];
+    constexpr int NI = C == 32 ? 16 : 8;  // dIn accumulator rows holding input channels
+    // C = 32: A operand of the dIn product, step s: W^T[ci = lane%32][co = 2s + lane/32]
+    float wt[C == 32 ? C / 2 : 1];
+    if constexpr (C == 32)
+#pragma unroll
+        for (int s2 = 0; s2 < C / 2; ++s2) wt[s2] = F.w[(2 * s2 + (lane >> 5)) * C + (lane & 31)];
+    else
+        for (int e = tid; e < C * C; e += 256) ws[e] = F.w[e];
     float bacc[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) bacc[c] = 0.f;
@@ -793,7 +800,7 @@ __global__ __launch_bounds__(256) void convT_bwd(CtBatch b)
     const int s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3;
     for (int base = li * 256; base < np; base += nblk * 256)
     {
-        __syncthreads();  // ws written / the previous chunk's tiles read
+        __syncthreads();  // the previous chunk's tiles are read
         const int p = base + tid;
         u32x4 xv[Q], gv[Q];
 #pragma unroll
@@ -833,40 +840,91 @@ __global__ __launch_bounds__(256) void convT_bwd(CtBatch b)
         }
 #pragma unroll
         for (int c = 0; c < C; ++c) bacc[c] += gs[c];
-        __syncthreads();  // ws (first chunk) and this chunk's tiles visible
-        if (p < np)
+        __syncthreads();  // this chunk's tiles are visible
+        // dIn. C = 32: FP32 MFMA (A = W^T from registers, B = dOut(2y,2x) of the wave's 32-pixel group
+        // from the tile), masked by the staged input and stored through the permlane32 swap. C = 16
+        // (half the MFMA rows would be padding): FP32 VALU per thread, output channels in order.
+        if constexpr (C == 16)
         {
-            float x[C], g[C];
+            if (p < np)
+            {
+                float x[C], g[C];
+#pragma unroll
+                for (int k = 0; k < Q; ++k)
+                {
+                    const uint32_t xw[4] = {xv[k].x, xv[k].y, xv[k].z, xv[k].w};
+                    const uint32_t gw[4] = {gv[k].x, gv[k].y, gv[k].z, gv[k].w};
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                    {
+                        x[8 * k + 2 * j] = u_lo(xw[j]); x[8 * k + 2 * j + 1] = u_hi(xw[j]);
+                        g[8 * k + 2 * j] = u_lo(gw[j]); g[8 * k + 2 * j + 1] = u_hi(gw[j]);
+                    }
+                }
+                float d[C];
+#pragma unroll
+                for (int c = 0; c < C; ++c) d[c] = 0.f;
+#pragma unroll
+                for (int co = 0; co < C; ++co)
+#pragma unroll
+                    for (int c = 0; c < C; c += 4)
+                    {
+                        const float4 wq = *reinterpret_cast<const float4 *>(&ws[co * C + c]);
+                        d[c] += wq.x * g[co]; d[c + 1] += wq.y * g[co]; d[c + 2] += wq.z * g[co]; d[c + 3] += wq.w * g[co];
+                    }
+#pragma unroll
+                for (int c = 0; c < C; c += 8)
+                    *reinterpret_cast<u32x4 *>(F.din + (size_t)p * C + c) =
+                        u32x4{u_pack2(x[c] > 0.f ? d[c] : 0.f, x[c + 1] > 0.f ? d[c + 1] : 0.f),
+                              u_pack2(x[c + 2] > 0.f ? d[c + 2] : 0.f, x[c + 3] > 0.f ? d[c + 3] : 0.f),
+                              u_pack2(x[c + 4] > 0.f ? d[c + 4] : 0.f, x[c + 5] > 0.f ? d[c + 5] : 0.f),
+                              u_pack2(x[c + 6] > 0.f ? d[c + 6] : 0.f, x[c + 7] > 0.f ? d[c + 7] : 0.f)};
+            }
+        }
+        else
+#pragma unroll
+        for (int gq = 0; gq < 2; ++gq)
+        {
+            const int pl = 64 * wv + 32 * gq + (lane & 31), hh = lane >> 5;
+            float xb[C / 2];
 #pragma unroll
             for (int k = 0; k < Q; ++k)
             {
-                const uint32_t xw[4] = {xv[k].x, xv[k].y, xv[k].z, xv[k].w};
-                const uint32_t gw[4] = {gv[k].x, gv[k].y, gv[k].z, gv[k].w};
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(&tile[1][pl * C + 8 * k]);
+                const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                {
-                    x[8 * k + 2 * j] = u_lo(xw[j]); x[8 * k + 2 * j + 1] = u_hi(xw[j]);
-                    g[8 * k + 2 * j] = u_lo(gw[j]); g[8 * k + 2 * j + 1] = u_hi(gw[j]);
-                }
+                for (int j = 0; j < 4; ++j) xb[4 * k + j] = hh ? u_hi(wd[j]) : u_lo(wd[j]);
             }
-            float d[C];
+            f32x16 dacc;
 #pragma unroll
-            for (int c = 0; c < C; ++c) d[c] = 0.f;
+            for (int k = 0; k < 16; ++k) dacc[k] = 0.f;
 #pragma unroll
-            for (int co = 0; co < C; ++co)
+            for (int s2 = 0; s2 < C / 2; ++s2) dacc = __builtin_amdgcn_mfma_f32_32x32x2f32(wt[s2], xb[s2], dacc, 0, 0, 0);
+            // rows c(i) = 8(i/4) + 4hh + i%4: four bf16 inputs at 8j + 4hh per j
+            uint32_t d[NI / 2];
 #pragma unroll
-                for (int c = 0; c < C; c += 4)
+            for (int j = 0; j < NI / 4; ++j)
+            {
+                const uint2 xm = *reinterpret_cast<const uint2 *>(&tile[0][pl * C + 8 * j + 4 * hh]);
+                const float xs[4] = {u_lo(xm.x), u_hi(xm.x), u_lo(xm.y), u_hi(xm.y)};
+                d[2 * j] = u_pack2(xs[0] > 0.f ? dacc[4 * j] : 0.f, xs[1] > 0.f ? dacc[4 * j + 1] : 0.f);
+                d[2 * j + 1] = u_pack2(xs[2] > 0.f ? dacc[4 * j + 2] : 0.f, xs[3] > 0.f ? dacc[4 * j + 3] : 0.f);
+            }
+            u32x4 v[NI / 8];
+#pragma unroll
+            for (int g = 0; g < NI / 8; ++g)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
                 {
-                    const float4 wq = *reinterpret_cast<const float4 *>(&ws[co * C + c]);
-                    d[c] += wq.x * g[co]; d[c + 1] += wq.y * g[co]; d[c + 2] += wq.z * g[co]; d[c + 3] += wq.w * g[co];
+                    const auto sw = __builtin_amdgcn_permlane32_swap(d[4 * g + e], d[4 * g + 2 + e], false, false);
+                    v[g][e] = (uint32_t)sw[0];
+                    v[g][2 + e] = (uint32_t)sw[1];
                 }
+            const int pp = base + pl;
+            if (pp < np)
 #pragma unroll
-            for (int c = 0; c < C; c += 8)
-                *reinterpret_cast<u32x4 *>(F.din + (size_t)p * C + c) =
-                    u32x4{u_pack2(x[c] > 0.f ? d[c] : 0.f, x[c + 1] > 0.f ? d[c + 1] : 0.f),
-                          u_pack2(x[c + 2] > 0.f ? d[c + 2] : 0.f, x[c + 3] > 0.f ? d[c + 3] : 0.f),
-                          u_pack2(x[c + 4] > 0.f ? d[c + 4] : 0.f, x[c + 5] > 0.f ? d[c + 5] : 0.f),
-                          u_pack2(x[c + 6] > 0.f ? d[c + 6] : 0.f, x[c + 7] > 0.f ? d[c + 7] : 0.f)};
+                for (int g = 0; g < NI / 8; ++g)
+                    *reinterpret_cast<u32x4 *>(F.din + (size_t)pp * C + 16 * g + 8 * hh) = v[g];
         }
         // dW: pixels past the end of the image were staged as zeros
         if constexpr (C == 32)
