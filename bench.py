@@ -26,7 +26,7 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 PUBLISHED_FPS = 1000.0 / 22.26  # paper Table 3 total per 1080p frame (RTX 4090)
 
-# Algorithmic work per pixel (both halves where applicable), FMA = 2 flops; see DESIGN.md.
+# Algorithmic work per pixel (both halves where applicable), FMA = 2 flops.
 ALGO = {
     "regression": {"flop_per_px": 5200.0, "bytes_per_px": 65.0},
     "filter_train": {"flop_per_px": 14700.0, "bytes_per_px": 152.0},
@@ -281,8 +281,8 @@ def cpu_baseline(torch, H, W, repeats=5):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by render stream, "
-              "see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path would shard by render "
+              "stream, one process per GPU, with no communication inside a frame", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

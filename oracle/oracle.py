@@ -1,8 +1,8 @@
 """FP64 oracle of cross-denoiser's per-frame filtering path.
 
 TEST INFRASTRUCTURE ONLY. Imported by tests/, by the smoke check of __graft_entry__ and by the
-cpu_baseline leg of bench.py, and only as the checker. The product (the ``xdn`` package and
-libxdn.so) never imports it.
+cpu_baseline leg of bench.py, and only to check results against or to time as the CPU baseline.
+The product (the ``xdn`` package and libxdn.so) never imports it.
 
 The stencil kernels are a C restatement (``xdn_oracle.c``, built into ``liboracle.so`` by the
 Makefile next to this file); the pointwise parts of ``scripts/main.py`` are restated here in numpy
@@ -10,7 +10,8 @@ float64. All functions take NHWC float32 numpy arrays ``[B,H,W,C]`` (the same la
 reference ops) and return float64 arrays unless the output is a mask or a selection, which is
 returned bit-exact in its reference dtype.
 
-Parity pinning of each function is documented in DESIGN.md and exercised by tests/test_oracle.py.
+Each function is pinned by tests/test_oracle.py (the reference's golden vectors where they exist,
+closed forms, invariants and brute force on small inputs).
 """
 from __future__ import annotations
 

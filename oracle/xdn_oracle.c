@@ -1,9 +1,9 @@
 /*
  * xdn_oracle.c -- FP64 host restatement of cross-denoiser's per-frame filtering path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/, the smoke check of
- * __graft_entry__ and the cpu_baseline leg of bench.py load it. The product library (libxdn.so)
- * never links, calls or falls back to it.
+ * TEST INFRASTRUCTURE ONLY. This library is the oracle results are checked against: only tests/,
+ * the smoke check of __graft_entry__ and the cpu_baseline leg of bench.py load it. The product
+ * library (libxdn.so) never links, calls or falls back to it.
  *
  * Every value path is computed in double precision from the FP32 inputs, following the kernel
  * bodies of the reference (file:line cited per function, paths relative to the reference root).
