@@ -4,15 +4,21 @@ Per frame: the online training step (xdn.pipeline.Denoiser: frame-0 zero history
 main.py:323-330, the history handoff of :338-341), then the report of :344-347 -- the displayed
 frame out + emissive + envLight, the reference ref + ref_emissive + ref_envLight and their RelL2
 (main.py:191-196) -- in one fused pass (xdn_frame_error), and the reference's log line
-``[Frame %3d] relL2: %.4f (...)``. Writing EXR images is not part of this module.
+``[Frame %3d] relL2: %.4f (...)``. With an output directory, the images of main.py:350-359
+(olsavg, out, ref, path) are collected per frame and written at the end as
+``{name}_{frame:04d}.exr`` (ZIP-compressed FLOAT R, G, B; out/ols/ref clamped at zero,
+main.py:150-167).
 
-    python -m xdn.driver --height 1080 --width 1920 --frames 16 [--network unet]
-    python -m xdn.driver --scene <dir of {type}_{frame:04d}.npy> [--frames 101]
+    python -m xdn.driver --height 1080 --width 1920 --frames 16 [--network unet] [--out-dir DIR]
+    python -m xdn.driver --scene <dir of {type}_{frame:04d}.npy or .exr> [--frames 101]
 """
 from __future__ import annotations
 
+import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 
+import numpy as np
 import torch
 
 from . import pipeline
@@ -70,6 +76,43 @@ class FrameDriver:
         return errs
 
 
+def frame_images(frame: dict, result: dict) -> dict:
+    """The images main.py:350-359 saves for one frame, as [H,W,3] float32 host arrays: olsavg (the
+    two cross-regression outputs back from log space, averaged, plus emissive and envLight), out (the
+    displayed frame), ref (with its emissive and envLight) and path (the mean of the two half-sample
+    colours plus emissive and envLight)."""
+    zero = torch.zeros_like(frame["color"])
+    e, l = frame.get("emissive", zero), frame.get("envLight", zero)
+    ols1 = torch.clamp(torch.expm1(result["ols1"]), min=0.0)
+    ols2 = torch.clamp(torch.expm1(result["ols2"]), min=0.0)
+    imgs = {"olsavg": (ols1 + ols2) * 0.5 + e + l, "out": result["shown"],
+            "ref": frame["ref"] + frame.get("ref_emissive", zero) + frame.get("ref_envLight", zero),
+            "path": (frame["color"] + frame["color2"]) * 0.5 + e + l}
+    return {k: v[0].detach().float().cpu().numpy() for k, v in imgs.items()}
+
+
+def write_image(path: str, img: np.ndarray) -> None:
+    """main.py:150-167: out/ols/ref images clamped at zero; R, G, B names up to three channels, ch{i}
+    beyond; ZIP compression, FLOAT."""
+    from . import exr
+    name = os.path.basename(path)
+    if any(k in name for k in ("out", "ols", "ref")):
+        img = np.maximum(0.0, img)
+    names = None if img.shape[-1] <= 3 else [f"ch{i}" for i in range(img.shape[-1])]
+    exr.write(path, img, channel_names=names, compression=exr.ZIP_COMPRESSION)
+
+
+def write_images(out_dir: str, images, workers: int = 8) -> int:
+    """Writes [(name, frame, image)] as {out_dir}/{name}_{frame:04d}.exr, highest frame first
+    (main.py:169-184); returns the number of files."""
+    os.makedirs(out_dir, exist_ok=True)
+    jobs = sorted(((os.path.join(out_dir, f"{n}_{i:04d}.exr"), img) for n, i, img in images),
+                  key=lambda j: int(j[0].rsplit("_", 1)[1].split(".")[0]), reverse=True)
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        list(pool.map(lambda j: write_image(*j), jobs))
+    return len(jobs)
+
+
 def main(argv=None):
     import argparse
 
@@ -82,22 +125,39 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--graphs", action="store_true", help="replay one captured step graph per frame")
     ap.add_argument("--scene", type=str, default=None,
-                    help="directory of {type}_{frame:04d}.npy buffers (the reference's converted dataset) "
-                         "instead of the synthetic stream")
+                    help="directory of {type}_{frame:04d}.npy buffers (the reference's converted dataset), or of "
+                         "the .exr files, instead of the synthetic stream")
+    ap.add_argument("--out-dir", type=str, default=None,
+                    help="write olsavg/out/ref/path_{frame:04d}.exr of every frame there (main.py --save_img)")
     a = ap.parse_args(argv)
     dev = torch.device("cuda:0")
+    images = []
     if a.scene:
         from . import data
-        scene = data.NpyScene(a.scene, max_num_frames=a.frames, device=dev)
+        try:
+            scene = data.NpyScene(a.scene, max_num_frames=a.frames, device=dev)
+        except FileNotFoundError:
+            scene = data.ExrScene(a.scene, max_num_frames=a.frames, device=dev)
         first_id, first = scene[0]
         H, W = first["color"].shape[1:3]
         drv = FrameDriver(H, W, device=dev, network=a.network, use_graphs=a.graphs)
-        drv.run((f for _, f in scene), first_index=first_id)
-        return
-    spec = synth.StreamSpec(a.height, a.width, seed=a.seed, pan=None, pan_max=3.0, disocclusion=0.10)
-    drv = FrameDriver(a.height, a.width, device=dev, network=a.network, use_graphs=a.graphs)
-    frames = (synth.make_frame(spec, t, device=dev, composition=True) for t in range(a.frames))
-    drv.run(frames)
+        frames = ((fid, f) for fid, f in scene)
+    else:
+        spec = synth.StreamSpec(a.height, a.width, seed=a.seed, pan=None, pan_max=3.0, disocclusion=0.10)
+        drv = FrameDriver(a.height, a.width, device=dev, network=a.network, use_graphs=a.graphs)
+        frames = ((t, synth.make_frame(spec, t, device=dev, composition=True)) for t in range(a.frames))
+    for fid, f in frames:
+        t0 = time.perf_counter()
+        r = drv.step(f)
+        e = float(r["relL2"][0])
+        ms = (time.perf_counter() - t0) * 1e3
+        print(f"[Frame {fid:3d}] relL2: {e:.4f} ({ms:.2f} ms)")
+        if a.out_dir:
+            images += [(k, fid, v) for k, v in frame_images(f, r).items()]
+    if a.out_dir:
+        print("Writing images...", end="", flush=True)
+        write_images(a.out_dir, images)
+        print("Done!")
 
 
 if __name__ == "__main__":

@@ -60,3 +60,22 @@ def test_driver_on_a_saved_scene_equals_direct_frames(dev, tmp_path):
     torch.cuda.synchronize()
     assert ea == eb
     assert torch.equal(a.shown, b.shown)
+
+
+@pytest.mark.gpu
+def test_driver_cli_on_an_exr_scene_writes_the_reference_images(dev, tmp_path, capsys):
+    """`python -m xdn.driver --scene <.exr dir> --out-dir` (main.py --save_img): the same errors as the
+    frames passed directly, one log line per frame, and olsavg/out/ref/path_{frame}.exr whose "out"
+    is the displayed frame of the step."""
+    from xdn import data, driver, exr
+    fr = _scene(tmp_path / "npy", frames=2, H=32, W=48)
+    data.save_exr_scene(str(tmp_path / "exr"), fr)
+    direct = driver.FrameDriver(32, 48, device=dev, seed=1234)
+    errs = direct.run([{k: v.to(dev) for k, v in f.items()} for f in fr], log=None)
+    shown = direct.shown.cpu().numpy()[0]
+    driver.main(["--scene", str(tmp_path / "exr"), "--frames", "2", "--out-dir", str(tmp_path / "out")])
+    lines = [l for l in capsys.readouterr().out.splitlines() if l.startswith("[Frame")]
+    assert len(lines) == 2 and [float(l.split()[3]) for l in lines] == [float(f"{e:.4f}") for e in errs]
+    names = sorted(os.listdir(tmp_path / "out"))
+    assert names == sorted(f"{n}_{i:04d}.exr" for n in ("olsavg", "out", "ref", "path") for i in range(2))
+    np.testing.assert_array_equal(exr.read(str(tmp_path / "out" / "out_0001.exr")), np.maximum(shown, 0))
