@@ -401,12 +401,31 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
         }
     };
 
-    f32x16 acc[9];
+    // CO > 16: 32x32x16 MFMA, M = 32 input channels, N = 32 output channels, K = 16 px (8 px of row
+    // 2wv + h per lane half). CO <= 16: 16x16x32 MFMA, M = 16 input channels (MTI tiles), N = 16
+    // output channels, K = 32 px = one strip row (8 px per lane group kg = lane / 16), a quarter of
+    // the MFMA work and of the accumulator registers.
+    constexpr bool N16 = CO <= 16;
+    constexpr int MTI = XS / 16;
+    typedef __attribute__((ext_vector_type(4))) float f32x4;
+    f32x16 acc[N16 ? 1 : 9];
+    f32x4 acc16[N16 ? 9 : 1][N16 ? MTI : 1];
+    if constexpr (N16)
+    {
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
+        for (int t = 0; t < 9; ++t)
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
+            for (int m = 0; m < MTI; ++m)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc16[t][m][k] = 0.f;
+    }
+    else
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
     float db = 0.f;
+    const int kg = lane >> 4;
     const int nstrip = sx * sy;
     const StripWalk sw = xcd_strips(li, nblk, nstrip);
     if (sw.first < sw.end) fetch(sw.first);
@@ -416,6 +435,35 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
         commit();
         __syncthreads();
         if (s + sw.step < sw.end) fetch(s + sw.step);
+        if constexpr (N16)
+        {
+            // lane group kg takes pixels 8kg .. 8kg+7 of the row; lane L of the group reads the
+            // transposed channel L (4 pixels per ds_read_b64_tr_b16)
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+            {
+                const int row = 2 * wv + r;
+                const uint16_t *dp = &ds[(row * US_W + 8 * kg + jq) * DS + 4 * qq];
+                const uint2 b0 = u_ld_tr(dp), b1 = u_ld_tr(dp + 4 * DS);
+                const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
+                db += (u_lo(B.x) + u_hi(B.x)) + (u_lo(B.y) + u_hi(B.y)) + (u_lo(B.z) + u_hi(B.z)) +
+                      (u_lo(B.w) + u_hi(B.w));
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int sh = 0; sh < 3; ++sh)
+#pragma unroll
+                        for (int m = 0; m < MTI; ++m)
+                        {
+                            const uint16_t *xp = &xs[((row + dy) * UX_C + 8 * kg + sh + jq) * XS + 16 * m + 4 * qq];
+                            const uint2 a0 = u_ld_tr(xp), a1 = u_ld_tr(xp + 4 * XS);
+                            acc16[dy * 3 + sh][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y}),
+                                __builtin_bit_cast(bf16x8, B), acc16[dy * 3 + sh][m], 0, 0, 0);
+                        }
+            }
+            continue;
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {
@@ -442,12 +490,21 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
     for (int t = 0; t < 9; ++t)
     {
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
+        if constexpr (N16)
         {
-            const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
-            red[wv * 1024 + ci * 32 + ch] = acc[t][k];
+            // 16x16 accumulators: column co = lane % 16, rows ci = 16m + 4kg + i
+#pragma unroll
+            for (int m = 0; m < MTI; ++m)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) red[wv * 1024 + (16 * m + 4 * kg + i) * 32 + (lane & 15)] = acc16[t][m][i];
         }
+        else
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+            {
+                const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
+                red[wv * 1024 + ci * 32 + ch] = acc[t][k];
+            }
         __syncthreads();
         for (int e = tid; e < CS * CO; e += 256)
         {
@@ -455,8 +512,17 @@ __global__ __launch_bounds__(256, 2) void uwgrad(UwBatch b)
             out[t * CS * CO + e] = (red[r] + red[1024 + r]) + (red[2048 + r] + red[3072 + r]);
         }
     }
-    db += __shfl_xor(db, 32);
-    if (h == 0) dbs[wv][ch] = db;
+    if constexpr (N16)
+    {
+        db += __shfl_xor(db, 16);
+        db += __shfl_xor(db, 32);
+        if (lane < 16) dbs[wv][lane] = db;
+    }
+    else
+    {
+        db += __shfl_xor(db, 32);
+        if (h == 0) dbs[wv][ch] = db;
+    }
     __syncthreads();
     if (tid < CO) out[9 * CS * CO + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
 }
