@@ -358,12 +358,34 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
     const int cbx = (XS > 16 ? 16 * grp : 0) + 4 * qq;   // channel offset addressed by this lane
     const int cbd = (DS > 16 ? 16 * grp : 0) + 4 * qq;
 
-    f32x16 acc[9];
+    // With fewer than four 16x16 channel tiles (the first layer: 16 inputs, the last: 6 outputs)
+    // the product runs on v_mfma_f32_16x16x32_bf16 instead: M = 16 input channels (MTI tiles),
+    // N = 16 output channels (NTI tiles), K = 32 px = one strip row (8 px per lane group
+    // kg = lane / 16), which skips the padded half or three quarters of the 32x32 tiles.
+    constexpr int MTI = XS / 16, NTI = DS / 16;
+    constexpr bool T16 = MTI * NTI < 4;
+    typedef __attribute__((ext_vector_type(4))) float f32x4;
+    f32x16 acc[T16 ? 1 : 9];
+    f32x4 acc16[T16 ? 9 : 1][T16 ? MTI : 1][T16 ? NTI : 1];
+    float db16[T16 ? NTI : 1] = {};
+    if constexpr (T16)
+    {
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
+        for (int t = 0; t < 9; ++t)
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
+            for (int m = 0; m < MTI; ++m)
+#pragma unroll
+                for (int n = 0; n < NTI; ++n)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) acc16[t][m][n][k] = 0.f;
+    }
+    else
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
     float db = 0.f;
+    const int kg = lane >> 4;
 
     // Software pipeline: the global loads of strip s + nblk are issued into registers before the
     // MFMAs of strip s and land in LDS after them.
@@ -490,6 +512,40 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
         commit();
         __syncthreads();
         if (s + sw.step < sw.end) fetch(s + sw.step);
+        if constexpr (T16)
+        {
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+            {
+                const int row = 2 * wv + r;
+                u32x4 Bn[NTI];
+#pragma unroll
+                for (int n = 0; n < NTI; ++n)
+                {
+                    const uint16_t *dp = &ds[(row * WS_W + 8 * kg + jq) * DS + 16 * n + 4 * qq];
+                    const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * DS);
+                    Bn[n] = u32x4{b0.x, b0.y, b1.x, b1.y};
+                    db16[n] += (bf_lo(Bn[n].x) + bf_hi(Bn[n].x)) + (bf_lo(Bn[n].y) + bf_hi(Bn[n].y)) +
+                               (bf_lo(Bn[n].z) + bf_hi(Bn[n].z)) + (bf_lo(Bn[n].w) + bf_hi(Bn[n].w));
+                }
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int sh = 0; sh < 3; ++sh)
+#pragma unroll
+                        for (int m = 0; m < MTI; ++m)
+                        {
+                            const uint16_t *xp = &xs[((row + dy) * WX_C + 8 * kg + sh + jq) * XS + 16 * m + 4 * qq];
+                            const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
+                            const bf16x8 A = __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y});
+#pragma unroll
+                            for (int n = 0; n < NTI; ++n)
+                                acc16[dy * 3 + sh][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    A, __builtin_bit_cast(bf16x8, Bn[n]), acc16[dy * 3 + sh][m][n], 0, 0, 0);
+                        }
+            }
+            continue;
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c)
         {
@@ -517,17 +573,48 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
     for (int t = 0; t < 9; ++t)
     {
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
+        if constexpr (T16)
         {
-            const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
-            red[wv * 1024 + ci * 32 + ch] = acc[t][k];
+            // 16x16 accumulators: column co = 16n + lane % 16, rows ci = 16m + 4kg + i
+#pragma unroll
+            for (int m = 0; m < MTI; ++m)
+#pragma unroll
+                for (int n = 0; n < NTI; ++n)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        red[wv * 1024 + (16 * m + 4 * kg + i) * 32 + 16 * n + (lane & 15)] = acc16[t][m][n][i];
         }
+        else
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+            {
+                const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
+                red[wv * 1024 + ci * 32 + ch] = acc[t][k];
+            }
         __syncthreads();
-        for (int e = tid; e < 1024; e += 256) out[t * 1024 + e] = (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]);
+        // entries outside the computed channel tiles (padding) are zero
+        for (int e = tid; e < 1024; e += 256)
+            out[t * 1024 + e] = !T16 || ((e >> 5) < 16 * MTI && (e & 31) < 16 * NTI)
+                                    ? (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e])
+                                    : 0.f;
     }
-    db += __shfl_xor(db, 32);
-    if (h == 0) dbs[wv][ch] = db;
+    if constexpr (T16)
+    {
+#pragma unroll
+        for (int n = 0; n < NTI; ++n)
+        {
+            float v = db16[n];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (lane < 16) dbs[wv][16 * n + lane] = v;
+        }
+        if (NTI == 1 && lane >= 16 && lane < 32) dbs[wv][lane] = 0.f;
+    }
+    else
+    {
+        db += __shfl_xor(db, 32);
+        if (h == 0) dbs[wv][ch] = db;
+    }
     __syncthreads();
     if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
 }
