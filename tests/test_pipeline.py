@@ -140,11 +140,11 @@ def test_server_graph_mode_runs(dev):
     assert res["frames"] == 18 and res["p99_ms"] >= res["p50_ms"] > 0
 
 
-def test_pipelined_frames_equal_sequential_steps(dev):
+@pytest.mark.parametrize("H,W", [(48, 64), (36, 52)])
+def test_pipelined_frames_equal_sequential_steps(dev, H, W):
     """Frame t+1's cross-regression on a second stream (overlapping frame t) gives results identical
-    to stepping frame by frame."""
+    to stepping frame by frame (36x52 leaves partial tiles on both axes)."""
     from xdn import pipeline, synth
-    H, W = 48, 64
     spec = synth.StreamSpec(H, W, seed=41, pan=(1.25, 0.5), disocclusion=0.1)
     frames = [{k: v.to(dev) for k, v in synth.make_frame(spec, t).items()} for t in range(5)]
     seq = pipeline.Denoiser(H, W, device=dev, seed=9)
@@ -155,6 +155,8 @@ def test_pipelined_frames_equal_sequential_steps(dev):
     torch.cuda.synchronize()
     for k in ("out", "out_log", "ols1", "ols2", "loss"):
         assert torch.equal(seq.outputs()[k], pip.outputs()[k]), k
+    for name, ch in (("prev_out", 3), ("prev_ols1", 3), ("prev_ols2", 3), ("success_prev", 1), ("succ_e", 1)):
+        assert torch.equal(seq.buffer(name, ch), pip.buffer(name, ch)), name
     assert torch.equal(seq.params, pip.params)
     assert pip.t == seq.t == 5
 
