@@ -47,53 +47,19 @@ struct MinMax9 {
     float mn[9], mx[9];  // sets avg, 1, 2 x channels
 };
 
-// Selection of one pixel: current values (a, b, their average m) against the reprojected history
-// (p, p1, p2) with the current window extrema `cur`.
-__device__ __forceinline__ void emit_values(const TsFrame &F, size_t c, float sc, const MinMax9 *cur, bool center_ok,
-                                            float3 a, float3 b, float3 p, float3 p1, float3 p2)
-{
-    const float3 m = make_float3((a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f);
-    if (sc + 1e-4f < 1.f)
-    {
-        st3(F.o, c, m);
-        st3(F.o1, c, a);
-        st3(F.o2, c, b);
-        F.osucc[c] = 0.f;
-        return;
-    }
-    const float pv[9] = {p.x, p.y, p.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z};
-    bool keep[3];
-#pragma unroll
-    for (int s = 0; s < 3; ++s)
-    {
-        bool k = true;
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch)
-        {
-            // 1x1 previous window: its extrema are the center value when it is usable, else the
-            // initial (+BIG, -BIG) of the reference
-            const float pmn = center_ok ? pv[3 * s + ch] : SCR_BIG;
-            const float pmx = center_ok ? pv[3 * s + ch] : -SCR_BIG;
-            k = k && (pmn < cur->mx[3 * s + ch]) && (cur->mn[3 * s + ch] < pmx);
-        }
-        keep[s] = k;
-    }
-    st3(F.o, c, keep[0] ? p : m);
-    st3(F.o1, c, keep[1] ? p1 : a);
-    st3(F.o2, c, keep[2] ? p2 : b);
-    F.osucc[c] = keep[0] ? 1.f : 0.f;
-}
-
 // Every global load of the block is issued up front (the pixel's history and the success input with
 // the tile staging), so a block makes one round trip to memory instead of three. Staging clamps the
 // coordinates into the image: a clamped tap repeats a pixel that is already inside the window, so
 // the extrema over the fixed 7-tap windows equal those over the in-image taps the reference visits.
-__global__ __launch_bounds__(256) void screening_tile(TsBatch b)
+//
+// The row pass writes its minima over the staged tile (all reads done first, one barrier between)
+// and its maxima to a second array: 37 KB of LDS, four resident blocks per CU.
+__global__ __launch_bounds__(256, 4) void screening_tile(TsBatch b)
 {
     constexpr int EX = TS_TX + 2 * TS_R, EY = TS_TY + 2 * TS_R;
     constexpr int EXP = EX + 2;             // row pitch (even: 8-B aligned pairs)
-    __shared__ float v[9][EY][EXP];         // avg.xyz, c1.xyz, c2.xyz
-    __shared__ float2 hm[9][EY][TS_TX];     // row-pass (min, max)
+    __shared__ float v[9][EY][EXP];         // avg.xyz, c1.xyz, c2.xyz; then the row minima
+    __shared__ float hx[9][EY][TS_TX];      // row maxima
     __shared__ uint8_t ms[EY][EX];          // success mask (fused erosion), 1 outside the image
     const int tile = xcd_tile(blockIdx.x, b.map.total);
     const TileLoc L = locate(b.map, tile);
@@ -120,11 +86,16 @@ __global__ __launch_bounds__(256) void screening_tile(TsBatch b)
         v[6][ly][lx] = bb.x; v[7][ly][lx] = bb.y; v[8][ly][lx] = bb.z;
     }
     __syncthreads();
+    // this pixel's current values, read before the row pass overwrites the staged tile
+    const float3 a = make_float3(v[3][oly + TS_R][olx + TS_R], v[4][oly + TS_R][olx + TS_R], v[5][oly + TS_R][olx + TS_R]);
+    const float3 bb = make_float3(v[6][oly + TS_R][olx + TS_R], v[7][oly + TS_R][olx + TS_R], v[8][oly + TS_R][olx + TS_R]);
     // row pass: a thread takes two neighbouring outputs (window columns x..x+6 and x+1..x+7 of the
     // staged row), sharing the six common taps
-    if (threadIdx.x < EY * TS_TX / 2)
+    const bool row = threadIdx.x < EY * TS_TX / 2;
+    const int ly = threadIdx.x / (TS_TX / 2), lx = 2 * (threadIdx.x % (TS_TX / 2));
+    float4 rr[9];  // (min, max) of the two outputs
+    if (row)
     {
-        const int ly = threadIdx.x / (TS_TX / 2), lx = 2 * (threadIdx.x % (TS_TX / 2));
 #pragma unroll
         for (int k = 0; k < 9; ++k)
         {
@@ -132,22 +103,21 @@ __global__ __launch_bounds__(256) void screening_tile(TsBatch b)
             const float2 t0 = r[0], t1 = r[1], t2 = r[2], t3 = r[3];
             const float cmn = fminf(fminf(fminf(t0.y, t1.x), t1.y), fminf(fminf(t2.x, t2.y), t3.x));
             const float cmx = fmaxf(fmaxf(fmaxf(t0.y, t1.x), t1.y), fmaxf(fmaxf(t2.x, t2.y), t3.x));
-            *reinterpret_cast<float4 *>(&hm[k][ly][lx]) =
-                make_float4(fminf(cmn, t0.x), fmaxf(cmx, t0.x), fminf(cmn, t3.y), fmaxf(cmx, t3.y));
+            rr[k] = make_float4(fminf(cmn, t0.x), fminf(cmn, t3.y), fmaxf(cmx, t0.x), fmaxf(cmx, t3.y));
+        }
+    }
+    __syncthreads();
+    if (row)
+    {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+        {
+            *reinterpret_cast<float2 *>(&v[k][ly][lx]) = make_float2(rr[k].x, rr[k].y);
+            *reinterpret_cast<float2 *>(&hx[k][ly][lx]) = make_float2(rr[k].z, rr[k].w);
         }
     }
     __syncthreads();
     if (!live) return;
-    MinMax9 cur;
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-    {
-        float2 r[2 * TS_R + 1];
-#pragma unroll
-        for (int yy = 0; yy <= 2 * TS_R; ++yy) r[yy] = hm[k][oly + yy][olx];
-        cur.mn[k] = fminf(fminf(fminf(r[0].x, r[1].x), fminf(r[2].x, r[3].x)), fminf(fminf(r[4].x, r[5].x), r[6].x));
-        cur.mx[k] = fmaxf(fmaxf(fmaxf(r[0].y, r[1].y), fmaxf(r[2].y, r[3].y)), fmaxf(fmaxf(r[4].y, r[5].y), r[6].y));
-    }
     float sc = rs;
     if (b.fuse)
     {
@@ -160,10 +130,43 @@ __global__ __launch_bounds__(256) void screening_tile(TsBatch b)
         sc = m ? 1.f : 0.f;
         if (F.succ_e) F.succ_e[c] = sc;
     }
-    const float3 a = make_float3(v[3][oly + TS_R][olx + TS_R], v[4][oly + TS_R][olx + TS_R], v[5][oly + TS_R][olx + TS_R]);
-    const float3 bb = make_float3(v[6][oly + TS_R][olx + TS_R], v[7][oly + TS_R][olx + TS_R], v[8][oly + TS_R][olx + TS_R]);
-    // win_prev == 1: the previous extrema are the center values, taken iff sc >= 1
-    emit_values(F, c, sc, &cur, sc >= 1.f, a, bb, p, p1, p2);
+    const float3 m = make_float3((a.x + bb.x) * 0.5f, (a.y + bb.y) * 0.5f, (a.z + bb.z) * 0.5f);
+    if (sc + 1e-4f < 1.f)
+    {
+        st3(F.o, c, m);
+        st3(F.o1, c, a);
+        st3(F.o2, c, bb);
+        F.osucc[c] = 0.f;
+        return;
+    }
+    // column pass, folded into the keep tests set by set. win_prev == 1: the previous extrema are
+    // the center values when sc >= 1, else the initial (+BIG, -BIG) of the reference
+    const bool center_ok = sc >= 1.f;
+    const float pv[9] = {p.x, p.y, p.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z};
+    bool keep[3];
+#pragma unroll
+    for (int st = 0; st < 3; ++st)
+    {
+        bool kp = true;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            const int k = 3 * st + ch;
+            float rn[2 * TS_R + 1], rx[2 * TS_R + 1];
+#pragma unroll
+            for (int yy = 0; yy <= 2 * TS_R; ++yy) rn[yy] = v[k][oly + yy][olx], rx[yy] = hx[k][oly + yy][olx];
+            const float mn = fminf(fminf(fminf(rn[0], rn[1]), fminf(rn[2], rn[3])), fminf(fminf(rn[4], rn[5]), rn[6]));
+            const float mx = fmaxf(fmaxf(fmaxf(rx[0], rx[1]), fmaxf(rx[2], rx[3])), fmaxf(fmaxf(rx[4], rx[5]), rx[6]));
+            const float pmn = center_ok ? pv[k] : SCR_BIG;
+            const float pmx = center_ok ? pv[k] : -SCR_BIG;
+            kp = kp && (pmn < mx) && (mn < pmx);
+        }
+        keep[st] = kp;
+    }
+    st3(F.o, c, keep[0] ? p : m);
+    st3(F.o1, c, keep[1] ? p1 : a);
+    st3(F.o2, c, keep[2] ? p2 : bb);
+    F.osucc[c] = keep[0] ? 1.f : 0.f;
 }
 
 __global__ __launch_bounds__(256) void screening_generic(TsBatch b)
