@@ -341,7 +341,7 @@ int xdn_reproject(const float *mvec, int n_list, const float *const *inputs, con
                 f.H = height;
                 f.W = width;
             }
-            if (int e = hip_check(xdn::launch_reproject(n, fr.data(), nl, dims + l0, S(stream)), "reproject")) return e;
+            if (int e = hip_check(xdn::launch_reproject(n, fr.data(), nl, dims + l0, false, S(stream)), "reproject")) return e;
         }
         l0 += nl;
     } while (l0 < n_list);

@@ -209,7 +209,7 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
             f.out[0] = sc[i].rp[0]; f.out[1] = sc[i].rp[1]; f.out[2] = sc[i].rp[2];
             f.H = s.height; f.W = s.width;
         }
-        if (int e = hip_check(xdn::launch_reproject(n, fr.data(), 3, dims, st), "reproject")) return e;
+        if (int e = hip_check(xdn::launch_reproject(n, fr.data(), 3, dims, true, st), "reproject")) return e;
     }
     if (!(stages & XDN_STAGE_FINISH)) return XDN_OK;
     // 3. 3x3 erosion + temporal screening (win_curr 7, win_prev 1, main.py:270-280)

@@ -39,7 +39,8 @@ __device__ __forceinline__ bool reproj_valid(int W, int H, int px, int py, float
 
 // FNL > 0: the buffer list is FNL buffers of FD channels, known at compile time (the training
 // step reprojects three 3-channel buffers); FNL == 0 reads the list shape from the batch.
-template <int FNL, int FD>
+// UNIT_OPAC: both opacities are one (the training step's case): their loads are skipped.
+template <int FNL, int FD, bool UNIT_OPAC>
 __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
 {
     const int tile = xcd_tile(blockIdx.x, b.map.total);
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
     // Three round trips to memory: the pixel's own inputs; every bilinear tap's validity inputs,
     // loaded at coordinates clamped into the image and used only where the tap is in the image;
     // the list values of the valid taps.
-    const float op = F.opac[c];
+    const float op = UNIT_OPAC ? 1.f : F.opac[c];
     const float mx = F.mvec[3 * c], my = F.mvec[3 * c + 1];
     const float fwN = F.pnFw[3 * c + 1];
     const float Z = F.linZ[3 * c], fwZ = F.linZ[3 * c + 1];
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
             const int lx = bx + (t & 1), ly = by + (t >> 1);
             inb[t] = !(lx >= W || lx < 0 || ly >= H || ly < 0);
             li[t] = (size_t)min(max(ly, 0), H - 1) * W + min(max(lx, 0), W - 1);
-            po[t] = F.prevOpac[li[t]];
+            po[t] = UNIT_OPAC ? 1.f : F.prevOpac[li[t]];
             pz[t] = F.prevLinZ[3 * li[t]];
             pn[t] = ld3(F.prevNrm, li[t]);
         }
@@ -129,7 +130,7 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
                     const int qx = ipx + xx, qy = ipy + yy;
                     if (qx >= W || qx < 0 || qy >= H || qy < 0) continue;
                     const size_t qi = (size_t)qy * W + qx;
-                    if (F.prevOpac[qi] == 0.f) continue;
+                    if (!UNIT_OPAC && F.prevOpac[qi] == 0.f) continue;
                     if (!reproj_valid(W, H, ipx, ipy, Z, F.prevLinZ[3 * qi], fwZ, n, ld3(F.prevNrm, qi), fwN)) continue;
 #pragma unroll
                     for (int l = 0; l < XDN_MAX_LIST; ++l)
@@ -176,7 +177,8 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
 
 static int div_up(int a, int b) { return (a + b - 1) / b; }
 
-hipError_t launch_reproject(int n, const RpFrame *fr, int n_list, const int *dims, hipStream_t st)
+hipError_t launch_reproject(int n, const RpFrame *fr, int n_list, const int *dims, bool unit_opacity,
+                            hipStream_t st)
 {
     RpBatch b{};
     int tx[XDN_MAX_FRAMES], ty[XDN_MAX_FRAMES];
@@ -192,10 +194,12 @@ hipError_t launch_reproject(int n, const RpFrame *fr, int n_list, const int *dim
     if (b.map.total == 0) return hipSuccess;
     bool three3 = n_list == 3;
     for (int l = 0; l < n_list; ++l) three3 = three3 && dims[l] == 3;
-    if (three3)
-        hipLaunchKernelGGL((reproject_kernel<3, 3>), dim3(b.map.total), dim3(256), 0, st, b);
+    if (three3 && unit_opacity)
+        hipLaunchKernelGGL((reproject_kernel<3, 3, true>), dim3(b.map.total), dim3(256), 0, st, b);
+    else if (three3)
+        hipLaunchKernelGGL((reproject_kernel<3, 3, false>), dim3(b.map.total), dim3(256), 0, st, b);
     else
-        hipLaunchKernelGGL((reproject_kernel<0, 0>), dim3(b.map.total), dim3(256), 0, st, b);
+        hipLaunchKernelGGL((reproject_kernel<0, 0, false>), dim3(b.map.total), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 

@@ -81,7 +81,9 @@ struct RpFrame {
     float *out[XDN_MAX_LIST];
     int H, W;
 };
-hipError_t launch_reproject(int n, const RpFrame *fr, int n_list, const int *dims, hipStream_t st);
+// unit_opacity: opac and prevOpac are all ones (not read).
+hipError_t launch_reproject(int n, const RpFrame *fr, int n_list, const int *dims, bool unit_opacity,
+                            hipStream_t st);
 
 // Temporal screening (optionally with the 3x3 erosion of a u8 success mask fused in).
 struct TsFrame {
