@@ -8,9 +8,8 @@
 // exact given identical inputs; the set average is computed as (c1 + c2) * 0.5f in FP32.
 //
 // screening_tile (win_curr 7, win_prev 1): 32x8 pixels per workgroup; the tile plus a 3-px halo of
-// (c1, c2, avg) goes to LDS, then a separable min/max (7-tap row pass into LDS, two outputs per
-// thread; 7-tap column pass) gives the 18 extrema per pixel with 14 instead of 49 taps, unrolled
-// into v_min3/v_max3 chains.
+// (c1, c2, avg) goes to LDS, then a separable min/max (7-tap column pass in place in LDS, 7-tap row
+// pass per pixel) gives the 18 extrema per pixel with 14 instead of 49 taps.
 // screening_generic: any window sizes, direct loads.
 #include "xdn_common.h"
 #include "xdn_kernels.h"
@@ -52,14 +51,17 @@ struct MinMax9 {
 // coordinates into the image: a clamped tap repeats a pixel that is already inside the window, so
 // the extrema over the fixed 7-tap windows equal those over the in-image taps the reference visits.
 //
-// The row pass writes its minima over the staged tile (all reads done first, one barrier between)
-// and its maxima to a second array: 37 KB of LDS, four resident blocks per CU.
-__global__ __launch_bounds__(256, 4) void screening_tile(TsBatch b)
+// Column pass first: a thread owns whole staged columns (set k, column x: 14 rows) and writes their
+// seven-row window minima (rows 0..7) and maxima (rows 8..15) back over them, the 8 windows of a
+// column from a prefix/suffix split at row 7 (18 min + 18 max instead of 48 + 48). The row pass then
+// reads seven neighbours of each. 22 KB of LDS: seven resident blocks per CU.
+__global__ __launch_bounds__(256) void screening_tile(TsBatch b)
 {
     constexpr int EX = TS_TX + 2 * TS_R, EY = TS_TY + 2 * TS_R;
-    constexpr int EXP = EX + 2;             // row pitch (even: 8-B aligned pairs)
-    __shared__ float v[9][EY][EXP];         // avg.xyz, c1.xyz, c2.xyz; then the row minima
-    __shared__ float hx[9][EY][TS_TX];      // row maxima
+    constexpr int EXP = EX;                 // row pitch
+    constexpr int NCOL = 9 * EX;            // column jobs
+    static_assert(EY == 14 && 2 * TS_TY == 16, "the column pass assumes 7-row windows over 14 rows");
+    __shared__ float v[9][2 * TS_TY][EXP];  // avg.xyz, c1.xyz, c2.xyz; then column minima / maxima
     __shared__ uint8_t ms[EY][EX];          // success mask (fused erosion), 1 outside the image
     const int tile = xcd_tile(blockIdx.x, b.map.total);
     const TileLoc L = locate(b.map, tile);
@@ -86,35 +88,34 @@ __global__ __launch_bounds__(256, 4) void screening_tile(TsBatch b)
         v[6][ly][lx] = bb.x; v[7][ly][lx] = bb.y; v[8][ly][lx] = bb.z;
     }
     __syncthreads();
-    // this pixel's current values, read before the row pass overwrites the staged tile
+    // this pixel's current values, read before the column pass overwrites the staged tile
     const float3 a = make_float3(v[3][oly + TS_R][olx + TS_R], v[4][oly + TS_R][olx + TS_R], v[5][oly + TS_R][olx + TS_R]);
     const float3 bb = make_float3(v[6][oly + TS_R][olx + TS_R], v[7][oly + TS_R][olx + TS_R], v[8][oly + TS_R][olx + TS_R]);
-    // row pass: a thread takes two neighbouring outputs (window columns x..x+6 and x+1..x+7 of the
-    // staged row), sharing the six common taps
-    const bool row = threadIdx.x < EY * TS_TX / 2;
-    const int ly = threadIdx.x / (TS_TX / 2), lx = 2 * (threadIdx.x % (TS_TX / 2));
-    float4 rr[9];  // (min, max) of the two outputs
-    if (row)
-    {
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-        {
-            const float2 *r = reinterpret_cast<const float2 *>(&v[k][ly][lx]);
-            const float2 t0 = r[0], t1 = r[1], t2 = r[2], t3 = r[3];
-            const float cmn = fminf(fminf(fminf(t0.y, t1.x), t1.y), fminf(fminf(t2.x, t2.y), t3.x));
-            const float cmx = fmaxf(fmaxf(fmaxf(t0.y, t1.x), t1.y), fmaxf(fmaxf(t2.x, t2.y), t3.x));
-            rr[k] = make_float4(fminf(cmn, t0.x), fminf(cmn, t3.y), fmaxf(cmx, t0.x), fmaxf(cmx, t3.y));
-        }
-    }
     __syncthreads();
-    if (row)
+    for (int j = threadIdx.x; j < NCOL; j += 256)
     {
+        const int k = j / EX, x = j - k * EX;
+        float t[EY];
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
+        for (int r = 0; r < EY; ++r) t[r] = v[k][r][x];
+        // window r covers rows r..r+6: suffix extrema of rows r..6 and prefix extrema of rows 7..r+6
+        float sn[7], sx[7], pn[7], px[7];
+        sn[6] = sx[6] = t[6];
+#pragma unroll
+        for (int r = 5; r >= 0; --r) sn[r] = fminf(t[r], sn[r + 1]), sx[r] = fmaxf(t[r], sx[r + 1]);
+        pn[0] = px[0] = t[7];
+#pragma unroll
+        for (int r = 1; r < 7; ++r) pn[r] = fminf(pn[r - 1], t[7 + r]), px[r] = fmaxf(px[r - 1], t[7 + r]);
+        v[k][0][x] = sn[0];
+        v[k][TS_TY][x] = sx[0];
+#pragma unroll
+        for (int r = 1; r < 7; ++r)
         {
-            *reinterpret_cast<float2 *>(&v[k][ly][lx]) = make_float2(rr[k].x, rr[k].y);
-            *reinterpret_cast<float2 *>(&hx[k][ly][lx]) = make_float2(rr[k].z, rr[k].w);
+            v[k][r][x] = fminf(sn[r], pn[r - 1]);
+            v[k][TS_TY + r][x] = fmaxf(sx[r], px[r - 1]);
         }
+        v[k][7][x] = pn[6];
+        v[k][TS_TY + 7][x] = px[6];
     }
     __syncthreads();
     if (!live) return;
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(256, 4) void screening_tile(TsBatch b)
         F.osucc[c] = 0.f;
         return;
     }
-    // column pass, folded into the keep tests set by set. win_prev == 1: the previous extrema are
+    // row pass, folded into the keep tests set by set. win_prev == 1: the previous extrema are
     // the center values when sc >= 1, else the initial (+BIG, -BIG) of the reference
     const bool center_ok = sc >= 1.f;
     const float pv[9] = {p.x, p.y, p.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z};
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(256, 4) void screening_tile(TsBatch b)
             const int k = 3 * st + ch;
             float rn[2 * TS_R + 1], rx[2 * TS_R + 1];
 #pragma unroll
-            for (int yy = 0; yy <= 2 * TS_R; ++yy) rn[yy] = v[k][oly + yy][olx], rx[yy] = hx[k][oly + yy][olx];
+            for (int xx = 0; xx <= 2 * TS_R; ++xx) rn[xx] = v[k][oly][olx + xx], rx[xx] = v[k][TS_TY + oly][olx + xx];
             const float mn = fminf(fminf(fminf(rn[0], rn[1]), fminf(rn[2], rn[3])), fminf(fminf(rn[4], rn[5]), rn[6]));
             const float mx = fmaxf(fmaxf(fmaxf(rx[0], rx[1]), fmaxf(rx[2], rx[3])), fmaxf(fmaxf(rx[4], rx[5]), rx[6]));
             const float pmn = center_ok ? pv[k] : SCR_BIG;
