@@ -68,7 +68,7 @@ static_assert(sizeof(UcBatch) <= 4096, "kernel argument too large");
 constexpr int UT_W = 32, UT_H = 16;
 
 template <int SRC, int CA, int CB, int CO, int EPI, bool HEAD = false>
-__global__ __launch_bounds__(256) void uconv3x3(UcBatch b)
+__global__ __launch_bounds__(256, (CB == 0 && CA <= 16 && CO <= 24) ? 4 : 1) void uconv3x3(UcBatch b)
 {
     static_assert(!HEAD || (CO == 8 && EPI == UC_EPI_RELU), "the head follows the 8-channel ReLU layer");
     constexpr int CI = SRC == UC_SRC_IMAGES ? 15 : CA + CB;
