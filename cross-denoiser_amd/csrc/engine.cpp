@@ -21,6 +21,7 @@ using namespace xdn_abi;
 namespace {
 
 constexpr int kRegWin = 17, kSparsity = 4, kFilterWin = 11;  // scripts/main.py:71-73
+constexpr float kAdamB1 = 0.9f, kAdamB2 = 0.999f;              // Keras Adam defaults (main.py:81)
 
 // Scratch layout of one slot (all offsets 256-byte aligned).
 struct SlotScratch {
@@ -279,8 +280,20 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
             scale[i] = (float)(1.0 / (3.0 * (double)px(s.height, s.width)));
             outp[i] = s.avg_loss;
         }
+        // the loss-sum launch also advances the device Adam step counters used by step 7
+        xdn::AdamTick tick{};
+        tick.lr = lr; tick.b1 = kAdamB1; tick.b2 = kAdamB2;
+        for (int i = 0; i < n; ++i)
+            if (slots[i].adam_t)
+            {
+                tick.counter[tick.n] = slots[i].adam_t;
+                tick.alpha[tick.n] = sc[i].adam_alpha;
+                ++tick.n;
+            }
         if (int e = hip_check(xdn::launch_filter_train(n, fr.data(), kFilterWin, st), "filter_train")) return e;
-        if (int e = hip_check(xdn::launch_sums(n, parts.data(), cnt.data(), scale.data(), outp.data(), st), "loss")) return e;
+        if (int e = hip_check(xdn::launch_sums(n, parts.data(), cnt.data(), scale.data(), outp.data(), st, &tick),
+                              "loss"))
+            return e;
     }
     // 6. network backward: per layer one weight-gradient launch and one data-gradient launch; the
     //    per-block weight-gradient partials of every (slot, layer) are reduced by one launch pair
@@ -344,9 +357,7 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
         std::vector<xdn::AdamJob> jobs(n);
         std::vector<xdn::PackJob> pj(unet ? 0 : 5 * n);
         std::vector<xdn::UPackJob> upj;
-        std::vector<int *> counters;
-        std::vector<float *> alphas;
-        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-7f;
+        const float b1 = kAdamB1, b2 = kAdamB2, eps = 1e-7f;
         for (int i = 0; i < n; ++i)
         {
             const xdn_slot &s = slots[i];
@@ -355,11 +366,6 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
             jobs[i].count = (int)(unet ? kULay.count : kLay.count);
             jobs[i].alpha = (float)((double)lr * sqrt(1.0 - pow((double)b2, t)) / (1.0 - pow((double)b1, t)));
             jobs[i].alpha_dev = s.adam_t ? sc[i].adam_alpha : nullptr;
-            if (s.adam_t)
-            {
-                counters.push_back(s.adam_t);
-                alphas.push_back(sc[i].adam_alpha);
-            }
             char *pk = static_cast<char *>(s.packed);
             if (unet)
             {
@@ -375,11 +381,6 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
                 j.ci = kCnn5[l].ci; j.co = kCnn5[l].co;
             }
         }
-        if (!counters.empty())
-            if (int e = hip_check(xdn::launch_adam_tick((int)counters.size(), counters.data(), alphas.data(), lr, b1,
-                                                        b2, st),
-                                  "adam step counter"))
-                return e;
         if (int e = hip_check(xdn::launch_adam(n, jobs.data(), b1, b2, eps, st), "adam")) return e;
         if (unet)
         {

@@ -8,7 +8,8 @@ namespace xdn {
 
 // Keras Adam (tf.keras.optimizers.Adam as used by scripts/main.py:81,303):
 //   m += (g - m)(1 - b1); v += (g^2 - v)(1 - b2); p -= alpha m / (sqrt(v) + eps),
-//   alpha = lr sqrt(1 - b2^t) / (1 - b1^t) (computed on the host in double).
+//   alpha = lr sqrt(1 - b2^t) / (1 - b1^t) (computed in double on the host, or on the device from a
+//   step counter by the loss-sum launch that precedes the update).
 struct AdamBatch {
     int n;
     AdamJob j[XDN_MAX_FRAMES];
@@ -31,41 +32,6 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamBatch b)
     }
 }
 
-struct TickBatch {
-    int n;
-    int *counter[XDN_MAX_FRAMES];
-    float *alpha[XDN_MAX_FRAMES];
-    float lr, b1, b2;
-};
-
-__global__ void adam_tick_kernel(TickBatch b)
-{
-    const int i = threadIdx.x;
-    if (i >= b.n) return;
-    const int t = *b.counter[i] + 1;
-    *b.counter[i] = t;
-    *b.alpha[i] = (float)((double)b.lr * sqrt(1.0 - pow((double)b.b2, (double)t)) / (1.0 - pow((double)b.b1, (double)t)));
-}
-
-hipError_t launch_adam_tick(int n, int *const *counter, float *const *alpha, float lr, float b1, float b2,
-                            hipStream_t st)
-{
-    if (n <= 0) return hipSuccess;
-    if (n > XDN_MAX_FRAMES) return hipErrorInvalidValue;
-    TickBatch b{};
-    b.n = n;
-    for (int i = 0; i < n; ++i)
-    {
-        b.counter[i] = counter[i];
-        b.alpha[i] = alpha[i];
-    }
-    b.lr = lr;
-    b.b1 = b1;
-    b.b2 = b2;
-    hipLaunchKernelGGL(adam_tick_kernel, dim3(1), dim3(64), 0, st, b);
-    return hipGetLastError();
-}
-
 hipError_t launch_adam(int n, const AdamJob *jobs, float b1, float b2, float eps, hipStream_t st)
 {
     AdamBatch b{};
@@ -86,18 +52,29 @@ hipError_t launch_adam(int n, const AdamJob *jobs, float b1, float b2, float eps
 }
 
 // out[f] = scale[f] * sum(parts[f][0 .. count[f])) in a fixed order (one workgroup per frame).
+// Optionally also the Adam step counters' advance (block 0, one thread per counter).
 struct SumBatch {
     int n;
     const float *parts[XDN_MAX_FRAMES];
     int count[XDN_MAX_FRAMES];
     float scale[XDN_MAX_FRAMES];
     float *out[XDN_MAX_FRAMES];
+    AdamTick tick;
 };
+static_assert(sizeof(SumBatch) <= 4096, "kernel argument too large");
 
 __global__ __launch_bounds__(256) void sum_kernel(SumBatch b)
 {
     __shared__ float red[256];
     const int f = blockIdx.x;
+    if (f == 0 && threadIdx.x < b.tick.n)
+    {
+        const int i = threadIdx.x;
+        const int t = *b.tick.counter[i] + 1;
+        *b.tick.counter[i] = t;
+        *b.tick.alpha[i] = (float)((double)b.tick.lr * sqrt(1.0 - pow((double)b.tick.b2, (double)t)) /
+                                   (1.0 - pow((double)b.tick.b1, (double)t)));
+    }
     // eight loads in flight per thread; the sum keeps the order i, i + 256, ...
     float s = 0.f;
     const int cnt = b.count[f];
@@ -122,10 +99,15 @@ __global__ __launch_bounds__(256) void sum_kernel(SumBatch b)
 }
 
 hipError_t launch_sums(int n, const float *const *parts, const int *count, const float *scale, float *const *out,
-                       hipStream_t st)
+                       hipStream_t st, const AdamTick *tick)
 {
     SumBatch b{};
     b.n = n;
+    if (tick)
+    {
+        if (tick->n < 0 || tick->n > XDN_MAX_FRAMES || tick->n > n) return hipErrorInvalidValue;
+        b.tick = *tick;
+    }
     for (int i = 0; i < n; ++i)
     {
         b.parts[i] = parts[i];
