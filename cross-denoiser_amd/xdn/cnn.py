@@ -101,7 +101,7 @@ class _HipNet:
         self.grad = torch.zeros_like(self.params)
         self.m = torch.zeros_like(self.params)
         self.v = torch.zeros_like(self.params)
-        self.packed = torch.empty(int(self._fn("packed_bytes")()), device=self.device, dtype=torch.uint8)
+        self.packed = torch.zeros(int(self._fn("packed_bytes")()), device=self.device, dtype=torch.uint8)
         self.step_count = 0
         self.resize(shape)
         self.pack()

@@ -100,7 +100,7 @@ class Denoiser:
         self.grad = torch.zeros_like(self.params)
         self.adam_m = torch.zeros_like(self.params)
         self.adam_v = torch.zeros_like(self.params)
-        self.packed = torch.empty(int(getattr(lib, f"xdn_{network}_packed_bytes")()), device=dev, dtype=torch.uint8)
+        self.packed = torch.zeros(int(getattr(lib, f"xdn_{network}_packed_bytes")()), device=dev, dtype=torch.uint8)
         self.avg_loss = torch.zeros(1, device=dev, dtype=torch.float32)
         self.adam_t = torch.zeros(1, device=dev, dtype=torch.int32)  # Adam updates done (device counter)
         self.use_graphs = use_graphs

@@ -181,16 +181,18 @@ def test_online_denoiser_reduces_error(dev, network):
     assert errs[-1][0] < 0.25 * errs[-1][1]
 
 
-def test_unet_step_uses_the_unet_ops(dev):
-    """In a step with network="unet" the network output, its gradient and the Adam update are those
-    of the standalone U-Net entry points (xdn_unet_forward / _backward, xdn_adam) on the step's own
-    network inputs."""
+@pytest.mark.parametrize("network", ["cnn5", "unet"])
+def test_step_uses_the_network_ops(dev, network):
+    """In a step the network output, its gradient, the Adam update and the repacked fragments are
+    those of the standalone entry points (xdn_<network>_forward / _backward, xdn_adam,
+    xdn_<network>_pack) on the step's own network inputs."""
     from xdn import cnn, pipeline, synth
     H, W = 48, 64
     spec = synth.StreamSpec(H, W, seed=13, pan=(1.0, 0.5), disocclusion=0.1)
     frames = [{k: v.to(dev) for k, v in synth.make_frame(spec, t).items()} for t in range(2)]
-    den = pipeline.Denoiser(H, W, device=dev, seed=21, network="unet")
-    net = cnn.HipUNet(dev, seed=21, shape=(1, H, W))
+    den = pipeline.Denoiser(H, W, device=dev, seed=21, network=network)
+    net = cnn.make(network, dev, seed=21, shape=(1, H, W))
+    assert torch.equal(net.params, den.params) and torch.equal(net.packed, den.packed)
     den.step(frames[0])
     net.forward(*[t.clone() for t in (den.outputs()["ols1"], den.outputs()["ols2"])],
                 den.buffer("prev_out", 3).clone(), frames[0]["albedo"], frames[0]["normal"])
@@ -201,9 +203,14 @@ def test_unet_step_uses_the_unet_ops(dev):
     torch.cuda.synchronize()
     assert torch.equal(net.grad, den.grad)
     assert torch.allclose(net.params, den.params, rtol=1e-6, atol=1e-9)
-    # second frame: history present, same network state on both sides
+    # the step's Adam step counter and step size live on the device; the standalone update takes
+    # them from the host, so the parameters may differ in the last bit; the step's fragments are
+    # exactly the standalone packing of its own parameters
     net.params.copy_(den.params)
     net.pack()
+    torch.cuda.synchronize()
+    assert torch.equal(net.packed, den.packed)
+    # second frame: history present, same network state on both sides
     den.step(frames[1])
     net.forward(den.outputs()["ols1"], den.outputs()["ols2"], den.buffer("prev_out", 3), frames[1]["albedo"],
                 frames[1]["normal"])
