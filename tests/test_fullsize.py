@@ -1,7 +1,8 @@
 """Parity at the benchmark's frame size, 1920x1080 (and 3840x2160, the largest stream of the serving
 configurations, for the index-exact ops), against the FP64 oracle: the reprojection mask and the
-screening selections bit-exact, cross-regression and spatial-filter outputs within the relative
-L2 bar of the small-size tests (1e-4; reprojected values 1e-6), and the per-frame training step's three
+screening selections bit-exact, cross-regression, spatial-filter and fused training-filter
+(filter + aggregation + loss + gradient) outputs within the relative L2 bar of the small-size tests
+(1e-4; reprojected values 1e-6), and the per-frame training step's three
 schedules (frame by frame, stage-split on two HIP streams, captured graphs) bit-identical to one
 another over three frames."""
 import numpy as np
@@ -10,7 +11,7 @@ import torch
 
 from helpers import log_images, rand_img, rel_l2, to_dev, to_np, unit_normals
 from oracle import oracle as O
-from test_gpu_parity import reproject_case, screening_case, sf_case
+from test_gpu_parity import reproject_case, screening_case, sf_case, train_case, train_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -70,6 +71,17 @@ def test_spatial_filter_1080p(ops, dev):
     for name, g, r in zip(("acca", "accb", "wgtsuma", "wgtsumb"), got, ref):
         err = rel_l2(to_np(g), r)
         print(f"  spatial_filter 1080p {name}: rel L2 {err:.3e}", flush=True)
+        assert err <= TOL, name
+
+
+def test_filter_train_1080p(dev):
+    from xdn import pipeline
+    case = train_case(H, W, seed=7)
+    ref = train_oracle(*case)
+    got = pipeline.filter_train(*[to_dev(a, dev) for a in case], want_loss=True)
+    for name, g, r in zip(("out_log", "out", "loss", "avg_loss", "dparam"), got, ref):
+        err = rel_l2(to_np(g), r)
+        print(f"  filter_train 1080p {name}: rel L2 {err:.3e}", flush=True)
         assert err <= TOL, name
 
 
