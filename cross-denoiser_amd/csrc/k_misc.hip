@@ -52,7 +52,8 @@ hipError_t launch_adam(int n, const AdamJob *jobs, float b1, float b2, float eps
 }
 
 // out[f] = scale[f] * sum(parts[f][0 .. count[f])) in a fixed order (one workgroup per frame).
-// Optionally also the Adam step counters' advance (block 0, one thread per counter).
+// Optionally also the Adam step counters' advance (one extra block, one thread per counter, so its
+// double-precision step size runs beside the sums).
 struct SumBatch {
     int n;
     const float *parts[XDN_MAX_FRAMES];
@@ -67,13 +68,17 @@ __global__ __launch_bounds__(256) void sum_kernel(SumBatch b)
 {
     __shared__ float red[256];
     const int f = blockIdx.x;
-    if (f == 0 && threadIdx.x < b.tick.n)
+    if (f == b.n)
     {
-        const int i = threadIdx.x;
-        const int t = *b.tick.counter[i] + 1;
-        *b.tick.counter[i] = t;
-        *b.tick.alpha[i] = (float)((double)b.tick.lr * sqrt(1.0 - pow((double)b.tick.b2, (double)t)) /
-                                   (1.0 - pow((double)b.tick.b1, (double)t)));
+        if (threadIdx.x < b.tick.n)
+        {
+            const int i = threadIdx.x;
+            const int t = *b.tick.counter[i] + 1;
+            *b.tick.counter[i] = t;
+            *b.tick.alpha[i] = (float)((double)b.tick.lr * sqrt(1.0 - pow((double)b.tick.b2, (double)t)) /
+                                       (1.0 - pow((double)b.tick.b1, (double)t)));
+        }
+        return;
     }
     // eight loads in flight per thread; the sum keeps the order i, i + 256, ...
     float s = 0.f;
@@ -116,7 +121,7 @@ hipError_t launch_sums(int n, const float *const *parts, const int *count, const
         b.out[i] = out[i];
     }
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(sum_kernel, dim3(n), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(sum_kernel, dim3(n + (b.tick.n > 0 ? 1 : 0)), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 
