@@ -1,7 +1,8 @@
 """Parity at the benchmark's frame size, 1920x1080 (and 3840x2160, the largest stream of the serving
 configurations, for the index-exact ops), against the FP64 oracle: the reprojection mask and the
 screening selections bit-exact, cross-regression, spatial-filter and fused training-filter
-(filter + aggregation + loss + gradient) outputs within the relative L2 bar of the small-size tests
+(filter + aggregation + loss + gradient) outputs and the backward ops (band gradient, the filter's
+input gradients, the cross-regression adjoint) within the relative L2 bars of the small-size tests
 (1e-4; reprojected values 1e-6), and the per-frame training step's three
 schedules (frame by frame, stage-split on two HIP streams, captured graphs) bit-identical to one
 another over three frames."""
@@ -11,7 +12,8 @@ import torch
 
 from helpers import log_images, rand_img, rel_l2, to_dev, to_np, unit_normals
 from oracle import oracle as O
-from test_gpu_parity import reproject_case, screening_case, sf_case, train_case, train_oracle
+from test_gpu_parity import (CR_GRAD_TOL, _cots, cr_grad_case, reproject_case, screening_case, sf_case,
+                             train_case, train_oracle)
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +74,35 @@ def test_spatial_filter_1080p(ops, dev):
         err = rel_l2(to_np(g), r)
         print(f"  spatial_filter 1080p {name}: rel L2 {err:.3e}", flush=True)
         assert err <= TOL, name
+
+
+def test_spatial_filter_gradients_1080p(ops, dev):
+    args = sf_case(H, W, seed=21, band_lo=0.1, band_hi=3.0)
+    cots = _cots(1, H, W, 9)
+    dargs = [to_dev(a, dev) for a in args]
+    dcots = [to_dev(c, dev) for c in cots]
+    err = rel_l2(to_np(ops.spatial_filter_grad(*dcots, *dargs, win_size=11)),
+                 O.spatial_filter_grad(*cots, *args, win_size=11))
+    print(f"  spatial_filter_grad 1080p: rel L2 {err:.3e}", flush=True)
+    assert err <= TOL
+    got = ops.spatial_filter_grad_inputs(*dcots, *dargs, win_size=11)
+    ref = O.spatial_filter_grad_inputs(*cots, *args, win_size=11)
+    for name, g, r in zip(("imga", "imgb", "albedo", "normal"), got, ref):
+        err = rel_l2(to_np(g), r)
+        print(f"  spatial_filter_grad_inputs 1080p {name}: rel L2 {err:.3e}", flush=True)
+        assert err <= TOL, name
+
+
+def test_cross_regression_grad_1080p(ops, dev):
+    args = cr_grad_case(H, W, seed=3)
+    cots = _cots(1, H, W, 12)
+    ref = O.cross_regression_grad(*args, *cots)
+    got = ops.cross_regression_grad(*[to_dev(a, dev) for a in args], *[to_dev(c, dev) for c in cots],
+                                    win_size=17, sparsity=4)
+    for name, g, r in zip(("imga", "imgb", "albedo", "normal"), got, ref):
+        err = rel_l2(to_np(g), r)
+        print(f"  cross_regression_grad 1080p {name}: rel L2 {err:.3e}", flush=True)
+        assert err <= CR_GRAD_TOL, name
 
 
 def test_filter_train_1080p(dev):
