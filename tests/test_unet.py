@@ -223,7 +223,7 @@ def test_reference_backward_matches_autograd_cpu(monkeypatch):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("B,H,W", [(1, 48, 80), (2, 32, 64), (1, 36, 44), (1, 256, 320)])
+@pytest.mark.parametrize("B,H,W", [(1, 48, 80), (2, 32, 64), (1, 36, 44), (1, 256, 320), (1, 1080, 1920)])
 def test_unet_forward_backward(dev, B, H, W):
     from xdn import cnn
     torch.manual_seed(H * W + B)
@@ -256,7 +256,8 @@ def test_unet_forward_backward(dev, B, H, W):
     for buf, src, l in ((A6, A5, 6), (A9, A8, 9)):
         d, far = bf16_mismatch(own[buf], convT(own[src], *L[l]))
         print(f"  convT layer {l} forward on own operands: {d} differ, {far} by more than 1 ulp", flush=True)
-        assert far <= 2 and d <= 2e-3 * own[buf].numel() + 2
+        # the same size-scaled bar as the 3x3 layers (1080p: 3 of 3.3e7 elements beyond one ulp)
+        assert far <= 2 + 1e-5 * own[buf].numel() and d <= 2e-3 * own[buf].numel() + 2
     e = rel_l2(out.cpu().numpy(), head(own[A10], *L[11]).permute(0, 2, 3, 1).numpy())
     print(f"  head forward on own operands: rel L2 {e:.2e}", flush=True)
     assert e <= 1e-5
