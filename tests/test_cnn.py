@@ -55,7 +55,7 @@ def ref_backward(layers, acts, dparam):
     return grads
 
 
-@pytest.mark.parametrize("B,H,W", [(1, 48, 80), (1, 37, 45), (2, 32, 64), (1, 128, 96)])
+@pytest.mark.parametrize("B,H,W", [(1, 48, 80), (1, 37, 45), (2, 32, 64), (1, 128, 96), (1, 1080, 1920)])
 def test_cnn5_forward_backward(dev, B, H, W):
     from xdn import cnn
     torch.manual_seed(H * W)
