@@ -352,11 +352,16 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
         }
         if (int e = hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), st), "cnn wgrad reduce")) return e;
     }
-    // 7. Keras Adam on every slot's parameters, then rebuild the bf16 fragments
+    // 7. Keras Adam on every slot's parameters and the bf16 fragments: the cnn5 update writes each
+    //    weight's fragment slots itself; the U-Net's fragments are rebuilt by one packing launch
     {
         std::vector<xdn::AdamJob> jobs(n);
-        std::vector<xdn::PackJob> pj(unet ? 0 : 5 * n);
         std::vector<xdn::UPackJob> upj;
+        xdn::AdamPackLayer lay[xdn::ADAM_PACK_LAYERS];
+        for (int l = 0; l < 5; ++l)
+            lay[l] = xdn::AdamPackLayer{(int)kLay.w_off[l], (int)kLay.b_off[l], kCnn5[l].ci, kCnn5[l].co,
+                                        (int)kLay.fwd_off[l], l > 0 ? (int)kLay.dg_off[l] : -1,
+                                        (int)kLay.bias_off[l]};
         const float b1 = kAdamB1, b2 = kAdamB2, eps = 1e-7f;
         for (int i = 0; i < n; ++i)
         {
@@ -366,28 +371,13 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
             jobs[i].count = (int)(unet ? kULay.count : kLay.count);
             jobs[i].alpha = (float)((double)lr * sqrt(1.0 - pow((double)b2, t)) / (1.0 - pow((double)b1, t)));
             jobs[i].alpha_dev = s.adam_t ? sc[i].adam_alpha : nullptr;
-            char *pk = static_cast<char *>(s.packed);
-            if (unet)
-            {
-                unet_pack_jobs(s.params, s.packed, upj);
-                continue;
-            }
-            for (int l = 0; l < 5; ++l)
-            {
-                xdn::PackJob &j = pj[5 * i + l];
-                j.w = s.params + kLay.w_off[l]; j.b = s.params + kLay.b_off[l];
-                j.frag_fwd = pk + kLay.fwd_off[l]; j.frag_dgrad = l > 0 ? pk + kLay.dg_off[l] : nullptr;
-                j.bias32 = reinterpret_cast<float *>(pk + kLay.bias_off[l]);
-                j.ci = kCnn5[l].ci; j.co = kCnn5[l].co;
-            }
+            jobs[i].packed = unet ? nullptr : s.packed;
+            if (unet) unet_pack_jobs(s.params, s.packed, upj);
         }
-        if (int e = hip_check(xdn::launch_adam(n, jobs.data(), b1, b2, eps, st), "adam")) return e;
-        if (unet)
-        {
-            if (int e = hip_check(xdn::launch_upack((int)upj.size(), upj.data(), st), "unet pack")) return e;
-        }
-        else if (int e = hip_check(xdn::launch_pack(5 * n, pj.data(), st), "pack"))
+        if (int e = hip_check(xdn::launch_adam(n, jobs.data(), b1, b2, eps, st, unet ? nullptr : lay), "adam"))
             return e;
+        if (unet)
+            if (int e = hip_check(xdn::launch_upack((int)upj.size(), upj.data(), st), "unet pack")) return e;
     }
     return XDN_OK;
 }

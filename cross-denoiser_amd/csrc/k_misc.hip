@@ -13,13 +13,46 @@ namespace xdn {
 struct AdamBatch {
     int n;
     AdamJob j[XDN_MAX_FRAMES];
+    AdamPackLayer lay[ADAM_PACK_LAYERS];
     float b1, b2, eps;
 };
+static_assert(sizeof(AdamBatch) <= 4096, "kernel argument too large");
+
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+
+// Writes the updated parameter i into its forward and data-gradient fragment slots (or the FP32
+// bias block), with the same round-to-nearest-even bf16 conversion as the full packing.
+__device__ __forceinline__ void pack_param(const AdamBatch &b, char *pk, int i, float x)
+{
+    int l = 0;
+#pragma unroll
+    for (int k = 1; k < ADAM_PACK_LAYERS; ++k)
+        if (i >= b.lay[k].w_off) l = k;
+    const AdamPackLayer &g = b.lay[l];
+    if (i >= g.b_off)
+    {
+        reinterpret_cast<float *>(pk + g.bias_off)[i - g.b_off] = x;
+        return;
+    }
+    const uint16_t v = (uint16_t)(__builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{x, 0.f}, bf16x2)) & 0xFFFFu);
+    const int e = i - g.w_off, co = e % g.co, r = e / g.co, ci = r % g.ci, t = r / g.ci;
+    {
+        const int kcf = (g.ci + 15) >> 4, kc = ci >> 4, h = (ci >> 3) & 1, jj = ci & 7;
+        reinterpret_cast<uint16_t *>(pk + g.fwd_off)[((t * kcf + kc) * 64 + 32 * h + co) * 8 + jj] = v;
+    }
+    if (g.dg_off >= 0)
+    {
+        const int kcd = (g.co + 15) >> 4, kc = co >> 4, h = (co >> 3) & 1, jj = co & 7;
+        reinterpret_cast<uint16_t *>(pk + g.dg_off)[(((8 - t) * kcd + kc) * 64 + 32 * h + ci) * 8 + jj] = v;
+    }
+}
 
 __global__ __launch_bounds__(256) void adam_kernel(AdamBatch b)
 {
     const AdamJob &J = b.j[blockIdx.y];
     const float alpha = J.alpha_dev ? *J.alpha_dev : J.alpha;
+    char *pk = static_cast<char *>(J.packed);
     for (int i = blockIdx.x * 256 + threadIdx.x; i < J.count; i += gridDim.x * 256)
     {
         const float g = J.grad[i];
@@ -28,14 +61,21 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamBatch b)
         v += (g * g - v) * (1.f - b.b2);
         J.m[i] = m;
         J.v[i] = v;
-        J.param[i] -= (m * alpha) / (sqrtf(v) + b.eps);
+        const float p = J.param[i] - (m * alpha) / (sqrtf(v) + b.eps);
+        J.param[i] = p;
+        if (pk) pack_param(b, pk, i, p);
     }
 }
 
-hipError_t launch_adam(int n, const AdamJob *jobs, float b1, float b2, float eps, hipStream_t st)
+hipError_t launch_adam(int n, const AdamJob *jobs, float b1, float b2, float eps, hipStream_t st,
+                       const AdamPackLayer *layers)
 {
     AdamBatch b{};
     b.n = n;
+    for (int i = 0; i < n; ++i)
+        if (jobs[i].packed && !layers) return hipErrorInvalidValue;
+    if (layers)
+        for (int l = 0; l < ADAM_PACK_LAYERS; ++l) b.lay[l] = layers[l];
     int maxc = 0;
     for (int i = 0; i < n; ++i)
     {

@@ -11,9 +11,22 @@ struct AdamJob {
     int count;
     float alpha;              // lr * sqrt(1 - b2^t) / (1 - b1^t)
     const float *alpha_dev;   // when set, the step size is read from here instead
+    void *packed;             // when set, the updated weights also go to these bf16 fragments (layout below)
 };
 
-hipError_t launch_adam(int n, const AdamJob *jobs, float b1, float b2, float eps, hipStream_t st);
+// Fragment layout of a network whose flat parameters are, per layer, a [9][ci][co] kernel then a
+// [co] bias: per layer a forward B-fragment block frag[t][kc][lane][8] (element 8h + j of lane
+// 32h + n holds W[t][16 kc + 8h + j][n]), a data-gradient block (W[8 - t][n][16 kc + 8h + j], none
+// when dg_off < 0) and an FP32 bias block; offsets in bytes from the packed base. Entries outside
+// ci x co are not written (they stay as the full packing left them: zero).
+struct AdamPackLayer {
+    int w_off, b_off, ci, co;
+    int fwd_off, dg_off, bias_off;
+};
+constexpr int ADAM_PACK_LAYERS = 5;
+
+hipError_t launch_adam(int n, const AdamJob *jobs, float b1, float b2, float eps, hipStream_t st,
+                       const AdamPackLayer *layers = nullptr);
 // Device Adam step counters: per entry t = ++(*counter[i]); alpha[i] = lr * sqrt(1 - b2^t) / (1 - b1^t)
 // (double, rounded once).
 struct AdamTick {
