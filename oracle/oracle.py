@@ -351,6 +351,30 @@ def band_alpha_grad(param, gband, galpha):
     return np.concatenate([gx, ga], axis=-1)
 
 
+def filter_train(ols1, ols2, albedo, normal, param, success, success_prev, prev_out, opacity, prev_ols1,
+                 prev_ols2, win_size=11):
+    """The spatiotemporal part of train_step (scripts/main.py:283-302) for a fixed network output
+    `param` [B,H,W,6]: band/alpha head (main.py:110-113), spatial_filter(ols1, ols2, ...) and
+    aggregation_loss with the labels crossed as main.py:283-286 sets them (label1 = ols2,
+    label2 = ols1, prev_label1 = prev_ols2, prev_label2 = prev_ols1), then the analytic gradient of
+    each image's mean loss back to `param`.
+
+    Returns (out_log, out, loss, avg_loss [B], dparam) in float64."""
+    band, alpha = band_alpha(param)
+    band32 = band.astype(np.float32)
+    acc1, acc2, wsum1, wsum2 = spatial_filter(ols1, ols2, albedo, normal, band32, win_size)
+    agg = (acc1, wsum1, acc2, wsum2, alpha, success, success_prev, prev_out, opacity,
+           ols2, ols1, prev_ols2, prev_ols1)
+    out_log, out, loss, _ = aggregation_loss(*agg)
+    B = np.asarray(ols1).shape[0]
+    avg = np.array([aggregation_loss(*[np.asarray(a)[b:b + 1] for a in agg])[3] for b in range(B)])
+    grads = [aggregation_loss_grad(*[np.asarray(a)[b:b + 1] for a in agg]) for b in range(B)]
+    gsum1, gsumw1, gsum2, gsumw2, galpha = [np.concatenate([g[k] for g in grads], 0) for k in range(5)]
+    gband = spatial_filter_grad(gsum1, gsum2, gsumw1, gsumw2, ols1, ols2, albedo, normal, band32, win_size)
+    dparam = band_alpha_grad(param, gband, galpha)
+    return out_log, out, loss, avg, dparam
+
+
 def frame_report(out, ref, emissive=None, env_light=None, ref_emissive=None, ref_env_light=None):
     """Displayed frame and its RelL2 error of the driver loop (scripts/main.py:344-346), with RelL2
     as defined at main.py:191-196: per element (y_true - y_pred)^2 / (mean_c(y_true)^2 + 1e-2),

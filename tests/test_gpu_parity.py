@@ -298,19 +298,7 @@ def train_case(H, W, seed, B=1):
 
 
 def train_oracle(ia, ib, alb, nrm, param, succ, sprev, prev_out, opac, pols1, pols2):
-    band, alpha = O.band_alpha(param)
-    band32 = band.astype(np.float32)
-    accA, accB, wA, wB = O.spatial_filter(ia, ib, alb, nrm, band32)
-    agg = (accA, wA, accB, wB, alpha, succ, sprev, prev_out, opac, ib, ia, pols2, pols1)
-    out_log, out, loss, _ = O.aggregation_loss(*agg)
-    B = ia.shape[0]
-    avg = np.array([O.aggregation_loss(*[a[b:b + 1] for a in agg])[3] for b in range(B)])
-    # per-image mean: the gradient of each image's own mean loss
-    grads = [O.aggregation_loss_grad(*[a[b:b + 1] for a in agg]) for b in range(B)]
-    gsum1, gsumw1, gsum2, gsumw2, galpha = [np.concatenate([g[k] for g in grads], 0) for k in range(5)]
-    gband = O.spatial_filter_grad(gsum1, gsum2, gsumw1, gsumw2, ia, ib, alb, nrm, band32)
-    dparam = O.band_alpha_grad(param, gband, galpha)
-    return out_log, out, loss, avg, dparam
+    return O.filter_train(ia, ib, alb, nrm, param, succ, sprev, prev_out, opac, pols1, pols2)
 
 
 @pytest.mark.parametrize("B,H,W", [(1, 64, 64), (1, 37, 51), (2, 48, 40)])
