@@ -194,12 +194,17 @@ class Denoiser:
         self.advance(frame)
         return self.outputs()
 
-    def _graph_for(self, frame) -> "StepGraph":
-        """The captured step for this frame's buffers and history parity (captured on first use)."""
+    def graph_key(self, frame) -> tuple:
+        """What the next step's slot depends on besides this stream's own buffers: the history
+        parity, the frame's buffers and the previous frame's G-buffers."""
         names = ("color", "color2", "albedo", "normal", "mvec", "linearZ", "pnFwidth", "opacity")
         pf = self.prev_frame
-        key = (self.t % 2, tuple(frame[k].data_ptr() for k in names),
-               None if pf is None else (pf["linearZ"].data_ptr(), pf["normal"].data_ptr()))
+        return (self.t % 2, tuple(frame[k].data_ptr() for k in names),
+                None if pf is None else (pf["linearZ"].data_ptr(), pf["normal"].data_ptr()))
+
+    def _graph_for(self, frame) -> "StepGraph":
+        """The captured step for this frame's buffers and history parity (captured on first use)."""
+        key = self.graph_key(frame)
         g = self._graphs.get(key)
         if g is None:
             if self._cap_stream is None:
