@@ -6,9 +6,11 @@
     the same stream stepped alone.
   - 16 streams: 4 x 3840x2160 + 12 x 1920x1080, bursty arrivals, half of them joining at 0.25 s, under
     a 12 GB budget (two 3840x2160 slots and a few 1920x1080 ones) that forces evictions and
-    readmissions: restored network weights, Adam moments and
-    step counters equal the saved ones, and every stream again matches its alone run (a new slot at
-    each readmission, with the history restarted).
+    readmissions. Each stream's last frame comes one on/off cycle (0.4 s) after its burst, so every
+    stream goes idle with a frame still to come: the joiners evict the idle early streams, whose last
+    frames then readmit them. Restored network weights, Adam moments and step counters equal the
+    saved ones, and every stream again matches its alone run (a new slot at each readmission, with
+    the history restarted).
 """
 import numpy as np
 import pytest
@@ -52,6 +54,8 @@ class RestoreChecked(Server):
 
 def test_config4_mix_under_eviction(dev):
     reqs = config4(frames_per_stream=3, seed=6)
+    for r in reqs:  # the last frame one on/off cycle later: it arrives in the stream's next burst
+        r.arrivals = r.arrivals[:2] + [r.arrivals[2] + 0.4]
     srv = RestoreChecked(dev, hbm_budget=12 << 30, pool_frames=2, idle_after=0.0)
     res = srv.run(reqs)
     print(f"  config 4: {res['frames']} frames, {res['evictions']} evictions, {srv.restores} restores, "

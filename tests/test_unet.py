@@ -316,3 +316,49 @@ def test_unet_adam_step_and_repack(dev):
     L = [(w.cpu(), b.cpu()) for w, b in net.layers()]
     ref_out, _, _ = unet_ref_forward([t.cpu() for t in imgs], L)
     assert rel_l2(out.cpu().numpy(), ref_out.numpy()) <= 5e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("H,W", [(128, 160), (256, 320)])
+def test_unet_error_against_unrounded_float64(dev, H, W, monkeypatch):
+    """The U-Net's error in the reference's own arithmetic (FP32 Keras, scripts/main.py:83-118,
+    stood in for by float64 with no bf16 rounding anywhere): the network output, every layer's
+    weight gradient, and what the difference does downstream, the filtered radiance and the
+    network-output gradient of the fused training filter (FP32 on the GPU for both)."""
+    from test_gpu_parity import train_case
+    from xdn import cnn, pipeline
+    case = train_case(H, W, seed=H + 3)
+    ia, ib, alb, nrm, _, succ, sprev, prev_out, opac, pols1, pols2 = case
+    imgs = [ia, ib, prev_out, alb, nrm]
+    net = cnn.HipUNet(dev, seed=1234, shape=(1, H, W))  # the step's Glorot-uniform initialisation
+    out = net.forward(*[torch.from_numpy(x).to(dev) for x in imgs])
+    torch.cuda.synchronize()
+    L = [(w.cpu(), b.cpu()) for w, b in net.layers()]
+    monkeypatch.setitem(globals(), "bf", lambda x: x.double())
+    ref_out, ref_a, x0 = unet_ref_forward([torch.from_numpy(x) for x in imgs], L)
+    e_out = rel_l2(out.cpu().numpy(), ref_out.numpy())
+
+    # downstream: the same filter step fed the kernels' output and the unrounded one
+    rest = [to_dev_np(x, dev) for x in (succ, sprev, prev_out, opac, pols1, pols2)]
+    head4 = [to_dev_np(x, dev) for x in (ia, ib, alb, nrm)]
+    got = pipeline.filter_train(*head4, out, *rest)
+    ref = pipeline.filter_train(*head4, ref_out.float().to(dev), *rest)
+    e_rad = rel_l2(got[1].cpu().numpy(), ref[1].cpu().numpy())
+    e_dp = rel_l2(got[4].cpu().numpy(), ref[4].cpu().numpy())
+
+    # weight gradients for the same upstream gradient (the unrounded chain's)
+    dparam = ref[4].double().cpu()
+    net.backward(ref[4].contiguous())
+    torch.cuda.synchronize()
+    gw = [(w.cpu().double(), b.cpu().double()) for w, b in cnn.unet_unflatten(net.grad)]
+    g_ref, _ = unet_ref_backward(L, ref_a, x0, dparam)
+    e_w = max(rel_l2(gw[l][0].numpy(), g_ref[l][0].numpy()) for l in range(12))
+    e_b = max(rel_l2(gw[l][1].numpy(), g_ref[l][1].numpy()) for l in range(12))
+    print(f"  unet {H}x{W} vs unrounded float64: output {e_out:.2e}, weight gradients <= {e_w:.2e}, "
+          f"bias gradients <= {e_b:.2e}; downstream radiance {e_rad:.2e}, network-output gradient {e_dp:.2e}",
+          flush=True)
+    assert e_out <= 3e-2 and e_w <= 1e-1 and e_b <= 1e-1 and e_rad <= 1e-2 and e_dp <= 1e-1
+
+
+def to_dev_np(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
