@@ -44,6 +44,7 @@ PROTOTYPES = {
     "xdn_erode_temporal_screening": (_I, [_P] * 11 + [_I] * 5 + [_P]),
     "xdn_filter_train_workspace_size": (_Z, [_I, _I, _I]),
     "xdn_filter_train": (_I, [_P] * 16 + [_P, _Z] + [_I] * 4 + [_P]),
+    "xdn_filter_train_noalloc": (_I, [_P] * 16 + [_I] * 4 + [_P]),
     "xdn_adam": (_I, [_P] * 4 + [_I, _I, _F, _F, _F, _F, _P]),
     "xdn_frame_error_workspace_size": (_Z, [_I, _I, _I]),
     "xdn_frame_error": (_I, [_P] * 8 + [_P, _Z] + [_I] * 3 + [_P]),

@@ -6,8 +6,9 @@
  *   - Every image is a contiguous NHWC float32 device buffer [B, H, W, C] (C stated per argument),
  *     the layout of the reference ops (scripts/ops.py). B images are processed independently.
  *   - `stream` is a hipStream_t (NULL = default stream). All work is enqueued on it; no entry
- *     point synchronizes, allocates device memory or keeps device state between calls, so every
- *     call can be captured in a hipGraph.
+ *     point synchronizes, allocates device memory or keeps device state between calls (the one
+ *     exception, xdn_filter_train_noalloc, uses a fixed array of the library instead of a
+ *     workspace), so every call can be captured in a hipGraph.
  *   - Return value: 0 on success, XDN_EINVAL for bad arguments (nothing launched), XDN_EHIP for a
  *     HIP launch error. xdn_last_error() gives the message of the last failure on the calling
  *     thread; argument messages repeat the reference's text where it has one.
@@ -153,6 +154,21 @@ int xdn_filter_train(const float *ols1, const float *ols2, const float *albedo, 
                      const float *opacity, const float *prev_ols1, const float *prev_ols2, float *out_log, float *out,
                      float *dparam, float *loss, float *avg_loss, void *workspace, size_t workspace_bytes, int batch,
                      int height, int width, int win_size, void *stream);
+
+/* The same op with no workspace argument and no device allocation. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The per-tile loss partials go to a fixed
+ * array inside the library, one per GPU, held until the process ends:
+ * XDN_FILTER_TRAIN_SCRATCH_BYTES = 1 MiB, i.e. batch x ceil(H/16) x ceil(W/16) <= 262,144 tiles
+ * (e.g. batch 16 at 1920x1080, or one 8192x8192 image). Larger inputs return XDN_EINVAL before
+ * anything is launched. Calls on one GPU share that array: calls that overlap on the device
+ * (other streams, other threads) race and must be serialized by the caller. */
+#define XDN_FILTER_TRAIN_SCRATCH_BYTES (1u << 20)
+int xdn_filter_train_noalloc(const float *ols1, const float *ols2, const float *albedo, const float *normal,
+                             const float *param, const float *success, const float *success_prev,
+                             const float *prev_out, const float *opacity, const float *prev_ols1,
+                             const float *prev_ols2, float *out_log, float *out, float *dparam, float *loss,
+                             float *avg_loss, int batch, int height, int width, int win_size, void *stream);
 
 /* Keras Adam step (tf.keras.optimizers.Adam, scripts/main.py:81,303) on a flat parameter buffer:
  * m += (g-m)(1-b1); v += (g^2-v)(1-b2); p -= alpha m/(sqrt(v)+eps) with
