@@ -610,11 +610,27 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
     std::vector<xdn::ConvFrame> cf(batch);
     std::vector<xdn::WgFrame> wf(batch);
     std::vector<xdn::RedJob> jobs;
+    std::vector<xdn::BwdFrame> bf(batch);
     int cur = 0;  // dY ping-pong index holding the gradient of the current layer's output
     for (int l = 4; l >= 0; --l)
     {
-        // weight gradient of layer l: x = input of layer l, dy = gradient of its pre-activation
         const int nb = ws.nblk[l];
+        if (l >= 1 && l <= 3)
+        {
+            // middle layer: weight gradient and data gradient into layer l-1 in one kernel
+            for (int b = 0; b < batch; ++b)
+            {
+                const size_t o32 = (size_t)b * N * 32;
+                bf[b] = xdn::BwdFrame{ws.dY[cur] + o32, ws.X[l - 1] + o32, ws.M[l - 1] + (size_t)b * N,
+                                      ws.dY[1 - cur] + o32, pk + kLay.dg_off[l],
+                                      ws.part[l] + (size_t)b * nb * xdn::WG_PART, height, width, nb};
+            }
+            if (int e = hip_check(xdn::launch_bwd(batch, bf.data(), S(stream)), "cnn5 backward")) return e;
+            jobs.push_back(xdn::RedJob{ws.part[l], ws.stage[l], grad + kLay.w_off[l], batch * nb, kCnn5[l].ci, kCnn5[l].co});
+            cur = 1 - cur;
+            continue;
+        }
+        // weight gradient of layer l: x = input of layer l, dy = gradient of its pre-activation
         for (int b = 0; b < batch; ++b)
         {
             xdn::WgFrame &f = wf[b];

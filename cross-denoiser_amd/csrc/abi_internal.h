@@ -73,7 +73,7 @@ struct Cnn5Ws {
     uint16_t *X[4], *dY[2];
     uint32_t *M[4];  // ReLU bits of X[l], one word per pixel
     float *part[5], *stage[5];
-    int nblk[5];  // weight-gradient blocks per image and layer
+    int nblk[5];  // weight-gradient blocks (partials) per image and layer
     size_t bytes;
     Cnn5Ws(void *base, int B, int H, int W)
     {
@@ -85,7 +85,8 @@ struct Cnn5Ws {
         for (int i = 0; i < 4; ++i) M[i] = reinterpret_cast<uint32_t *>(p + o), o += al256((size_t)B * H * W * 4);
         for (int l = 0; l < 5; ++l)
         {
-            nblk[l] = xdn::wgrad_blocks(H, W);
+            // the middle layers' weight gradients come from the fused backward kernel
+            nblk[l] = (l >= 1 && l <= 3) ? xdn::bwd_blocks(H, W) : xdn::wgrad_blocks(H, W);
             part[l] = reinterpret_cast<float *>(p + o);
             o += al256((size_t)B * nblk[l] * xdn::WG_PART * 4);
             stage[l] = reinterpret_cast<float *>(p + o);

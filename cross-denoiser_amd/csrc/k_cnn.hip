@@ -20,6 +20,9 @@
 //     read channel-major with the transposing ds_read_b64_tr_b16. A wave keeps all 9 taps' 32x32
 //     FP32 accumulators (AGPRs) and loops over many 8x32-px strips; per-block partial sums are
 //     reduced by wgrad_reduce in two deterministic stages.
+//   conv3x3_bwd  backward of a 32 -> 32 layer: the data gradient (conv3x3 scheme) and the weight
+//     gradient (conv3x3_wgrad scheme) of one 16x32-px tile from one staging of dY and X, eight
+//     waves per block; used for the three middle layers.
 //   pack_weights  FP32 master weights -> bf16 MFMA B fragments for the forward and the data
 //     gradient (flipped/transposed) convolutions.
 #include "xdn_common.h"
@@ -619,6 +622,231 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
     if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fused backward of a 32 -> 32 layer
+// ---------------------------------------------------------------------------------------------
+struct BwdBatch {
+    int n;
+    int blk_begin[XDN_MAX_FRAMES + 1];
+    BwdFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(BwdBatch) <= 4096, "kernel argument too large");
+
+// One staging of a 16x32-px tile of dY and of the layer input X (both with their 1-px halo, zero
+// outside the image) feeds both products of the layer's backward pass:
+//   data gradient  dX = conv(dY, flipped/transposed W), ReLU-masked by X's bits: the conv3x3
+//                  scheme (B operands read from the dY tile with ds_read_b128);
+//   weight gradient dW[t] += X(p + shift_t)^T dY(p) over the tile's pixels: the conv3x3_wgrad
+//                  scheme (transposing reads of both tiles), kept in registers across the block's
+//                  tiles.
+// dY is read from HBM once instead of once per product. Eight waves share the work of a tile:
+// wave w computes the data gradient of tile rows 2w, 2w+1 and the weight gradient of taps
+// 0..4 (w < 4) or 5..8 (w >= 4) over tile rows 4(w%4) .. 4(w%4)+3, so each wave holds 80 + 32
+// accumulators and the next tile's prefetch, which fits two waves per SIMD (one 512-lane block per
+// CU: the two staged tiles and the kernel fragments take 107 KB of LDS). The block walks a fixed
+// set of its frame's tiles (xcd_strips over tiles) and writes one partial, so a frame's gradient is
+// the same alone and in a batch.
+__global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
+{
+    constexpr int PS = 40;                 // dY pixel stride in bf16 (conflict-free b128 reads)
+    constexpr int XS = 32;                 // X pixel stride
+    constexpr int KC = 2;
+    constexpr int LW = CT_W + 2, LH = CT_H + 2;
+    constexpr int NCH = LH * LW * 4;       // 16-B chunks of one staged tile
+    constexpr int NT = 512;
+    constexpr int NV = (NCH + NT - 1) / NT;
+    __shared__ __attribute__((aligned(16))) uint16_t dys[LH * LW * PS];
+    __shared__ __attribute__((aligned(16))) uint16_t xs[LH * LW * XS];
+    __shared__ u32x4 wl[9 * KC * 64];
+    __shared__ float dbs[4][32];
+    static_assert(LH * LW * PS * 2 >= 4 * 1024 * 4, "partial reduction space");
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const BwdFrame &F = b.f[f];
+    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
+    const int H = F.H, W = F.W;
+    const int ntx = (W + CT_W - 1) / CT_W, nty = (H + CT_H - 1) / CT_H;
+    const int tid = threadIdx.x, lane = tid & 63, px = lane & 31, h = lane >> 5, ch = lane & 31;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    const int rg = wv & 3, tg = wv >> 2;  // weight-gradient row group and tap group
+    const int grp = (lane >> 4) & 1, s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3;
+    const int cb = 16 * grp + 4 * qq;      // channel offset addressed by this lane in transposing reads
+    const uint16_t *gdy = static_cast<const uint16_t *>(F.dy), *gx = static_cast<const uint16_t *>(F.x);
+
+    for (int e = tid; e < 9 * KC * 64; e += NT) wl[e] = static_cast<const u32x4 *>(F.wfrag)[e];
+
+    f32x16 wacc[5];  // taps 5 tg + j
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) wacc[j][k] = 0.f;
+    float db = 0.f;
+
+    u32x4 rd[NV], rx[NV];
+    auto fetch = [&](int t) {
+        const int y0 = (t / ntx) * CT_H, x0 = (t % ntx) * CT_W;
+#pragma unroll
+        for (int u = 0; u < NV; ++u)
+        {
+            const int i = tid + NT * u;
+            const int pix = i >> 2, q = i & 3;
+            const int ly = pix / LW, lx = pix - ly * LW;
+            const int gy = y0 + ly - 1, gxx = x0 + lx - 1;
+            rd[u] = u32x4{0u, 0u, 0u, 0u};
+            rx[u] = u32x4{0u, 0u, 0u, 0u};
+            if (i < NCH && gy >= 0 && gy < H && gxx >= 0 && gxx < W)
+            {
+                const size_t o = ((size_t)gy * W + gxx) * 32 + q * 8;
+                rd[u] = *reinterpret_cast<const u32x4 *>(gdy + o);
+                rx[u] = *reinterpret_cast<const u32x4 *>(gx + o);
+            }
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int u = 0; u < NV; ++u)
+        {
+            const int i = tid + NT * u;
+            if (i < NCH)
+            {
+                const int pix = i >> 2, q = i & 3;
+                *reinterpret_cast<u32x4 *>(&dys[pix * PS + q * 8]) = rd[u];
+                *reinterpret_cast<u32x4 *>(&xs[pix * XS + q * 8]) = rx[u];
+            }
+        }
+    };
+
+    const StripWalk sw = xcd_strips(li, nblk, ntx * nty);
+    if (sw.first < sw.end) fetch(sw.first);
+    for (int t = sw.first; t < sw.end; t += sw.step)
+    {
+        const int y0 = (t / ntx) * CT_H, x0 = (t % ntx) * CT_W;
+        __syncthreads();  // every wave is done reading the previous tile
+        commit();
+        __syncthreads();
+        if (t + sw.step < sw.end) fetch(t + sw.step);
+        uint32_t mrow[2];
+#pragma unroll
+        for (int orow = 0; orow < 2; ++orow)
+        {
+            const int gy = y0 + 2 * wv + orow, gxx = x0 + px;
+            mrow[orow] = (gy < H && gxx < W) ? F.mask[(size_t)gy * W + gxx] : 0u;
+        }
+
+        // ---- weight gradient: taps 5 tg .. over rows 4 rg + 2 pr + h, 8-px K chunks c ----
+#pragma unroll 1
+        for (int pr = 0; pr < 2; ++pr)
+        {
+            const int rr = 4 * rg + 2 * pr + h;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                const uint16_t *dp = &dys[((rr + 1) * LW + 8 * c + jq + 1) * PS + cb];
+                const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * PS);
+                const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
+                if (tg == 0)
+                    db += (bf_lo(B.x) + bf_hi(B.x)) + (bf_lo(B.y) + bf_hi(B.y)) + (bf_lo(B.z) + bf_hi(B.z)) +
+                          (bf_lo(B.w) + bf_hi(B.w));
+#pragma unroll
+                for (int j = 0; j < 5; ++j)
+                {
+                    if (tg == 1 && j == 4) continue;  // taps 5..8 only
+                    const int tap = 5 * tg + j, dy = tap / 3, sh = tap - 3 * dy;
+                    const uint16_t *xp = &xs[((rr + dy) * LW + 8 * c + sh + jq) * XS + cb];
+                    const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
+                    const u32x4 A = {a0.x, a0.y, a1.x, a1.y};
+                    wacc[j] = mfma(A, B, wacc[j]);
+                }
+            }
+        }
+
+        // ---- data gradient: output rows 2wv, 2wv+1 (the conv3x3 loop on the dY tile) ----
+        f32x16 acc[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc[r][k] = 0.f;
+#pragma unroll 1
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+            {
+                u32x4 wk[3], bx[4];
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) wk[dy] = wl[((dy * 3 + dx) * KC + kc) * 64 + lane];
+#pragma unroll
+                for (int lr = 0; lr < 4; ++lr)
+                    bx[lr] = *reinterpret_cast<const u32x4 *>(&dys[((2 * wv + lr) * LW + px + dx) * PS + kc * 16 + 8 * h]);
+#pragma unroll
+                for (int lr = 0; lr < 4; ++lr)
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy)
+                    {
+                        const int orow = lr - dy;
+                        if (orow >= 0 && orow < 2) acc[orow] = mfma(wk[dy], bx[lr], acc[orow]);
+                    }
+            }
+        // epilogue: keep the gradient where X's bf16 value is > 0, 16-B stores after permlane32 swaps
+#pragma unroll
+        for (int orow = 0; orow < 2; ++orow)
+        {
+            const int gy = y0 + 2 * wv + orow, gxx = x0 + px;
+            uint32_t d[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) d[q] = pack2(acc[orow][2 * q], acc[orow][2 * q + 1]);
+            const uint32_t m = mrow[orow] >> (4 * h);
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+            {
+                const int pos = 8 * (q >> 1) + 2 * (q & 1);
+                const uint32_t lo = (m >> pos) & 1u ? 0x0000FFFFu : 0u;
+                const uint32_t hi = (m >> (pos + 1)) & 1u ? 0xFFFF0000u : 0u;
+                d[q] &= lo | hi;
+            }
+            u32x4 v[2];
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                {
+                    const auto s = __builtin_amdgcn_permlane32_swap(d[4 * g + e], d[4 * g + 2 + e], false, false);
+                    v[g][e] = (uint32_t)s[0];
+                    v[g][2 + e] = (uint32_t)s[1];
+                }
+            if (gy < H && gxx < W)
+            {
+                uint16_t *o = static_cast<uint16_t *>(F.dx) + ((size_t)gy * W + gxx) * 32 + 8 * h;
+                *reinterpret_cast<u32x4 *>(o) = v[0];
+                *reinterpret_cast<u32x4 *>(o + 16) = v[1];
+            }
+        }
+    }
+
+    // partial: [t][ci][co] then [co]; tap t's four row-group waves are added in LDS
+    float *out = F.part + (size_t)li * WG_PART;
+    float *red = reinterpret_cast<float *>(dys);  // [4 row groups][1024]
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+    {
+        const int g = t < 5 ? 0 : 1, j = t - 5 * g;
+        __syncthreads();
+        if (tg == g)
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+            {
+                const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
+                red[rg * 1024 + ci * 32 + ch] = wacc[j][k];
+            }
+        __syncthreads();
+        for (int e = tid; e < 1024; e += NT)
+            out[t * 1024 + e] = (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]);
+    }
+    db += __shfl_xor(db, 32);
+    if (tg == 0 && h == 0) dbs[rg][ch] = db;
+    __syncthreads();
+    if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
+}
+
 // Two-stage deterministic reduction of the per-block partials, batched over jobs (a job is one
 // layer of one frame). Stage 1: stage[j][split][e] = sum of the partials split*P/S .. of job j;
 // stage 2 writes the flat gradient of the layer ([9][CI][CO] then [CO], unpadded).
@@ -811,6 +1039,32 @@ hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st)
     case CONV_L5_WGRAD: return launch_wgrad_t<CONV_IN_BF16, 32, CONV_IN_F32_6>(n, fr, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+int bwd_blocks(int H, int W)
+{
+    const int tiles = div_up(W, CT_W) * div_up(H, CT_H);
+    // ~16 tiles per block: one block per CU at 1920x1080 (4080 tiles)
+    int nb = div_up(tiles, 16);
+    if (nb > tiles) nb = tiles;
+    return nb < 1 ? 1 : nb;
+}
+
+hipError_t launch_bwd(int n, const BwdFrame *fr, hipStream_t st)
+{
+    BwdBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.blk_begin[i] = acc;
+        acc += fr[i].nblk;
+    }
+    b.blk_begin[n] = acc;
+    if (acc == 0) return hipSuccess;
+    hipLaunchKernelGGL(conv3x3_bwd, dim3(acc), dim3(512), 0, st, b);
+    return hipGetLastError();
 }
 
 hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st)
