@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--pipeline", action="store_true",
                     help="overlap frame t+1's cross-regression with frame t on a second HIP stream")
+    ap.add_argument("--graphs", action="store_true",
+                    help="each step replays the stream's captured HIP graph (one per frame buffer and history "
+                         "parity, captured during the warm-up, which then covers every frame)")
     ap.add_argument("--no-unet", action="store_true",
                     help="skip the secondary measurement of the step with the reference's U-Net")
     ap.add_argument("--latency-frames", type=int, default=256,
@@ -365,9 +368,10 @@ def main():
     H, W = args.height, args.width
     spec = synth.StreamSpec(H, W, seed=args.seed, pan=None, pan_max=3.0, disocclusion=0.10)
     frames = [synth.make_frame(spec, t, device=dev) for t in range(args.frames)]
-    den = pipeline.Denoiser(H, W, device=dev)
+    den = pipeline.Denoiser(H, W, device=dev, use_graphs=args.graphs)
+    warmup = max(args.warmup, args.frames + 1) if args.graphs else args.warmup
     torch.cuda.synchronize()
-    print(f"[bench] {H}x{W}, {args.frames} frames resident, warmup {args.warmup}", file=sys.stderr, flush=True)
+    print(f"[bench] {H}x{W}, {args.frames} frames resident, warmup {warmup}", file=sys.stderr, flush=True)
     st = torch.cuda.current_stream()
     aux = torch.cuda.Stream(dev)
 
@@ -393,19 +397,24 @@ def main():
         lat = np.array([evs[i].elapsed_time(evs[i + 1]) for i in range(n)])
         return max(wall, evs[0].elapsed_time(evs[-1]) / 1e3), lat
 
-    pipelined = args.pipeline
-    run(args.warmup, 0, pipelined)
+    pipelined = args.pipeline and not args.graphs
+    run(warmup, 0, pipelined)
     torch.cuda.synchronize()
-    secs, _ = timed(args.steps, args.warmup, pipelined)
+    secs, _ = timed(args.steps, warmup, pipelined)
     fps = args.steps / secs
-    print(f"[bench] timed {args.steps} frames ({'pipelined' if pipelined else 'frame by frame'}): "
-          f"{fps:.1f} frames/s", file=sys.stderr, flush=True)
-    # the other schedule over the same frames, for reference
-    secs_other, _ = timed(args.steps, args.warmup, not pipelined)
+    sched = "captured step graphs" if args.graphs else ("pipelined" if pipelined else "frame by frame")
+    print(f"[bench] timed {args.steps} frames ({sched}): {fps:.1f} frames/s", file=sys.stderr, flush=True)
+    # the other schedule over the same frames, for reference (with graphs: the eager steps)
+    if args.graphs:
+        den.use_graphs = False
+        secs_other, _ = timed(args.steps, warmup, False)
+        den.use_graphs = True
+    else:
+        secs_other, _ = timed(args.steps, warmup, not pipelined)
 
     # denoising quality on the last timed frame (RelL2 of scripts/main.py:191-196 against the
     # generator's clean reference), outside the timed region
-    last = frames[(args.warmup + args.steps - 1) % args.frames]
+    last = frames[(warmup + args.steps - 1) % args.frames]
     quality = {"relL2_output": round(pipeline.rel_l2(den.outputs()["out"], last["ref"]), 5),
                "relL2_noisy_input": round(pipeline.rel_l2(0.5 * (last["color"] + last["color2"]), last["ref"]), 5)}
     stages = stage_times(den, frames, args.kernel_reps, torch)
@@ -441,7 +450,7 @@ def main():
         cpu = cpu_baseline(torch, H, W)
     line = {
         "metric": "1080p denoised frames/sec (fwd+online bwd) + p99 frame latency, 1 GPU",
-        "value": round(fps, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": round(fps, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": warmup,
         "ms_per_step": round(1e3 * secs / args.steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": round(fps / PUBLISHED_FPS, 3),
         "vs_baseline_basis": "paper Table 3 total per 1920x1080 frame (22.26 ms, RTX 4090), which trains the "
@@ -453,9 +462,11 @@ def main():
                                "(cnn5 5x conv3x3 32ch)", "height": H, "width": W, "frames": args.frames,
                    "parallelism": "single stream"},
         "p50_ms": latency["p50_ms"], "p99_ms": latency["p99_ms"], "latency": latency,
-        "schedule": "frame t+1's cross-regression overlaps frame t on a second HIP stream" if pipelined else
-                    "frame by frame, one HIP stream",
-        ("ms_per_step_frame_by_frame" if pipelined else "ms_per_step_pipelined"): round(1e3 * secs_other / args.steps, 3),
+        "schedule": "frame by frame, each step the stream's captured HIP graph" if args.graphs else
+                    ("frame t+1's cross-regression overlaps frame t on a second HIP stream" if pipelined else
+                     "frame by frame, one HIP stream"),
+        ("ms_per_step_eager" if args.graphs else ("ms_per_step_frame_by_frame" if pipelined else "ms_per_step_pipelined")):
+            round(1e3 * secs_other / args.steps, 3),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "quality": quality,
         "reference_network_step": unet,

@@ -51,6 +51,26 @@ __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
                                                    0, 0, 0);
 }
 
+// Raw buffer accesses: a load outside [0, bytes) of its buffer returns zero and a store there is
+// dropped, so guarded accesses (image borders, partial tiles, optional outputs) need no branch. With
+// no branch around any vector memory access, the compiler knows exactly how many of them were issued
+// after a prefetch and waits for the prefetch alone, not for the stores issued behind it.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
+typedef __attribute__((ext_vector_type(3))) unsigned int u32x3;
+constexpr uint32_t kOob = 0x80000000u;  // offset past every buffer (all are smaller than 2 GiB)
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, size_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, p ? (int)bytes : 0, 0x00020000);
+}
+__device__ __forceinline__ u32x4 bld16(rsrc_t r, uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0); }
+__device__ __forceinline__ u32x3 bld12(rsrc_t r, uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0); }
+__device__ __forceinline__ u32x2 bld8(rsrc_t r, uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0); }
+__device__ __forceinline__ uint32_t bld4(rsrc_t r, uint32_t off) { return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0); }
+__device__ __forceinline__ void bst16(u32x4 v, rsrc_t r, uint32_t off) { __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0); }
+__device__ __forceinline__ void bst8(u32x2 v, rsrc_t r, uint32_t off) { __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0); }
+__device__ __forceinline__ void bst4(uint32_t v, rsrc_t r, uint32_t off) { __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0); }
+
 // ---------------------------------------------------------------------------------------------
 // Forward / data-gradient convolution
 // ---------------------------------------------------------------------------------------------
@@ -92,10 +112,17 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
     constexpr int NF = IN == CONV_IN_IMAGES ? 15 : 6;
     u32x4 rv[IN == CONV_IN_BF16 ? NV : 1];
     float fv[IN == CONV_IN_BF16 ? 1 : NV][NF];
+    // every load is issued (zero outside the image: offset kOob), so the loop's memory accesses
+    // carry no branches
     auto fetch = [&](int it) {
         const TileLoc T = locate(b.map, xcd_tile(it, total));
         const ConvFrame &G = b.f[T.f];
         const int gH = G.H, gW = G.W, tx0 = T.tx * CT_W, ty0 = T.ty * CT_H;
+        const size_t npx = (size_t)gH * gW;
+        rsrc_t rin[IN == CONV_IN_IMAGES ? 5 : 1];
+#pragma unroll
+        for (int im = 0; im < (IN == CONV_IN_IMAGES ? 5 : 1); ++im)
+            rin[im] = make_rsrc(G.in[im], npx * (IN == CONV_IN_BF16 ? CIN * 2 : (IN == CONV_IN_IMAGES ? 12 : 24)));
 #pragma unroll
         for (int u = 0; u < NV; ++u)
         {
@@ -105,35 +132,35 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
                 const int pix = i / (CIN / 8), q = i - pix * (CIN / 8);
                 const int ly = pix / LW, lx = pix - ly * LW;
                 const int gy = ty0 + ly - 1, gx = tx0 + lx - 1;
-                rv[u] = u32x4{0u, 0u, 0u, 0u};
-                if (i < LH * LW * (CIN / 8) && gy >= 0 && gy < gH && gx >= 0 && gx < gW)
-                    rv[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(G.in[0]) +
-                                                             ((size_t)gy * gW + gx) * CIN + q * 8);
+                const bool ok = i < LH * LW * (CIN / 8) && gy >= 0 && gy < gH && gx >= 0 && gx < gW;
+                rv[u] = bld16(rin[0], ok ? (uint32_t)((gy * gW + gx) * CIN + q * 8) * 2u : kOob);
             }
             else
             {
                 const int pix = tid + 256 * u;
                 const int ly = pix / LW, lx = pix - ly * LW;
                 const int gy = ty0 + ly - 1, gx = tx0 + lx - 1;
-#pragma unroll
-                for (int k = 0; k < NF; ++k) fv[u][k] = 0.f;
-                if (pix < LH * LW && gy >= 0 && gy < gH && gx >= 0 && gx < gW)
+                const bool ok = pix < LH * LW && gy >= 0 && gy < gH && gx >= 0 && gx < gW;
+                const uint32_t p = (uint32_t)(gy * gW + gx);
+                if constexpr (IN == CONV_IN_IMAGES)
                 {
-                    const size_t p = (size_t)gy * gW + gx;
-                    if constexpr (IN == CONV_IN_IMAGES)
-                    {
 #pragma unroll
-                        for (int im = 0; im < 5; ++im)
-                        {
-                            const float *sp = static_cast<const float *>(G.in[im]) + 3 * p;
-                            fv[u][3 * im] = sp[0]; fv[u][3 * im + 1] = sp[1]; fv[u][3 * im + 2] = sp[2];
-                        }
+                    for (int im = 0; im < 5; ++im)
+                    {
+                        const u32x3 v = bld12(rin[im], ok ? p * 12u : kOob);
+                        fv[u][3 * im] = __uint_as_float(v.x);
+                        fv[u][3 * im + 1] = __uint_as_float(v.y);
+                        fv[u][3 * im + 2] = __uint_as_float(v.z);
                     }
-                    else
-                    {
-                        const float *sp = static_cast<const float *>(G.in[0]) + 6 * p;
+                }
+                else
+                {
 #pragma unroll
-                        for (int k = 0; k < 6; ++k) fv[u][k] = sp[k];
+                    for (int k = 0; k < 3; ++k)
+                    {
+                        const u32x2 v = bld8(rin[0], ok ? p * 24u + 8u * k : kOob);
+                        fv[u][2 * k] = __uint_as_float(v.x);
+                        fv[u][2 * k + 1] = __uint_as_float(v.y);
                     }
                 }
             }
@@ -168,36 +195,54 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
         }
     };
 
-    float bias[16];  // bias of the lane's channels c(k)
+    // The frame's kernel fragments and bias go to LDS when the frame changes. A tile's staging is
+    // committed at the end of the previous iteration, behind its stores: waiting for the prefetch
+    // there does not wait for those stores.
+    __shared__ float4 wb[8];  // bias of the current frame, [32]
+    auto frame_setup = [&](int fi) {
+        const ConvFrame &G = b.f[fi];
+        for (int e = tid; e < 9 * KC * 64; e += 256) wl[e] = static_cast<const u32x4 *>(G.wfrag)[e];
+        if (tid < 32) reinterpret_cast<float *>(wb)[tid] = (EPI == CONV_EPI_MASK) ? 0.f : G.bias[tid];
+    };
     int cur_f = -1;
-    if ((int)blockIdx.x < total) fetch(blockIdx.x);
+    if ((int)blockIdx.x < total)
+    {
+        fetch(blockIdx.x);
+        cur_f = locate(b.map, xcd_tile(blockIdx.x, total)).f;
+        frame_setup(cur_f);
+        commit();
+    }
     for (int it = blockIdx.x; it < total; it += gridDim.x)
     {
     const TileLoc L = locate(b.map, xcd_tile(it, total));
     const ConvFrame &F = b.f[L.f];
     const int H = F.H, W = F.W;
     const int x0 = L.tx * CT_W, y0 = L.ty * CT_H;
-    __syncthreads();  // every wave is done reading the previous tile from LDS
-    if (L.f != cur_f)
-    {
-        for (int e = tid; e < 9 * KC * 64; e += 256) wl[e] = static_cast<const u32x4 *>(F.wfrag)[e];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) bias[k] = (EPI == CONV_EPI_MASK) ? 0.f : F.bias[8 * (k >> 2) + 4 * h + (k & 3)];
-        cur_f = L.f;
-    }
-    commit();
-    __syncthreads();
-    if (it + (int)gridDim.x < total) fetch(it + gridDim.x);
+    __syncthreads();  // tile `it` and its frame's fragments are in LDS
     // the backward's ReLU bits of the lane's pixel in each of its 4 output rows, loaded now so their
-    // latency hides behind the MFMAs
+    // latency hides behind the MFMAs, and before the prefetch so that waiting for them does not wait
+    // for it
     uint32_t mrow[4] = {0u, 0u, 0u, 0u};
     if constexpr (EPI == CONV_EPI_MASK)
+    {
+        const rsrc_t rm = make_rsrc(F.mask, (size_t)H * W * 4);
 #pragma unroll
         for (int orow = 0; orow < 4; ++orow)
         {
             const int gy = y0 + 4 * wv + orow, gx = x0 + px;
-            if (gy < H && gx < W) mrow[orow] = F.mask[(size_t)gy * W + gx];
+            mrow[orow] = bld4(rm, gy < H && gx < W ? (uint32_t)(gy * W + gx) * 4u : kOob);
         }
+    }
+    const int nxt = it + (int)gridDim.x;
+    // the next tile's loads (the last iteration reloads its own tile: no branch around the loads)
+    fetch(nxt < total ? nxt : it);
+    float bias[16];  // bias of the lane's channels c(k) = 8(k/4) + 4h + k%4
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+        const float4 bv = wb[2 * j + h];
+        bias[4 * j] = bv.x; bias[4 * j + 1] = bv.y; bias[4 * j + 2] = bv.z; bias[4 * j + 3] = bv.w;
+    }
 
     f32x16 acc[4];
 #pragma unroll
@@ -229,28 +274,23 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
         }
 
     // ---- epilogue, per output row: lane (px, h) holds channels c(k) of pixel (row, x0 + px) ----
+    const size_t npx = (size_t)H * W;
+    const rsrc_t rout = make_rsrc(F.out, npx * (EPI == CONV_EPI_F32_6 ? 24 : 64));
+    const rsrc_t rmo = make_rsrc(EPI == CONV_EPI_RELU ? F.mask_out : nullptr, npx * 4);
 #pragma unroll
     for (int orow = 0; orow < 4; ++orow)
     {
         const int gy = y0 + 4 * wv + orow, gx = x0 + px;
         const bool in = gy < H && gx < W;
-        const size_t p = (size_t)gy * W + gx;
+        const uint32_t p = (uint32_t)(gy * W + gx);
         if constexpr (EPI == CONV_EPI_F32_6)
         {
             // network output: channels 0..3 on the h = 0 lane, 4..5 on the h = 1 lane
-            float *o = static_cast<float *>(F.out) + p * 6;
             const float v0 = acc[orow][0] + bias[0], v1 = acc[orow][1] + bias[1];
             const float v2 = acc[orow][2] + bias[2], v3 = acc[orow][3] + bias[3];
-            if (in)
-            {
-                if (h == 0)
-                {
-                    *reinterpret_cast<float2 *>(o) = make_float2(v0, v1);
-                    *reinterpret_cast<float2 *>(o + 2) = make_float2(v2, v3);
-                }
-                else
-                    *reinterpret_cast<float2 *>(o + 4) = make_float2(v0, v1);
-            }
+            const uint32_t o = in ? p * 24u + 16u * h : kOob;
+            bst8(u32x2{__float_as_uint(v0), __float_as_uint(v1)}, rout, o);
+            bst8(u32x2{__float_as_uint(v2), __float_as_uint(v3)}, rout, h == 0 ? o + 8u : kOob);
         }
         else
         {
@@ -270,7 +310,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
                 for (int k = 0; k < 16; ++k) bits |= (r[k] > kBf16Tiny ? 1u : 0u) << (8 * (k >> 2) + (k & 3));
                 bits <<= 4 * h;
                 const auto pb = __builtin_amdgcn_permlane32_swap(bits, bits, false, false);
-                if (F.mask_out && in && h == 0) F.mask_out[p] = bits | (uint32_t)pb[1];
+                bst4(bits | (uint32_t)pb[1], rmo, in && h == 0 ? p * 4u : kOob);
             }
             else
             {
@@ -297,13 +337,21 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
                     v[g][e] = (uint32_t)sw[0];
                     v[g][2 + e] = (uint32_t)sw[1];
                 }
-            if (in)
-            {
-                uint16_t *o = static_cast<uint16_t *>(F.out) + p * 32 + 8 * h;
-                *reinterpret_cast<u32x4 *>(o) = v[0];
-                *reinterpret_cast<u32x4 *>(o + 16) = v[1];
-            }
+            const uint32_t o = in ? p * 64u + 16u * h : kOob;
+            bst16(v[0], rout, o);
+            bst16(v[1], rout, in ? o + 32u : kOob);
         }
+    }
+    __syncthreads();  // every wave is done reading tile `it`
+    if (nxt < total)
+    {
+        const int nf = locate(b.map, xcd_tile(nxt, total)).f;
+        if (nf != cur_f)
+        {
+            frame_setup(nf);
+            cur_f = nf;
+        }
+        commit();
     }
     }  // tile loop
 }
@@ -682,6 +730,11 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         for (int k = 0; k < 16; ++k) wacc[j][k] = 0.f;
     float db = 0.f;
 
+    // Branch-free buffer accesses (zero / dropped outside the image), so the wait for a prefetch
+    // at its commit, placed behind the tile's stores, does not wait for those stores.
+    const size_t npx = (size_t)H * W;
+    const rsrc_t rdy = make_rsrc(gdy, npx * 64), rxx = make_rsrc(gx, npx * 64);
+    const rsrc_t rmask = make_rsrc(F.mask, npx * 4), rdx = make_rsrc(F.dx, npx * 64);
     u32x4 rd[NV], rx[NV];
     auto fetch = [&](int t) {
         const int y0 = (t / ntx) * CT_H, x0 = (t % ntx) * CT_W;
@@ -692,14 +745,10 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
             const int pix = i >> 2, q = i & 3;
             const int ly = pix / LW, lx = pix - ly * LW;
             const int gy = y0 + ly - 1, gxx = x0 + lx - 1;
-            rd[u] = u32x4{0u, 0u, 0u, 0u};
-            rx[u] = u32x4{0u, 0u, 0u, 0u};
-            if (i < NCH && gy >= 0 && gy < H && gxx >= 0 && gxx < W)
-            {
-                const size_t o = ((size_t)gy * W + gxx) * 32 + q * 8;
-                rd[u] = *reinterpret_cast<const u32x4 *>(gdy + o);
-                rx[u] = *reinterpret_cast<const u32x4 *>(gx + o);
-            }
+            const bool ok = i < NCH && gy >= 0 && gy < H && gxx >= 0 && gxx < W;
+            const uint32_t o = ok ? (uint32_t)((gy * W + gxx) * 32 + q * 8) * 2u : kOob;
+            rd[u] = bld16(rdy, o);
+            rx[u] = bld16(rxx, o);
         }
     };
     auto commit = [&]() {
@@ -717,21 +766,25 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
     };
 
     const StripWalk sw = xcd_strips(li, nblk, ntx * nty);
-    if (sw.first < sw.end) fetch(sw.first);
+    if (sw.first < sw.end)
+    {
+        fetch(sw.first);
+        commit();
+    }
     for (int t = sw.first; t < sw.end; t += sw.step)
     {
         const int y0 = (t / ntx) * CT_H, x0 = (t % ntx) * CT_W;
-        __syncthreads();  // every wave is done reading the previous tile
-        commit();
-        __syncthreads();
-        if (t + sw.step < sw.end) fetch(t + sw.step);
+        __syncthreads();  // tile t is in LDS
+        // the ReLU bits first: waiting for them in the epilogue then does not wait for the prefetch
         uint32_t mrow[2];
 #pragma unroll
         for (int orow = 0; orow < 2; ++orow)
         {
             const int gy = y0 + 2 * wv + orow, gxx = x0 + px;
-            mrow[orow] = (gy < H && gxx < W) ? F.mask[(size_t)gy * W + gxx] : 0u;
+            mrow[orow] = bld4(rmask, gy < H && gxx < W ? (uint32_t)(gy * W + gxx) * 4u : kOob);
         }
+        const int nxt = t + sw.step;
+        fetch(nxt < sw.end ? nxt : t);  // the last tile reloads itself: no branch around the loads
 
         // ---- weight gradient: taps 5 tg .. over rows 4 rg + 2 pr + h, 8-px K chunks c ----
 #pragma unroll 1
@@ -813,13 +866,13 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
                     v[g][e] = (uint32_t)s[0];
                     v[g][2 + e] = (uint32_t)s[1];
                 }
-            if (gy < H && gxx < W)
-            {
-                uint16_t *o = static_cast<uint16_t *>(F.dx) + ((size_t)gy * W + gxx) * 32 + 8 * h;
-                *reinterpret_cast<u32x4 *>(o) = v[0];
-                *reinterpret_cast<u32x4 *>(o + 16) = v[1];
-            }
+            const bool in = gy < H && gxx < W;
+            const uint32_t o = in ? (uint32_t)(gy * W + gxx) * 64u + 16u * h : kOob;
+            bst16(v[0], rdx, o);
+            bst16(v[1], rdx, in ? o + 32u : kOob);
         }
+        __syncthreads();  // every wave is done reading tile t
+        if (nxt < sw.end) commit();
     }
 
     // partial: [t][ci][co] then [co]; tap t's four row-group waves are added in LDS
@@ -981,6 +1034,8 @@ static hipError_t launch_conv_t(int n, const ConvFrame *fr, hipStream_t st)
         tx[i] = div_up(fr[i].W, CT_W);
         ty[i] = div_up(fr[i].H, CT_H);
     }
+    for (int i = 0; i < n; ++i)  // buffer offsets are 32-bit with kOob = 2^31 past every buffer
+        if ((size_t)fr[i].H * fr[i].W * 64 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
     b.map = make_tilemap(n, tx, ty);
     if (b.map.total == 0) return hipSuccess;
     // persistent grid: two resident blocks per CU, a multiple of the 8 XCDs
@@ -1052,6 +1107,8 @@ int bwd_blocks(int H, int W)
 
 hipError_t launch_bwd(int n, const BwdFrame *fr, hipStream_t st)
 {
+    for (int i = 0; i < n; ++i)  // 32-bit buffer offsets, kOob = 2^31 past every buffer
+        if ((size_t)fr[i].H * fr[i].W * 64 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
     BwdBatch b{};
     b.n = n;
     int acc = 0;
