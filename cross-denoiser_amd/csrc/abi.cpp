@@ -569,32 +569,26 @@ int xdn_cnn5_forward(const float *ols1, const float *ols2, const float *prev_out
     Cnn5Ws ws(workspace, batch, height, width);
     const size_t N = px(height, width);
     const char *pk = static_cast<const char *>(packed);
+    // layer 1 on its own (the five FP32 images in), layers 2..5 in one row-streaming launch
     std::vector<xdn::ConvFrame> fr(batch);
-    for (int l = 0; l < 5; ++l)
+    std::vector<xdn::FwdStreamFrame> sf(batch);
+    for (int b = 0; b < batch; ++b)
     {
-        for (int b = 0; b < batch; ++b)
-        {
-            xdn::ConvFrame &f = fr[b];
-            memset(&f, 0, sizeof(f));
-            const size_t o3 = (size_t)b * N * 3, o32 = (size_t)b * N * 32;
-            if (l == 0)
-            {
-                f.in[0] = ols1 + o3; f.in[1] = ols2 + o3; f.in[2] = prev_out + o3; f.in[3] = albedo + o3;
-                f.in[4] = normal + o3;
-            }
-            else
-                f.in[0] = ws.X[l - 1] + o32;
-            f.out = l < 4 ? static_cast<void *>(ws.X[l] + o32) : static_cast<void *>(param + (size_t)b * N * 6);
-            f.mask_out = l < 4 ? ws.M[l] + (size_t)b * N : nullptr;
-            f.wfrag = pk + kLay.fwd_off[l];
-            f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[l]);
-            f.H = height;
-            f.W = width;
-        }
-        const int kind = l == 0 ? xdn::CONV_L1_FWD : (l < 4 ? xdn::CONV_MID_FWD : xdn::CONV_L5_FWD);
-        if (int e = hip_check(xdn::launch_conv(kind, batch, fr.data(), S(stream)), "cnn5 forward")) return e;
+        xdn::ConvFrame &f = fr[b];
+        memset(&f, 0, sizeof(f));
+        const size_t o3 = (size_t)b * N * 3, o32 = (size_t)b * N * 32;
+        f.in[0] = ols1 + o3; f.in[1] = ols2 + o3; f.in[2] = prev_out + o3; f.in[3] = albedo + o3;
+        f.in[4] = normal + o3;
+        f.out = ws.X[0] + o32;
+        f.mask_out = ws.M[0] + (size_t)b * N;
+        f.wfrag = pk + kLay.fwd_off[0];
+        f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[0]);
+        f.H = height;
+        f.W = width;
+        sf[b] = cnn5_stream_frame(ws, pk, (size_t)b * N, param + (size_t)b * N * 6, height, width);
     }
-    return XDN_OK;
+    if (int e = hip_check(xdn::launch_conv(xdn::CONV_L1_FWD, batch, fr.data(), S(stream)), "cnn5 forward")) return e;
+    return hip_check(xdn::launch_fwd_stream(batch, sf.data(), S(stream)), "cnn5 forward");
 }
 
 int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_out, const float *albedo,

@@ -96,6 +96,27 @@ struct Cnn5Ws {
     }
 };
 
+// Layers 2..5 of a cnn5 forward for the image at pixel offset `po` of the workspace's buffers.
+inline xdn::FwdStreamFrame cnn5_stream_frame(const Cnn5Ws &ws, const char *packed, size_t po, float *out, int H, int W)
+{
+    xdn::FwdStreamFrame f{};
+    f.x1 = ws.X[0] + po * 32;
+    for (int l = 1; l < 4; ++l)
+    {
+        f.x[l - 1] = ws.X[l] + po * 32;
+        f.m[l - 1] = ws.M[l] + po;
+    }
+    f.out = out;
+    for (int l = 1; l < 5; ++l)
+    {
+        f.wfrag[l - 1] = packed + kLay.fwd_off[l];
+        f.bias[l - 1] = reinterpret_cast<const float *>(packed + kLay.bias_off[l]);
+    }
+    f.H = H;
+    f.W = W;
+    return f;
+}
+
 // ---------------------------------------------------------------------------------------------
 // unet: the reference's bandwidth U-Net (scripts/main.py:83-118). Parameters in Keras order, per
 // layer kernel then bias: 3x3 kernels [3][3][ci][co], transposed 1x1 kernels [co][ci], the head

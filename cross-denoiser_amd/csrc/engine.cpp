@@ -3,8 +3,8 @@
 // One call advances every given slot (stream) by one frame. Each stage of train_step
 // (scripts/main.py:250-313) is ONE launch over all slots -- the frame-batch tile maps of the
 // kernels handle different resolutions in the same launch -- so n streams cost as many launches
-// as one: cross-regression (3), reprojection, erosion + screening, network forward (cnn5: 5
-// convolutions; the U-Net: 14 launches), fused filter + loss + backward, loss sums, network
+// as one: cross-regression (3), reprojection, erosion + screening, network forward (cnn5: layer 1,
+// then layers 2-5 in one row-streaming launch; the U-Net: 14 launches), fused filter + loss + backward, loss sums, network
 // backward (cnn5: 6 launches, the three middle layers' weight and data gradients fused; the
 // U-Net: 23 launches), one pair of
 // weight-gradient reductions, Adam, pack.
@@ -235,30 +235,25 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
     }
     else
     {
+        // layer 1 (the five FP32 images in), then layers 2..5 in one row-streaming launch
         std::vector<xdn::ConvFrame> fr(n);
-        for (int l = 0; l < 5; ++l)
+        std::vector<xdn::FwdStreamFrame> sf(n);
+        for (int i = 0; i < n; ++i)
         {
-            for (int i = 0; i < n; ++i)
-            {
-                const xdn_slot &s = slots[i];
-                const char *pk = static_cast<const char *>(s.packed);
-                xdn::ConvFrame &f = fr[i];
-                memset(&f, 0, sizeof(f));
-                if (l == 0)
-                {
-                    f.in[0] = s.ols1; f.in[1] = s.ols2; f.in[2] = sc[i].prev_out; f.in[3] = s.albedo; f.in[4] = s.normal;
-                }
-                else
-                    f.in[0] = cw[i].X[l - 1];
-                f.out = l < 4 ? static_cast<void *>(cw[i].X[l]) : static_cast<void *>(sc[i].param);
-                f.mask_out = l < 4 ? cw[i].M[l] : nullptr;
-                f.wfrag = pk + kLay.fwd_off[l];
-                f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[l]);
-                f.H = s.height; f.W = s.width;
-            }
-            const int kind = l == 0 ? xdn::CONV_L1_FWD : (l < 4 ? xdn::CONV_MID_FWD : xdn::CONV_L5_FWD);
-            if (int e = hip_check(xdn::launch_conv(kind, n, fr.data(), st), "cnn forward")) return e;
+            const xdn_slot &s = slots[i];
+            const char *pk = static_cast<const char *>(s.packed);
+            xdn::ConvFrame &f = fr[i];
+            memset(&f, 0, sizeof(f));
+            f.in[0] = s.ols1; f.in[1] = s.ols2; f.in[2] = sc[i].prev_out; f.in[3] = s.albedo; f.in[4] = s.normal;
+            f.out = cw[i].X[0];
+            f.mask_out = cw[i].M[0];
+            f.wfrag = pk + kLay.fwd_off[0];
+            f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[0]);
+            f.H = s.height; f.W = s.width;
+            sf[i] = cnn5_stream_frame(cw[i], pk, 0, sc[i].param, s.height, s.width);
         }
+        if (int e = hip_check(xdn::launch_conv(xdn::CONV_L1_FWD, n, fr.data(), st), "cnn forward")) return e;
+        if (int e = hip_check(xdn::launch_fwd_stream(n, sf.data(), st), "cnn forward")) return e;
     }
     // 5. fused spatial filter + temporal accumulation + loss + backward to the network output
     {

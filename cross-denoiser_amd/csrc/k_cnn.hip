@@ -357,6 +357,172 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Layers 2..5 of the forward, streamed down the rows
+// ---------------------------------------------------------------------------------------------
+constexpr int FS_VALID = 26;         // output columns per strip (32-px windows lose 1 px a side per layer)
+constexpr int FS_PX = CT_W + 2;      // ring pixels: window columns -1 .. 32
+constexpr int FS_PS = 40;            // ring pixel stride in bf16 (conflict-free b128 reads)
+constexpr int FS_SKEW = 9;           // steps from a strip's first layer-2 row to its first output row
+
+struct FwdStreamBatch {
+    int n;
+    int blk_begin[XDN_MAX_FRAMES + 1];
+    int nstrip[XDN_MAX_FRAMES], seg_rows[XDN_MAX_FRAMES];
+    FwdStreamFrame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(FwdStreamBatch) <= 4096, "kernel argument too large");
+
+// A block owns a strip of FS_VALID output columns [x0, x0 + 26) and a segment of rows [ys, ye).
+// Every layer runs on the same 32-px window [x0 - 3, x0 + 29); layer k's row is exact on
+// [x0 - 5 + k, x0 + 31 - k), so layer 5 is exact on the strip. Wave w computes layer w + 2: at step
+// s its row is r = ys - 3 + s - 2w, read from the previous layer's rows r - 1 .. r + 1 in a
+// 4-row LDS ring (written in earlier steps; the previous layer writes row r + 2 in this step), and
+// writes it to its own ring (zero outside the image: the next layer's 'same' padding). Layer 1's
+// output rows stream in from HBM two steps ahead through registers. Each layer stores the rows of
+// [ys, ye) and the columns of the strip, exactly as the one-layer conv3x3 kernel computes them
+// (same MFMA order: column shift, channel chunk, kernel row), so the outputs are bit-identical.
+// One barrier per step. Kernel fragments and bias of the wave's layer sit in registers.
+__global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t ring[4][4][FS_PX * FS_PS];  // [input of layer w + 2][slot]
+    const int tid = threadIdx.x, lane = tid & 63, px = lane & 31, h = lane >> 5;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const FwdStreamFrame &F = b.f[f];
+    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
+    const int nstrip = b.nstrip[f];
+    const int tile = xcd_tile(li, nblk);  // strip-fastest: neighbouring strips share an XCD
+    const int strip = tile % nstrip, seg = tile / nstrip;
+    const int H = F.H, W = F.W;
+    const int x0 = strip * FS_VALID, xw = x0 - 3;
+    const int ys = seg * b.seg_rows[f], ye = min(H, ys + b.seg_rows[f]);
+
+    for (int i = tid; i < 4 * 4 * FS_PX * FS_PS / 8; i += 256)
+        reinterpret_cast<u32x4 *>(&ring[0][0][0])[i] = u32x4{0u, 0u, 0u, 0u};
+
+    u32x4 wreg[18];  // [tap][chunk] fragments of this wave's layer
+    {
+        const u32x4 *wf = static_cast<const u32x4 *>(F.wfrag[wv]);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) wreg[i] = wf[i * 64 + lane];
+    }
+    float bias[16];  // channels c(k) = 8(k/4) + 4h + k%4
+#pragma unroll
+    for (int k = 0; k < 16; ++k) bias[k] = F.bias[wv][8 * (k >> 2) + 4 * h + (k & 3)];
+
+    const size_t npx = (size_t)H * W;
+    const rsrc_t rx1 = make_rsrc(F.x1, npx * 64);
+    const rsrc_t rout = make_rsrc(wv < 3 ? static_cast<const void *>(F.x[wv < 3 ? wv : 0]) : F.out,
+                                  npx * (wv < 3 ? 64 : 24));
+    const rsrc_t rmo = make_rsrc(wv < 3 ? F.m[wv < 3 ? wv : 0] : nullptr, npx * 4);
+    // layer 1's rows: chunk tid (< 136) = pixel tid / 4 of the ring (column xw - 1 + tid / 4), quarter tid % 4
+    const int lpx = tid >> 2, lq = tid & 3, lcol = xw - 1 + lpx;
+    const bool lrow = tid < FS_PX * 4 && lcol >= 0 && lcol < W;
+    auto xoff = [&](int y) -> uint32_t {
+        return lrow && y >= 0 && y < H ? (uint32_t)((y * W + lcol) * 32 + lq * 8) * 2u : kOob;
+    };
+    auto put = [&](int y, u32x4 v) {
+        if (tid < FS_PX * 4) *reinterpret_cast<u32x4 *>(&ring[0][y & 3][lpx * FS_PS + lq * 8]) = v;
+    };
+    __syncthreads();
+    {
+        const u32x4 a = bld16(rx1, xoff(ys - 4)), c = bld16(rx1, xoff(ys - 3)), d = bld16(rx1, xoff(ys - 2));
+        put(ys - 4, a);
+        put(ys - 3, c);
+        put(ys - 2, d);
+    }
+    // Two rows in flight, each followed by four (dropped) stores as in a step: the steps' waits for a
+    // prefetch then count the same accesses behind it whether it was issued here or in a step.
+    u32x4 pf0 = bld16(rx1, xoff(ys - 1));
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+    bst16(zero4, rout, kOob); bst16(zero4, rout, kOob + 32u); bst4(0u, rmo, kOob); bst8(u32x2{0u, 0u}, rout, kOob + 64u);
+    u32x4 pf1 = bld16(rx1, xoff(ys));
+    bst16(zero4, rout, kOob + 96u); bst16(zero4, rout, kOob + 128u); bst4(0u, rmo, kOob + 4u);
+    bst8(u32x2{0u, 0u}, rout, kOob + 160u);
+    __syncthreads();
+
+    const uint16_t *in = &ring[wv][0][0];
+    const int gx = xw + px;
+    const bool cin = gx >= 0 && gx < W;
+    const bool ccol = cin && px >= 3 && px < 3 + FS_VALID;  // a column this strip stores
+    auto step = [&](int s, u32x4 &pf) {
+        put(ys - 1 + s, pf);                // layer 1's row ys - 1 + s, loaded two steps ago
+        pf = bld16(rx1, xoff(ys + 1 + s));  // ... and the one two steps ahead
+        const int r = ys - 3 + s - 2 * wv;
+        f32x16 acc;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[k] = 0.f;
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+                {
+                    const u32x4 bfr = *reinterpret_cast<const u32x4 *>(
+                        &in[(((r - 1 + dy) & 3) * FS_PX + px + dx) * FS_PS + kc * 16 + 8 * h]);
+                    acc = mfma(wreg[(dy * 3 + dx) * 2 + kc], bfr, acc);
+                }
+        const bool rin = r >= 0 && r < H;
+        const bool st = rin && ccol && r >= ys && r < ye;
+        const uint32_t p = (uint32_t)(r * W + gx);
+        // Every wave issues the same four stores (the unused ones at kOob), so the number of memory
+        // accesses behind a prefetch is the same on every path and its wait excludes the stores.
+        const bool live = rin && cin;  // zero outside the image: the next layer's padding
+        float v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = live ? fmaxf(acc[k] + bias[k], 0.f) : 0.f;
+        uint32_t d[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) d[q] = pack2(v[2 * q], v[2 * q + 1]);
+        uint32_t bits = 0u;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) bits |= (v[k] > kBf16Tiny ? 1u : 0u) << (8 * (k >> 2) + (k & 3));
+        bits <<= 4 * h;
+        const auto pb = __builtin_amdgcn_permlane32_swap(bits, bits, false, false);
+        u32x4 o2[2];
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+            {
+                const auto sw = __builtin_amdgcn_permlane32_swap(d[4 * g + e], d[4 * g + 2 + e], false, false);
+                o2[g][e] = (uint32_t)sw[0];
+                o2[g][2 + e] = (uint32_t)sw[1];
+            }
+        const bool l5 = wv == 3;
+        // layer 5: the network output, channels 0..3 on the h = 0 lane (16 B), 4..5 on h = 1 (8 B)
+        const float f0 = acc[0] + bias[0], f1 = acc[1] + bias[1], f2 = acc[2] + bias[2], f3 = acc[3] + bias[3];
+        const u32x4 sA = l5 ? u32x4{__float_as_uint(f0), __float_as_uint(f1), __float_as_uint(f2), __float_as_uint(f3)}
+                            : o2[0];
+        const uint32_t oA = !st ? kOob : (l5 ? (h == 0 ? p * 24u : kOob) : p * 64u + 16u * h);
+        const uint32_t oB = st && !l5 ? p * 64u + 16u * h + 32u : kOob;
+        const uint32_t oM = st && !l5 && h == 0 ? p * 4u : kOob;
+        const uint32_t oC = st && l5 && h == 1 ? p * 24u + 16u : kOob;
+        bst16(sA, rout, oA);
+        bst16(o2[1], rout, oB);
+        bst4(bits | (uint32_t)pb[1], rmo, oM);
+        bst8(u32x2{__float_as_uint(f0), __float_as_uint(f1)}, rout, oC);
+        if (!l5)
+        {
+            uint16_t *nr = &ring[wv + 1][r & 3][(px + 1) * FS_PS + 8 * h];
+            *reinterpret_cast<u32x4 *>(nr) = o2[0];
+            *reinterpret_cast<u32x4 *>(nr + 16) = o2[1];
+        }
+        __syncthreads();
+    };
+    // an even number of steps (an extra last step computes rows no layer stores): the loop body is
+    // two steps with no branch, so the accesses behind each prefetch are the same every time
+    const int S = (ye - ys + FS_SKEW + 1) & ~1;
+    for (int s = 0; s < S; s += 2)
+    {
+        step(s, pf0);
+        step(s + 1, pf1);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Weight gradient
 // ---------------------------------------------------------------------------------------------
 struct WgBatch {
@@ -1121,6 +1287,33 @@ hipError_t launch_bwd(int n, const BwdFrame *fr, hipStream_t st)
     b.blk_begin[n] = acc;
     if (acc == 0) return hipSuccess;
     hipLaunchKernelGGL(conv3x3_bwd, dim3(acc), dim3(512), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_fwd_stream(int n, const FwdStreamFrame *fr, hipStream_t st)
+{
+    FwdStreamBatch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        const int H = fr[i].H, W = fr[i].W;
+        if ((size_t)H * W * 64 >= ((size_t)1 << 31)) return hipErrorInvalidValue;  // 32-bit buffer offsets
+        b.f[i] = fr[i];
+        b.nstrip[i] = div_up(W, FS_VALID);
+        // about three blocks per CU (all resident at once: 43.5 KB of LDS, <= 168 VGPRs) for one
+        // 1920x1080 frame; a segment re-runs FS_SKEW rows of its neighbour, so segments stay long.
+        // Measured: three resident blocks per CU instead of two, layers 2-5 -10 us.
+        int nseg = (768 + b.nstrip[i] / 2) / b.nstrip[i];
+        nseg = nseg < 1 ? 1 : (nseg > div_up(H, 32) ? div_up(H, 32) : nseg);
+        b.seg_rows[i] = div_up(H, nseg);
+        nseg = div_up(H, b.seg_rows[i]);
+        b.blk_begin[i] = acc;
+        acc += b.nstrip[i] * nseg;
+    }
+    b.blk_begin[n] = acc;
+    if (acc == 0) return hipSuccess;
+    hipLaunchKernelGGL(cnn5_fwd_stream, dim3(acc), dim3(256), 0, st, b);
     return hipGetLastError();
 }
 

@@ -50,6 +50,19 @@ struct BwdFrame {
     int H, W, nblk;
 };
 
+// Layers 2..5 of the cnn5 forward in one row-streaming launch: a block walks a 26-px-wide strip of
+// rows down a segment of the frame, each wave computing one layer's row per step from the rows
+// of the layer below held in LDS.
+struct FwdStreamFrame {
+    const void *x1;          // bf16 [H][W][32]: output of layer 1
+    void *x[3];              // bf16 [H][W][32]: outputs of layers 2, 3, 4
+    uint32_t *m[3];          // their ReLU bits
+    float *out;              // FP32 [H][W][6]: network output (layer 5)
+    const void *wfrag[4];    // forward B fragments [9][2][64] (16 B each) of layers 2..5
+    const float *bias[4];    // [32] bias of layers 2..5 (zero padded)
+    int H, W;
+};
+
 struct PackJob {
     const float *w, *b;      // FP32 master [9][ci][co], [co]
     void *frag_fwd, *frag_dgrad;  // dgrad may be null
@@ -79,6 +92,7 @@ hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st);
 hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st);
 int wgrad_blocks(int H, int W);
 hipError_t launch_bwd(int n, const BwdFrame *fr, hipStream_t st);
+hipError_t launch_fwd_stream(int n, const FwdStreamFrame *fr, hipStream_t st);
 int bwd_blocks(int H, int W);  // blocks (= weight-gradient partials) per image of launch_bwd
 
 }  // namespace xdn
