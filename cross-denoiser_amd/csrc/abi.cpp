@@ -609,17 +609,20 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
     for (int l = 4; l >= 0; --l)
     {
         const int nb = ws.nblk[l];
-        if (l >= 1 && l <= 3)
+        if (l >= 1)
         {
-            // middle layer: weight gradient and data gradient into layer l-1 in one kernel
+            // layers 2..5: weight gradient and data gradient into layer l-1 in one kernel (the last
+            // layer's dY is the FP32 gradient of the network output)
             for (int b = 0; b < batch; ++b)
             {
                 const size_t o32 = (size_t)b * N * 32;
-                bf[b] = xdn::BwdFrame{ws.dY[cur] + o32, ws.X[l - 1] + o32, ws.M[l - 1] + (size_t)b * N,
+                const void *dy = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6)
+                                        : static_cast<const void *>(ws.dY[cur] + o32);
+                bf[b] = xdn::BwdFrame{dy, ws.X[l - 1] + o32, ws.M[l - 1] + (size_t)b * N,
                                       ws.dY[1 - cur] + o32, pk + kLay.dg_off[l],
                                       ws.part[l] + (size_t)b * nb * xdn::WG_PART, height, width, nb};
             }
-            if (int e = hip_check(xdn::launch_bwd(batch, bf.data(), S(stream)), "cnn5 backward")) return e;
+            if (int e = hip_check(xdn::launch_bwd(batch, bf.data(), l == 4, S(stream)), "cnn5 backward")) return e;
             jobs.push_back(xdn::RedJob{ws.part[l], ws.stage[l], grad + kLay.w_off[l], batch * nb, kCnn5[l].ci, kCnn5[l].co});
             cur = 1 - cur;
             continue;

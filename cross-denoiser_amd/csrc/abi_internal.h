@@ -85,8 +85,8 @@ struct Cnn5Ws {
         for (int i = 0; i < 4; ++i) M[i] = reinterpret_cast<uint32_t *>(p + o), o += al256((size_t)B * H * W * 4);
         for (int l = 0; l < 5; ++l)
         {
-            // the middle layers' weight gradients come from the fused backward kernel
-            nblk[l] = (l >= 1 && l <= 3) ? xdn::bwd_blocks(H, W) : xdn::wgrad_blocks(H, W);
+            // the weight gradients of layers 2..5 come from the fused backward kernel
+            nblk[l] = l >= 1 ? xdn::bwd_blocks(H, W) : xdn::wgrad_blocks(H, W);
             part[l] = reinterpret_cast<float *>(p + o);
             o += al256((size_t)B * nblk[l] * xdn::WG_PART * 4);
             stage[l] = reinterpret_cast<float *>(p + o);

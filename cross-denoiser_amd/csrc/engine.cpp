@@ -5,7 +5,7 @@
 // kernels handle different resolutions in the same launch -- so n streams cost as many launches
 // as one: cross-regression (3), reprojection, erosion + screening, network forward (cnn5: layer 1,
 // then layers 2-5 in one row-streaming launch; the U-Net: 14 launches), fused filter + loss + backward, loss sums, network
-// backward (cnn5: 6 launches, the three middle layers' weight and data gradients fused; the
+// backward (cnn5: 5 launches, the weight and data gradients of layers 2-5 fused; the
 // U-Net: 23 launches), one pair of
 // weight-gradient reductions, Adam, pack.
 // Every slot owns its history, network, Adam moments and scratch; nothing is shared.
@@ -311,19 +311,21 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
         int cur = 0;
         for (int l = 4; l >= 0; --l)
         {
-            if (l >= 1 && l <= 3)
+            if (l >= 1)
             {
-                // middle layer: weight gradient and data gradient into layer l-1 in one kernel
+                // layers 2..5: weight gradient and data gradient into layer l-1 in one kernel (the
+                // last layer's dY is the FP32 gradient of the network output)
                 for (int i = 0; i < n; ++i)
                 {
                     const xdn_slot &s = slots[i];
-                    bf[i] = xdn::BwdFrame{cw[i].dY[cur], cw[i].X[l - 1], cw[i].M[l - 1], cw[i].dY[1 - cur],
+                    const void *dy = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
+                    bf[i] = xdn::BwdFrame{dy, cw[i].X[l - 1], cw[i].M[l - 1], cw[i].dY[1 - cur],
                                           static_cast<const char *>(s.packed) + kLay.dg_off[l], cw[i].part[l],
                                           s.height, s.width, cw[i].nblk[l]};
                     jobs.push_back(xdn::RedJob{cw[i].part[l], cw[i].stage[l], s.grad + kLay.w_off[l], cw[i].nblk[l],
                                                kCnn5[l].ci, kCnn5[l].co});
                 }
-                if (int e = hip_check(xdn::launch_bwd(n, bf.data(), st), "cnn backward")) return e;
+                if (int e = hip_check(xdn::launch_bwd(n, bf.data(), l == 4, st), "cnn backward")) return e;
                 cur = 1 - cur;
                 continue;
             }

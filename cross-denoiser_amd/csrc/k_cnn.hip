@@ -860,15 +860,21 @@ static_assert(sizeof(BwdBatch) <= 4096, "kernel argument too large");
 // CU: the two staged tiles and the kernel fragments take 107 KB of LDS). The block walks a fixed
 // set of its frame's tiles (xcd_strips over tiles) and writes one partial, so a frame's gradient is
 // the same alone and in a batch.
+// L5 (the last layer, 32 -> 6): dY is the FP32 6-channel gradient of the network output, staged as
+// bf16 16 channels (zero padded) like the one-layer kernels do; the data gradient has one 16-channel
+// K chunk, and the weight gradient runs on v_mfma_f32_16x16x32_bf16 (M = 16 input channels, two
+// tiles; N = the 16 output channels; K = the 32 px of one tile row), as conv3x3_wgrad does for it.
+template <bool L5>
 __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
 {
-    constexpr int PS = 40;                 // dY pixel stride in bf16 (conflict-free b128 reads)
+    constexpr int PS = L5 ? 24 : 40;       // dY pixel stride in bf16 (conflict-free b128 reads)
     constexpr int XS = 32;                 // X pixel stride
-    constexpr int KC = 2;
+    constexpr int KC = L5 ? 1 : 2;
     constexpr int LW = CT_W + 2, LH = CT_H + 2;
-    constexpr int NCH = LH * LW * 4;       // 16-B chunks of one staged tile
+    constexpr int NCH = LH * LW * 4;       // 16-B chunks of one staged X tile (and dY tile, middle layers)
     constexpr int NT = 512;
     constexpr int NV = (NCH + NT - 1) / NT;
+    constexpr int NVD = L5 ? (LH * LW + NT - 1) / NT : NV;  // dY staging items per thread (L5: pixels)
     __shared__ __attribute__((aligned(16))) uint16_t dys[LH * LW * PS];
     __shared__ __attribute__((aligned(16))) uint16_t xs[LH * LW * XS];
     __shared__ u32x4 wl[9 * KC * 64];
@@ -883,25 +889,34 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
     const int tid = threadIdx.x, lane = tid & 63, px = lane & 31, h = lane >> 5, ch = lane & 31;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int rg = wv & 3, tg = wv >> 2;  // weight-gradient row group and tap group
-    const int grp = (lane >> 4) & 1, s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3;
+    const int grp = (lane >> 4) & 1, s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3, kg = lane >> 4;
     const int cb = 16 * grp + 4 * qq;      // channel offset addressed by this lane in transposing reads
-    const uint16_t *gdy = static_cast<const uint16_t *>(F.dy), *gx = static_cast<const uint16_t *>(F.x);
+    const uint16_t *gx = static_cast<const uint16_t *>(F.x);
 
     for (int e = tid; e < 9 * KC * 64; e += NT) wl[e] = static_cast<const u32x4 *>(F.wfrag)[e];
 
-    f32x16 wacc[5];  // taps 5 tg + j
+    typedef __attribute__((ext_vector_type(4))) float f32x4;
+    f32x16 wacc[L5 ? 1 : 5];         // taps 5 tg + j (middle layers)
+    f32x4 wacc16[L5 ? 5 : 1][2];     // L5: taps 5 tg + j, input-channel tiles m
 #pragma unroll
-    for (int j = 0; j < 5; ++j)
+    for (int j = 0; j < (L5 ? 1 : 5); ++j)
 #pragma unroll
         for (int k = 0; k < 16; ++k) wacc[j][k] = 0.f;
+#pragma unroll
+    for (int j = 0; j < (L5 ? 5 : 1); ++j)
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) wacc16[j][m][k] = 0.f;
     float db = 0.f;
 
     // Branch-free buffer accesses (zero / dropped outside the image), so the wait for a prefetch
     // at its commit, placed behind the tile's stores, does not wait for those stores.
     const size_t npx = (size_t)H * W;
-    const rsrc_t rdy = make_rsrc(gdy, npx * 64), rxx = make_rsrc(gx, npx * 64);
+    const rsrc_t rdy = make_rsrc(F.dy, npx * (L5 ? 24 : 64)), rxx = make_rsrc(gx, npx * 64);
     const rsrc_t rmask = make_rsrc(F.mask, npx * 4), rdx = make_rsrc(F.dx, npx * 64);
-    u32x4 rd[NV], rx[NV];
+    u32x4 rd[L5 ? 1 : NV], rx[NV];
+    float fd[L5 ? NVD : 1][6];
     auto fetch = [&](int t) {
         const int y0 = (t / ntx) * CT_H, x0 = (t % ntx) * CT_W;
 #pragma unroll
@@ -913,9 +928,26 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
             const int gy = y0 + ly - 1, gxx = x0 + lx - 1;
             const bool ok = i < NCH && gy >= 0 && gy < H && gxx >= 0 && gxx < W;
             const uint32_t o = ok ? (uint32_t)((gy * W + gxx) * 32 + q * 8) * 2u : kOob;
-            rd[u] = bld16(rdy, o);
+            if constexpr (!L5) rd[u] = bld16(rdy, o);
             rx[u] = bld16(rxx, o);
         }
+        if constexpr (L5)
+#pragma unroll
+            for (int u = 0; u < NVD; ++u)
+            {
+                const int pix = tid + NT * u;
+                const int ly = pix / LW, lx = pix - ly * LW;
+                const int gy = y0 + ly - 1, gxx = x0 + lx - 1;
+                const bool ok = pix < LH * LW && gy >= 0 && gy < H && gxx >= 0 && gxx < W;
+                const uint32_t p = (uint32_t)(gy * W + gxx);
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                {
+                    const u32x2 v = bld8(rdy, ok ? p * 24u + 8u * k : kOob);
+                    fd[u][2 * k] = __uint_as_float(v.x);
+                    fd[u][2 * k + 1] = __uint_as_float(v.y);
+                }
+            }
     };
     auto commit = [&]() {
 #pragma unroll
@@ -925,10 +957,22 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
             if (i < NCH)
             {
                 const int pix = i >> 2, q = i & 3;
-                *reinterpret_cast<u32x4 *>(&dys[pix * PS + q * 8]) = rd[u];
+                if constexpr (!L5) *reinterpret_cast<u32x4 *>(&dys[pix * PS + q * 8]) = rd[u];
                 *reinterpret_cast<u32x4 *>(&xs[pix * XS + q * 8]) = rx[u];
             }
         }
+        if constexpr (L5)
+#pragma unroll
+            for (int u = 0; u < NVD; ++u)
+            {
+                const int pix = tid + NT * u;
+                if (pix < LH * LW)
+                {
+                    const u32x4 a = {pack2(fd[u][0], fd[u][1]), pack2(fd[u][2], fd[u][3]), pack2(fd[u][4], fd[u][5]), 0u};
+                    *reinterpret_cast<u32x4 *>(&dys[pix * PS]) = a;
+                    *reinterpret_cast<u32x4 *>(&dys[pix * PS + 8]) = u32x4{0u, 0u, 0u, 0u};
+                }
+            }
     };
 
     const StripWalk sw = xcd_strips(li, nblk, ntx * nty);
@@ -952,15 +996,43 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         const int nxt = t + sw.step;
         fetch(nxt < sw.end ? nxt : t);  // the last tile reloads itself: no branch around the loads
 
-        // ---- weight gradient: taps 5 tg .. over rows 4 rg + 2 pr + h, 8-px K chunks c ----
-#pragma unroll 1
-        for (int pr = 0; pr < 2; ++pr)
+        if constexpr (!L5)
         {
-            const int rr = 4 * rg + 2 * pr + h;
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
+            // ---- weight gradient: taps 5 tg .. over rows 4 rg + 2 pr + h, 8-px K chunks c ----
+#pragma unroll 1
+            for (int pr = 0; pr < 2; ++pr)
             {
-                const uint16_t *dp = &dys[((rr + 1) * LW + 8 * c + jq + 1) * PS + cb];
+                const int rr = 4 * rg + 2 * pr + h;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                {
+                    const uint16_t *dp = &dys[((rr + 1) * LW + 8 * c + jq + 1) * PS + cb];
+                    const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * PS);
+                    const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
+                    if (tg == 0)
+                        db += (bf_lo(B.x) + bf_hi(B.x)) + (bf_lo(B.y) + bf_hi(B.y)) + (bf_lo(B.z) + bf_hi(B.z)) +
+                              (bf_lo(B.w) + bf_hi(B.w));
+#pragma unroll
+                    for (int j = 0; j < 5; ++j)
+                    {
+                        if (tg == 1 && j == 4) continue;  // taps 5..8 only
+                        const int tap = 5 * tg + j, dy = tap / 3, sh = tap - 3 * dy;
+                        const uint16_t *xp = &xs[((rr + dy) * LW + 8 * c + sh + jq) * XS + cb];
+                        const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
+                        const u32x4 A = {a0.x, a0.y, a1.x, a1.y};
+                        wacc[j] = mfma(A, B, wacc[j]);
+                    }
+                }
+            }
+        }
+        else
+        {
+            // ---- weight gradient on 16x16x32: tile rows 4 rg + r, K = the row's 32 px ----
+#pragma unroll 1
+            for (int r = 0; r < 4; ++r)
+            {
+                const int row = 4 * rg + r;
+                const uint16_t *dp = &dys[((row + 1) * LW + 8 * kg + jq + 1) * PS + 4 * qq];
                 const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * PS);
                 const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
                 if (tg == 0)
@@ -971,10 +1043,15 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
                 {
                     if (tg == 1 && j == 4) continue;  // taps 5..8 only
                     const int tap = 5 * tg + j, dy = tap / 3, sh = tap - 3 * dy;
-                    const uint16_t *xp = &xs[((rr + dy) * LW + 8 * c + sh + jq) * XS + cb];
-                    const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
-                    const u32x4 A = {a0.x, a0.y, a1.x, a1.y};
-                    wacc[j] = mfma(A, B, wacc[j]);
+#pragma unroll
+                    for (int m = 0; m < 2; ++m)
+                    {
+                        const uint16_t *xp = &xs[((row + dy) * LW + 8 * kg + sh + jq) * XS + 16 * m + 4 * qq];
+                        const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
+                        wacc16[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y}), __builtin_bit_cast(bf16x8, B),
+                            wacc16[j][m], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -1050,18 +1127,38 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         const int g = t < 5 ? 0 : 1, j = t - 5 * g;
         __syncthreads();
         if (tg == g)
+        {
+            if constexpr (!L5)
 #pragma unroll
-            for (int k = 0; k < 16; ++k)
-            {
-                const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
-                red[rg * 1024 + ci * 32 + ch] = wacc[j][k];
-            }
+                for (int k = 0; k < 16; ++k)
+                {
+                    const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
+                    red[rg * 1024 + ci * 32 + ch] = wacc[j][k];
+                }
+            else
+                // 16x16 accumulators: column co = lane % 16, rows ci = 16m + 4kg + i
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        red[rg * 1024 + (16 * m + 4 * kg + i) * 32 + (lane & 15)] = wacc16[j][m][i];
+        }
         __syncthreads();
+        // L5: output channels 16..31 are padding (zero)
         for (int e = tid; e < 1024; e += NT)
-            out[t * 1024 + e] = (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]);
+            out[t * 1024 + e] = !L5 || (e & 31) < 16 ? (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]) : 0.f;
     }
-    db += __shfl_xor(db, 32);
-    if (tg == 0 && h == 0) dbs[rg][ch] = db;
+    if constexpr (!L5)
+    {
+        db += __shfl_xor(db, 32);
+        if (tg == 0 && h == 0) dbs[rg][ch] = db;
+    }
+    else
+    {
+        db += __shfl_xor(db, 16);
+        db += __shfl_xor(db, 32);
+        if (tg == 0 && lane < 32) dbs[rg][lane] = lane < 16 ? db : 0.f;
+    }
     __syncthreads();
     if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
 }
@@ -1271,7 +1368,7 @@ int bwd_blocks(int H, int W)
     return nb < 1 ? 1 : nb;
 }
 
-hipError_t launch_bwd(int n, const BwdFrame *fr, hipStream_t st)
+hipError_t launch_bwd(int n, const BwdFrame *fr, bool last, hipStream_t st)
 {
     for (int i = 0; i < n; ++i)  // 32-bit buffer offsets, kOob = 2^31 past every buffer
         if ((size_t)fr[i].H * fr[i].W * 64 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
@@ -1286,7 +1383,10 @@ hipError_t launch_bwd(int n, const BwdFrame *fr, hipStream_t st)
     }
     b.blk_begin[n] = acc;
     if (acc == 0) return hipSuccess;
-    hipLaunchKernelGGL(conv3x3_bwd, dim3(acc), dim3(512), 0, st, b);
+    if (last)
+        hipLaunchKernelGGL(conv3x3_bwd<true>, dim3(acc), dim3(512), 0, st, b);
+    else
+        hipLaunchKernelGGL(conv3x3_bwd<false>, dim3(acc), dim3(512), 0, st, b);
     return hipGetLastError();
 }
 

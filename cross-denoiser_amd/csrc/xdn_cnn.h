@@ -41,11 +41,11 @@ struct WgFrame {
 // Fused backward of one 32 -> 32 layer: its weight gradient and its data gradient into the layer
 // below, both from one staging of dY.
 struct BwdFrame {
-    const void *dy;          // bf16 [H][W][32]: gradient of the layer's pre-activation output
+    const void *dy;          // bf16 [H][W][32] (last layer: FP32 [H][W][6]): gradient of the layer's output
     const void *x;           // bf16 [H][W][32]: the layer's input
     const uint32_t *mask;    // [H][W] ReLU bits of x
     void *dx;                // bf16 [H][W][32]: gradient of x, ReLU-masked (dY of the layer below)
-    const void *wfrag;       // data-gradient B fragments [9][2][64] (16 B each)
+    const void *wfrag;       // data-gradient B fragments [9][KC][64] (16 B each; KC 2, last layer 1)
     float *part;             // [nblk][WG_PART] weight-gradient partials
     int H, W, nblk;
 };
@@ -91,7 +91,8 @@ constexpr int WG_MAX_JOBS = 48;  // per launch; longer job lists are split
 hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st);
 hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st);
 int wgrad_blocks(int H, int W);
-hipError_t launch_bwd(int n, const BwdFrame *fr, hipStream_t st);
+// last: the network's last layer (dy = the FP32 6-channel gradient of the output, L5 fragments)
+hipError_t launch_bwd(int n, const BwdFrame *fr, bool last, hipStream_t st);
 hipError_t launch_fwd_stream(int n, const FwdStreamFrame *fr, hipStream_t st);
 int bwd_blocks(int H, int W);  // blocks (= weight-gradient partials) per image of launch_bwd
 

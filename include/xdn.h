@@ -246,7 +246,9 @@ int xdn_unet_backward(const float *ols1, const float *ols2, const float *prev_ou
  * resolutions may differ): cross-regression with the log1p prologue, reprojection of the history
  * (opacities one, main.py:266), erosion + temporal screening (win 7/1), network forward (cnn5 or
  * the reference's U-Net, per slot.network; one call takes slots of one network), fused filter +
- * temporal accumulation + loss + backward, network backward, Keras Adam, fragment repack.
+ * temporal accumulation + loss + backward, network backward, Keras Adam. The cnn5 Adam update
+ * rewrites the bf16 fragment slots of every weight it updates (the U-Net's fragments are rebuilt by
+ * one packing launch), so the step never rebuilds the whole fragment buffer from params.
  * Labels and history handoff follow main.py:283-286 and :338-341: the caller passes this frame's
  * out_log/ols1/ols2 buffers as the next frame's prev_* (ping-pong) together with this frame's
  * linear_z/normal. Frame 0 of a stream uses zero history (main.py:323-330).
@@ -263,7 +265,9 @@ typedef struct xdn_slot {
     /* slot-owned scratch of xdn_slot_scratch_bytes(H, W) bytes, prepared once by xdn_slot_init */
     void *scratch;
     /* network state: params, grad, Adam m and v [param count of the slot's network], packed
-     * fragments [packed bytes of the network] */
+     * fragments [packed bytes of the network]. `packed` must come from xdn_slot_init (or
+     * xdn_cnn5_pack / xdn_unet_pack), and be packed again after any write to `params` from outside
+     * the step: the step only updates the fragments of the weights it changes. */
     float *params, *grad, *adam_m, *adam_v;
     void *packed;
     int adam_step; /* step number of this update, from 1 (ignored when adam_t is set) */
