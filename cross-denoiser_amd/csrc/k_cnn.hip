@@ -1324,8 +1324,11 @@ hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st)
 int wgrad_blocks(int H, int W)
 {
     const int strips = div_up(W, WS_W) * div_up(H, WS_H);
-    // ~16 strips per block (about two blocks per CU at 1080p): the strip loop is software-pipelined
+    // ~16 strips per block (about two blocks per CU at 1080p): the strip loop is software-pipelined;
+    // a smaller image still gets two blocks per CU where it has the strips (1280x720: 512 blocks of
+    // about 7 strips instead of 225 of 16)
     int nb = div_up(strips, 16);
+    if (nb < 512) nb = 512;
     if (nb > strips) nb = strips;
     return nb < 1 ? 1 : nb;
 }
@@ -1362,8 +1365,10 @@ hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st)
 int bwd_blocks(int H, int W)
 {
     const int tiles = div_up(W, CT_W) * div_up(H, CT_H);
-    // ~16 tiles per block: one block per CU at 1920x1080 (4080 tiles)
+    // ~16 tiles per block: one block per CU at 1920x1080 (4080 tiles); a smaller image still gets a
+    // block per CU where it has the tiles (1280x720: 256 blocks of about 7 tiles instead of 113 of 16)
     int nb = div_up(tiles, 16);
+    if (nb < 256) nb = 256;
     if (nb > tiles) nb = tiles;
     return nb < 1 ? 1 : nb;
 }
