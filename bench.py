@@ -244,9 +244,16 @@ def stage_times(den, frames, reps, torch):
                                     ptr(dparam), ptr(den.packed), ptr(den.grad), ptr(cnn_ws), cnn_ws_bytes, 1, H, W,
                                     s), "cnn bwd")
 
+    def t_ft_kernel():
+        # the fused training filter kernel alone (no loss-sum launch: avg_loss NULL)
+        check(lib.xdn_filter_train(ptr(ols1), ptr(ols2), ptr(f["albedo"]), ptr(f["normal"]), ptr(param),
+                                   ptr(succ_e), ptr(sprev), ptr(pout), ptr(f["opacity"]), ptr(pols1), ptr(pols2),
+                                   ptr(out_log), ptr(den.out), ptr(dparam), None, None, ptr(ft_ws),
+                                   ft_bytes, 1, H, W, 11, s), "ft kernel")
+
     out = {}
     for name, fn in (("cross_regression", t_cr), ("reproject", t_rp), ("screening", t_ts), ("cnn_fwd", t_cnn_fwd),
-                     ("filter_train", t_ft), ("cnn_bwd", t_cnn_bwd)):
+                     ("filter_train", t_ft), ("cnn_bwd", t_cnn_bwd), ("sf_train_tile", t_ft_kernel)):
         fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -418,6 +425,7 @@ def main():
     quality = {"relL2_output": round(pipeline.rel_l2(den.outputs()["out"], last["ref"]), 5),
                "relL2_noisy_input": round(pipeline.rel_l2(0.5 * (last["color"] + last["color2"]), last["ref"]), 5)}
     stages = stage_times(den, frames, args.kernel_reps, torch)
+    kernel_ms = {"sf_train_tile": stages.pop("sf_train_tile")}
     print(f"[bench] stages (ms): {stages}", file=sys.stderr, flush=True)
     # dominant custom kernel
     dom = max(("filter_train", "cross_regression"), key=lambda k: stages[k])
@@ -425,17 +433,21 @@ def main():
     if dom == "filter_train":
         flops = ALGO["filter_train"]["flop_per_px"] * npx
         name = "sf_train_tile<5> (fused spatial filter + aggregation + backward)"
+        dur = kernel_ms["sf_train_tile"]
     else:
         flops = (ALGO["regression"]["flop_per_px"] + ALGO["recon"]["flop_per_px"]) * npx
         name = "cross_regression (variance + regression_rows + recon_tile)"
-    achieved = flops / (stages[dom] * 1e-3) / 1e12
+        dur = stages[dom]
+    achieved = flops / (dur * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "traffic": profiled_traffic(name.split(" ")[0], H, W), "kernel": name,
+                "kernel_ms": round(dur, 4),
                 "algorithmic_flop_per_px": ALGO[dom]["flop_per_px"] if dom == "filter_train" else None,
                 "note": "VALU-bound FP32 kernel (no MFMA); peak is the published FP32 rate (the non-packed VALU "
-                        "rate is half of it); traffic = FETCH_SIZE + WRITE_SIZE bytes per launch from the committed "
-                        "counter profile, copy-calibrated"}
+                        "rate is half of it); kernel_ms = HIP events around back-to-back launches of the kernel "
+                        "alone on its stream; traffic = FETCH_SIZE + WRITE_SIZE bytes per launch from the "
+                        "committed counter profile, copy-calibrated"}
     unet = None
     if not args.no_unet:
         unet = unet_leg(frames, H, W, max(8, args.steps // 2), 4, args.kernel_reps, torch)

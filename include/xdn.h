@@ -145,7 +145,8 @@ int xdn_erode_temporal_screening(const float *current1, const float *current2, c
  *   ols1, ols2, albedo, normal, prev_out, prev_ols1, prev_ols2 [B,H,W,3]
  *   success (eroded), success_prev, opacity [B,H,W,1]
  *   out_log, out [B,H,W,3]; dparam [B,H,W,6] = d mean(loss of the image) / d param;
- *   loss [B,H,W,3] (may be NULL); avg_loss [B] (mean loss per image)
+ *   loss [B,H,W,3] (may be NULL); avg_loss [B] (mean loss per image; may be NULL, then no loss
+ *   sum is launched)
  *   workspace >= xdn_filter_train_workspace_size(B,H,W) bytes
  * ------------------------------------------------------------------------------------------- */
 size_t xdn_filter_train_workspace_size(int batch, int height, int width);
