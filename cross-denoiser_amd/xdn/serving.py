@@ -133,7 +133,8 @@ class Admission:
 # ------------------------------------------------------------------------------------------------
 class Server:
     def __init__(self, device="cuda", hbm_budget=64 << 30, max_batch=16, lr=1e-3, pool_frames=4, batching=True,
-                 idle_after=0.05, graphs=False, max_inflight=2, network="cnn5", graph_cache=64, graph_after=2):
+                 idle_after=0.05, graphs=False, max_inflight=2, network="cnn5", graph_cache=64, graph_after=2,
+                 min_batch=1, max_batch_px=None):
         """batching: same-stage kernels of all ready streams in one launch (xdn_denoise_step over
         many slots); otherwise one frame at a time. graphs: with batching off, each frame is the
         stream's captured step graph (one per frame buffers and history parity); with batching on,
@@ -141,9 +142,18 @@ class Server:
         launched `graph_after` - 1 times before is captured as one graph of the batched step and
         replayed from then on (at most `graph_cache` graphs, least recently used dropped; other
         batches run eagerly). max_inflight: batches on the device at once (each on its own HIP
-        stream). network: every stream's bandwidth network ("cnn5" or "unet")."""
+        stream). network: every stream's bandwidth network ("cnn5" or "unet"). min_batch: with
+        batching, fewer ready streams than this are launched one at a time (a batch of one each,
+        up to max_inflight on the device), so a light load keeps the per-frame latency of the
+        one-frame loop and only a backlog is batched. max_batch_px: a batch takes streams oldest
+        pending frame first while their frames' pixels add up to at most this (the first stream
+        always fits; None: no limit), so one batch's duration stays near one frame of that size
+        and a stream with a backlog, which advances one frame per batch, is not held back by
+        long batches."""
         self.adm = Admission(hbm_budget)
         self.max_batch = max_batch if batching else 1
+        self.min_batch = max(1, int(min_batch))
+        self.max_batch_px = max_batch_px
         self.batching = batching
         self.graphs = graphs
         self.lr = lr
@@ -227,9 +237,12 @@ class Server:
             inflight = still
             # admission of streams with pending frames; a stream already picked for this batch
             # (or on the device) is never evicted to make room for another
-            ready = []
+            ready, ready_px = [], 0
             for sid in sorted(pending, key=lambda s: pending[s][0][0] if pending[s] else 1e30):
                 if not pending[sid] or sid in busy:
+                    continue
+                px = by_sid[sid].height * by_sid[sid].width
+                if ready and self.max_batch_px is not None and ready_px + px > self.max_batch_px:
                     continue
                 if sid not in self.denoisers:
                     r = by_sid[sid]
@@ -245,8 +258,11 @@ class Server:
                     self.epoch[sid] = self._serial
                     self.admit_log.append((sid, pending[sid][0][1]))
                 ready.append(sid)
+                ready_px += px
                 if len(ready) >= self.max_batch:
                     break
+            if self.batching and len(ready) < self.min_batch:
+                ready = ready[:1]  # light load: one stream per launch, the others take the next slot
             if ready and len(inflight) < self.max_inflight:
                 slot = bi % self.max_inflight
                 bi += 1
@@ -389,6 +405,10 @@ def make_server(mode: str, device, **kw) -> Server:
     repeating batches replayed as captured graphs."""
     if mode not in MODES:
         raise ValueError(f"unknown serving mode {mode!r} (one of {MODES})")
+    if mode.startswith("batched"):
+        # batches of at most one 1920x1080 frame's pixels (e.g. two 1280x720 frames): with larger
+        # batches a stream with a backlog advances one frame per long batch and the p99 grows
+        kw.setdefault("max_batch_px", 1920 * 1080)
     return Server(device, batching=mode.startswith("batched"), graphs=mode.endswith("graphs"),
                   max_inflight=1 if mode == "plain" else 2, **kw)
 
@@ -407,7 +427,16 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--fps", type=float, default=60.0, help="mean arrival rate per stream (configuration 3)")
     ap.add_argument("--network", choices=("cnn5", "unet"), default="cnn5")
+    ap.add_argument("--min-batch", type=int, default=None, help="batched modes: ready streams below which "
+                    "each is launched alone (default: the mode's default)")
+    ap.add_argument("--max-batch-px", type=float, default=None, help="batched modes: pixel limit of one batch, "
+                    "in 1920x1080 frames (default: the mode's default; 0: no limit)")
     a = ap.parse_args(argv)
+    kw = {}
+    if a.min_batch is not None:
+        kw["min_batch"] = a.min_batch
+    if a.max_batch_px is not None:
+        kw["max_batch_px"] = int(a.max_batch_px * 1920 * 1080) if a.max_batch_px > 0 else None
     dev = torch.device("cuda:0")
     reqs = config3(a.frames, fps=a.fps, seed=a.seed) if a.config == 3 else config4(a.frames, seed=a.seed)
     warm_up(dev, [(r.height, r.width) for r in reqs], network=a.network)
@@ -415,9 +444,10 @@ def main(argv=None):
     for m in modes:
         print(f"[serving] config {a.config}, mode {m}, {len(reqs)} streams x {a.frames} frames", file=sys.stderr,
               flush=True)
-        srv = make_server(m, dev, hbm_budget=int(a.budget_gb * (1 << 30)), network=a.network)
+        srv = make_server(m, dev, hbm_budget=int(a.budget_gb * (1 << 30)), network=a.network, **kw)
         res = srv.run(reqs)
         res.update({"config": a.config, "mode": m, "network": a.network, "streams": len(reqs), "frames_per_stream": a.frames,
+                    "min_batch": srv.min_batch, "max_batch_px": srv.max_batch_px,
                     "budget_gb": a.budget_gb, "arrival_fps_per_stream": a.fps if a.config == 3 else 120.0,
                     "stream_sizes": sorted({f"{r.width}x{r.height}" for r in reqs})})
         print(json.dumps(res), flush=True)

@@ -94,6 +94,36 @@ def test_serving_never_evicts_a_stream_picked_for_the_batch():
     assert res["evictions"] >= 1 and [s for s, _ in srv.admit_log][:3] == [0, 1, 2]
 
 
+def test_min_batch_launches_a_light_load_one_stream_at_a_time():
+    """Below min_batch ready streams each is launched on its own; at min_batch or more they share
+    one launch (up to max_batch)."""
+    light = [StreamReq(i, 64, 64, 2, [0.0, 0.0], seed=i) for i in range(2)]
+    srv = HostServer(None, hbm_budget=1000, pool_frames=2, min_batch=3)
+    assert srv.run(light)["frames"] == 4
+    assert all(len(b) == 1 for b in srv.launched) and len(srv.launched) == 4
+    heavy = [StreamReq(i, 64, 64, 2, [0.0, 0.0], seed=i) for i in range(5)]
+    srv = HostServer(None, hbm_budget=1000, pool_frames=2, min_batch=3, max_batch=4)
+    assert srv.run(heavy)["frames"] == 10
+    assert srv.launched[0] == [0, 1, 2, 3] and max(len(b) for b in srv.launched) == 4
+    # without batching min_batch changes nothing: one stream per launch
+    srv = HostServer(None, hbm_budget=1000, pool_frames=2, min_batch=3, batching=False)
+    assert srv.run(heavy)["frames"] == 10 and all(len(b) == 1 for b in srv.launched)
+
+
+def test_max_batch_px_fills_a_batch_oldest_first_up_to_the_pixel_limit():
+    """A batch takes streams oldest pending frame first while their pixels fit the limit; a stream
+    that does not fit waits for the next batch, a frame larger than the limit runs alone."""
+    # equal arrival times: the streams are taken in their order in the request list
+    reqs = [StreamReq(0, 128, 128, 1, [0.0], seed=0)] + [StreamReq(i, 64, 64, 1, [0.0], seed=i) for i in (1, 2, 3)]
+    srv = HostServer(None, hbm_budget=1000, pool_frames=2, max_batch_px=2 * 64 * 64)
+    res = srv.run(reqs)
+    assert res["frames"] == 4
+    assert srv.launched == [[0], [1, 2], [3]]
+    srv = HostServer(None, hbm_budget=1000, pool_frames=2)  # no limit: up to max_batch streams
+    srv.run(reqs)
+    assert max(len(b) for b in srv.launched) >= 2
+
+
 def _replay_alone(dev, req, pool, admits, network, lr=1e-3):
     """Steps one stream frame by frame through Denoiser.step on the server's own frame buffers,
     starting a new slot at each frame index in `admits` with the network and Adam state carried
@@ -164,9 +194,9 @@ def test_batched_graphs_are_replayed(dev):
     from xdn.serving import make_server
     reqs = [StreamReq(i, h, w, 6, [0.002 * k for k in range(6)], seed=i)
             for i, (h, w) in enumerate([(64, 64), (48, 96)])]
-    srv = make_server("batched_graphs", dev, hbm_budget=1 << 40, pool_frames=2)
+    srv = make_server("batched_graphs", dev, hbm_budget=1 << 40, pool_frames=2, min_batch=2)
     res = srv.run(reqs)
-    assert res["frames"] == 12 and res["graphs"]["replayed"] > 0
+    assert res["frames"] == 12 and res["graphs"]["replayed"] > 0 and res["mean_batch"] > 1
     check_against_alone(dev, srv, reqs)
     srv.close()
 
