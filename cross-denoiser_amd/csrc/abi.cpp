@@ -75,6 +75,7 @@ static int cross_regression_impl(const float *color, const float *color2, const 
     if (H % rs != 0 || W % rs != 0) return fail(XDN_EINVAL, "Image height and width must be divisible by recon_scale");
     if (H % s != 0 || W % s != 0) return fail(XDN_EINVAL, "Image height and width must be divisible by sparsity");
     if (H <= win / 2 || W <= win / 2) return fail(XDN_EINVAL, "Image height and width must exceed win_size/2");
+    if (B == 0) return XDN_OK;  // empty batch: nothing to read or write
     if ((!color && (!imga || !imgb)) || !albedo || !normal || !outa || !outb || !vara || !varb)
         return fail(XDN_EINVAL, "cross_regression: null tensor");
     if (!ws || ws_bytes < xdn_cross_regression_workspace_size(B, H, W, s))
@@ -124,7 +125,8 @@ int xdn_cross_regression_from_color(const float *color, const float *color2, con
                                     void *workspace, size_t workspace_bytes, int batch, int height, int width,
                                     int win_size, int sparsity, void *stream)
 {
-    if (!color || !color2 || !y1 || !y2) return fail(XDN_EINVAL, "color, color2, y1 and y2 are required");
+    if (batch != 0 && (!color || !color2 || !y1 || !y2))
+        return fail(XDN_EINVAL, "color, color2, y1 and y2 are required");
     return cross_regression_impl(color, color2, nullptr, nullptr, albedo, normal, y1, y2, outa, outb, out_vara,
                                  out_varb, workspace, workspace_bytes, batch, height, width, win_size, sparsity, 1,
                                  stream);
@@ -166,6 +168,7 @@ int xdn_cross_regression_grad(const float *imga, const float *imgb, const float 
     if (win < 1 || s < 1) return fail(XDN_EINVAL, "win_size and sparsity must be >= 1");
     if (H % s != 0 || W % s != 0) return fail(XDN_EINVAL, "Image height and width must be divisible by sparsity");
     if (H <= win / 2 || W <= win / 2) return fail(XDN_EINVAL, "Image height and width must exceed win_size/2");
+    if (B == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!imga || !imgb || !albedo || !normal || !grad_outa || !grad_outb || !grad_imga || !grad_imgb ||
         !grad_albedo || !grad_normal)
         return fail(XDN_EINVAL, "cross_regression_grad: null tensor");
@@ -235,6 +238,7 @@ int xdn_spatial_filter(const float *imga, const float *imgb, const float *albedo
                        int batch, int height, int width, int win_size, void *stream)
 {
     if (int e = sf_common_check(batch, height, width, win_size)) return e;
+    if (batch == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!imga || !imgb || !albedo || !normal || !band || !out_acca || !out_accb || !out_wgtsuma || !out_wgtsumb)
         return fail(XDN_EINVAL, "spatial_filter: null tensor");
     const size_t N = px(height, width);
@@ -256,6 +260,7 @@ int xdn_spatial_filter_grad(const float *grad_acca, const float *grad_accb, cons
                             int width, int win_size, void *stream)
 {
     if (int e = sf_common_check(batch, height, width, win_size)) return e;
+    if (batch == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!grad_acca || !grad_accb || !grad_wgtsuma || !grad_wgtsumb || !imga || !imgb || !img_albedo || !img_normal ||
         !band || !out_grad_band)
         return fail(XDN_EINVAL, "spatial_filter_grad: null tensor");
@@ -281,6 +286,7 @@ int xdn_spatial_filter_grad_inputs(const float *grad_acca, const float *grad_acc
 {
     if (int e = sf_common_check(batch, height, width, win_size)) return e;
     if (win_size / 2 > 7) return fail(XDN_EINVAL, "input gradients support win_size <= 15");
+    if (batch == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!grad_acca || !grad_accb || !grad_wgtsuma || !grad_wgtsumb || !imga || !imgb || !img_albedo || !img_normal ||
         !band)
         return fail(XDN_EINVAL, "spatial_filter_grad_inputs: null tensor");
@@ -310,6 +316,7 @@ int xdn_reproject(const float *mvec, int n_list, const float *const *inputs, con
     if (n_list > 0 && !dims) return fail(XDN_EINVAL, "reproject: null dims");
     for (int l = 0; l < n_list; ++l)
         if (dims[l] < 1 || dims[l] > 4) return fail(XDN_EINVAL, "input_list[%d] has %d channels (1..4 supported)", l, dims[l]);
+    if (batch == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!mvec || !linear_z || !prev_linear_z || !normal || !prev_normal || !pn_fwidth || !opacity || !prev_opacity ||
         !success || (n_list > 0 && (!inputs || !outputs)))
         return fail(XDN_EINVAL, "reproject: null tensor");
@@ -354,6 +361,7 @@ static int screening_impl(const float *c1, const float *c2, const float *prev, c
 {
     if (int e = check_shape(B, H, W)) return e;
     if (win_curr < 1 || win_prev < 1) return fail(XDN_EINVAL, "win_curr and win_prev must be >= 1");
+    if (B == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!c1 || !c2 || !prev || !prev1 || !prev2 || !o || !o1 || !o2 || !osucc)
         return fail(XDN_EINVAL, "temporal_screening: null tensor");
     const size_t N = px(H, W);
@@ -386,7 +394,7 @@ int xdn_temporal_screening(const float *current1, const float *current2, const f
                            float *output2, float *success, int batch, int height, int width, int win_curr,
                            int win_prev, void *stream)
 {
-    if (!reproj_success) return fail(XDN_EINVAL, "reproj_success is required");
+    if (batch != 0 && !reproj_success) return fail(XDN_EINVAL, "reproj_success is required");
     return screening_impl(current1, current2, prev, prev1, prev2, reproj_success, nullptr, nullptr, output, output1,
                           output2, success, batch, height, width, win_curr, win_prev, stream);
 }
@@ -396,7 +404,7 @@ int xdn_erode_temporal_screening(const float *current1, const float *current2, c
                                  float *output1, float *output2, float *success, int batch, int height, int width,
                                  int win_curr, int win_prev, void *stream)
 {
-    if (!success_mask) return fail(XDN_EINVAL, "success_mask is required");
+    if (batch != 0 && !success_mask) return fail(XDN_EINVAL, "success_mask is required");
     return screening_impl(current1, current2, prev, prev1, prev2, nullptr, success_mask, eroded, output, output1,
                           output2, success, batch, height, width, win_curr, win_prev, stream);
 }
@@ -415,6 +423,7 @@ int xdn_filter_train(const float *ols1, const float *ols2, const float *albedo, 
 {
     if (int e = sf_common_check(batch, height, width, win_size)) return e;
     if (win_size / 2 != 5) return fail(XDN_EINVAL, "the fused training kernel supports win_size 11 (half 5) only");
+    if (batch == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!ols1 || !ols2 || !albedo || !normal || !param || !success || !success_prev || !prev_out || !opacity ||
         !prev_ols1 || !prev_ols2 || !out_log || !out || !dparam)
         return fail(XDN_EINVAL, "filter_train: null tensor");
@@ -468,6 +477,7 @@ int xdn_frame_error(const float *out, const float *emissive, const float *env_li
                     void *workspace, size_t workspace_bytes, int batch, int height, int width, void *stream)
 {
     if (int e = check_shape(batch, height, width)) return e;
+    if (batch == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!out || !ref || !err) return fail(XDN_EINVAL, "frame error: out, ref and err are required");
     if (!workspace || workspace_bytes < xdn_frame_error_workspace_size(batch, height, width))
         return fail(XDN_EINVAL, "frame error: workspace too small");
@@ -564,6 +574,7 @@ int xdn_cnn5_forward(const float *ols1, const float *ols2, const float *prev_out
                      int batch, int height, int width, void *stream)
 {
     if (int e = cnn5_check(batch, height, width, workspace_bytes)) return e;
+    if (batch == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!ols1 || !ols2 || !prev_out || !albedo || !normal || !packed || !param || !workspace)
         return fail(XDN_EINVAL, "cnn5 forward: null buffer");
     Cnn5Ws ws(workspace, batch, height, width);
@@ -596,6 +607,11 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
                       size_t workspace_bytes, int batch, int height, int width, void *stream)
 {
     if (int e = cnn5_check(batch, height, width, workspace_bytes)) return e;
+    if (batch == 0)  // empty batch: the gradient is a sum over no images
+    {
+        if (!grad) return fail(XDN_EINVAL, "cnn5 backward: null buffer");
+        return hip_check(xdn::launch_zero(grad, kLay.count, S(stream)), "cnn5 backward");
+    }
     if (!ols1 || !ols2 || !prev_out || !albedo || !normal || !dparam || !packed || !grad || !workspace)
         return fail(XDN_EINVAL, "cnn5 backward: null buffer");
     Cnn5Ws ws(workspace, batch, height, width);

@@ -58,7 +58,8 @@ extern "C" int xdn_filter_train_noalloc(const float *ols1, const float *ols2, co
                     "fixed %zu (use xdn_filter_train with a workspace)",
                     batch, height, width, need, kScratchBytes);
     void *ws = nullptr;
-    if (int e = scratch_ptr(&ws)) return e;
+    if (batch > 0)  // an empty batch only has its attributes checked: no device call
+        if (int e = scratch_ptr(&ws)) return e;
     // every tile writes its own partial before the loss sum reads it: nothing to clear
     return xdn_filter_train(ols1, ols2, albedo, normal, param, success, success_prev, prev_out, opacity, prev_ols1,
                             prev_ols2, out_log, out, dparam, loss, avg_loss, ws, kScratchBytes, batch, height, width,

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "abi_internal.h"
+#include "xdn_misc.h"
 #include "xdn_unet.h"
 
 using namespace xdn_abi;
@@ -320,7 +321,7 @@ int xdn_unet_pack(const float *params, void *packed, void *stream)
 
 static int unet_check(int B, int H, int W, size_t ws_bytes)
 {
-    if (B <= 0 || H <= 0 || W <= 0) return fail(XDN_EINVAL, "unet: invalid shape [%d, %d, %d]", B, H, W);
+    if (B < 0 || H <= 0 || W <= 0) return fail(XDN_EINVAL, "unet: invalid shape [%d, %d, %d]", B, H, W);
     if (B > XDN_MAX_FRAMES) return fail(XDN_EINVAL, "unet: batch must be <= %d", XDN_MAX_FRAMES);
     // two 2x2 poolings and two 2x upsamplings must give back the full size for the concatenations
     if (H % 4 || W % 4) return fail(XDN_EINVAL, "unet: height and width must be multiples of 4 (got %d x %d)", H, W);
@@ -348,6 +349,7 @@ int xdn_unet_forward(const float *ols1, const float *ols2, const float *prev_out
                      int batch, int height, int width, void *stream)
 {
     if (int e = unet_check(batch, height, width, workspace_bytes)) return e;
+    if (batch == 0) return XDN_OK;  // empty batch: nothing to read or write
     if (!ols1 || !ols2 || !prev_out || !albedo || !normal || !packed || !param || !workspace)
         return fail(XDN_EINVAL, "unet forward: null buffer");
     const UnetWs ws(workspace, batch, height, width);
@@ -361,6 +363,11 @@ int xdn_unet_backward(const float *ols1, const float *ols2, const float *prev_ou
                       size_t workspace_bytes, int batch, int height, int width, void *stream)
 {
     if (int e = unet_check(batch, height, width, workspace_bytes)) return e;
+    if (batch == 0)  // empty batch: the gradient is a sum over no images
+    {
+        if (!grad) return fail(XDN_EINVAL, "unet backward: null buffer");
+        return hip_check(xdn::launch_zero(grad, kULay.count, S(stream)), "unet backward");
+    }
     if (!ols1 || !ols2 || !prev_out || !albedo || !normal || !dparam || !packed || !grad || !workspace)
         return fail(XDN_EINVAL, "unet backward: null buffer");
     const UnetWs ws(workspace, batch, height, width);

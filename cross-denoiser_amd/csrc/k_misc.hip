@@ -240,4 +240,17 @@ hipError_t launch_frame_error(int n, const ErrFrame *fr, hipStream_t st)
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void zero_kernel(float *p, size_t count)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (size_t)gridDim.x * 256) p[i] = 0.f;
+}
+
+hipError_t launch_zero(float *p, size_t count, hipStream_t st)
+{
+    if (count == 0) return hipSuccess;
+    const size_t blocks = (count + 255) / 256;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)(blocks < 1024 ? blocks : 1024)), dim3(256), 0, st, p, count);
+    return hipGetLastError();
+}
+
 }  // namespace xdn

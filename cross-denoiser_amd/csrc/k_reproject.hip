@@ -59,7 +59,9 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
 
     // Three round trips to memory: the pixel's own inputs; every bilinear tap's validity inputs,
     // loaded at coordinates clamped into the image and used only where the tap is in the image;
-    // the list values of the valid taps.
+    // the list values of all four taps (compile-time lists; only the valid taps are summed). The
+    // 3x3 fallback of a pixel with no valid tap visits its neighbours one at a time: loading them
+    // all at once needs 120 registers per lane and was slower.
     const float op = UNIT_OPAC ? 1.f : F.opac[c];
     const float mx = F.mvec[3 * c], my = F.mvec[3 * c + 1];
     const float fwN = F.pnFw[3 * c + 1];
@@ -93,7 +95,37 @@ __global__ __launch_bounds__(256) void reproject_kernel(RpBatch b)
             v[t] = inb[t] && po[t] != 0.f && reproj_valid(W, H, ipx, ipy, Z, pz[t], fwZ, n, pn[t], fwN);
             valid = valid || v[t];
         }
-        if (valid)
+        if (valid && FNL > 0)
+        {
+            // every tap's list values in flight at once (clamped in-image indices, so the loads are
+            // in bounds), then summed in tap order where the tap is valid: the same sums as below
+            const float x = ppx - floorf(ppx), y = ppy - floorf(ppy);
+            const float w[4] = {(1.f - x) * (1.f - y), x * (1.f - y), (1.f - x) * y, x * y};
+            float val[4][FNL > 0 ? FNL : 1][FNL > 0 ? FD : 1];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int l = 0; l < FNL; ++l)
+#pragma unroll
+                    for (int k = 0; k < FD; ++k) val[t][l][k] = F.in[l][li[t] * FD + k];
+            float sumw = 0.f;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+#pragma unroll
+                for (int l = 0; l < FNL; ++l)
+#pragma unroll
+                    for (int k = 0; k < FD; ++k) acc[l][k] = v[t] ? __builtin_fmaf(w[t], val[t][l][k], acc[l][k]) : acc[l][k];
+                sumw = v[t] ? sumw + w[t] : sumw;
+            }
+            valid = (sumw >= 0.01f);
+            const float inv = 1.f / sumw;
+#pragma unroll
+            for (int l = 0; l < XDN_MAX_LIST; ++l)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc[l][k] = valid ? acc[l][k] * inv : 0.f;
+        }
+        else if (valid)
         {
             const float x = ppx - floorf(ppx), y = ppy - floorf(ppy);
             const float w[4] = {(1.f - x) * (1.f - y), x * (1.f - y), (1.f - x) * y, x * y};

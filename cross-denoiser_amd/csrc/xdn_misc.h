@@ -50,5 +50,7 @@ struct ErrFrame {
 };
 int frame_error_blocks(int H, int W);
 hipError_t launch_frame_error(int n, const ErrFrame *fr, hipStream_t st);
+// p[0 .. count) = 0 with a kernel (no memset node when the caller captures a graph)
+hipError_t launch_zero(float *p, size_t count, hipStream_t st);
 
 }  // namespace xdn

@@ -13,6 +13,10 @@
  *     HIP launch error. xdn_last_error() gives the message of the last failure on the calling
  *     thread; argument messages repeat the reference's text where it has one.
  *   - Pointers may not alias unless stated.
+ *   - An empty batch (batch 0, with valid H, W and attributes) returns 0 without reading or
+ *     writing any buffer, so its pointers may be NULL (the data pointers of zero-element tensors);
+ *     xdn_cnn5_backward and xdn_unet_backward, whose gradient has no batch dimension, write zeros
+ *     to grad.
  */
 #ifndef XDN_H
 #define XDN_H

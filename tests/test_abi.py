@@ -50,6 +50,41 @@ def test_argument_errors_without_gpu():
     assert rc == -1 and "channels" in lib.xdn_last_error().decode()
 
 
+def test_empty_batch_is_a_no_op_without_gpu():
+    """Batch 0 returns 0 with every buffer NULL (zero-element tensors have null data pointers) and
+    touches no device; attributes are still checked."""
+    from xdn._lib import lib
+    H, W = 32, 48
+    N = None
+    calls = {
+        "xdn_cross_regression": [N] * 8 + [N, 0, 0, H, W, 17, 4, 1, N],
+        "xdn_cross_regression_from_color": [N] * 10 + [N, 0, 0, H, W, 17, 4, N],
+        "xdn_cross_regression_grad": [N] * 12 + [N, 0, 0, H, W, 17, 4, N],
+        "xdn_spatial_filter": [N] * 9 + [0, H, W, 11, N],
+        "xdn_spatial_filter_grad": [N] * 10 + [0, H, W, 11, N],
+        "xdn_spatial_filter_grad_inputs": [N] * 13 + [0, H, W, 11, N],
+        "xdn_reproject": [N, 3, N, (ctypes.c_int * 3)(3, 3, 1)] + [N] * 8 + [N, 0, H, W, N],
+        "xdn_temporal_screening": [N] * 10 + [0, H, W, 7, 1, N],
+        "xdn_erode_temporal_screening": [N] * 11 + [0, H, W, 7, 1, N],
+        "xdn_filter_train": [N] * 16 + [N, 0, 0, H, W, 11, N],
+        "xdn_filter_train_noalloc": [N] * 16 + [0, H, W, 11, N],
+        "xdn_frame_error": [N] * 8 + [N, 0, 0, H, W, N],
+        "xdn_cnn5_forward": [N] * 7 + [N, 0, 0, H, W, N],
+        "xdn_unet_forward": [N] * 7 + [N, 0, 0, H, W, N],
+    }
+    for name, args in calls.items():
+        rc = getattr(lib, name)(*args)
+        assert rc == 0, (name, lib.xdn_last_error().decode())
+    # attributes are checked for an empty batch too
+    assert lib.xdn_cross_regression(*([N] * 8 + [N, 0, 0, H, W, 17, 2, 4, N])) == -1
+    assert lib.xdn_filter_train(*([N] * 16 + [N, 0, 0, H, W, 7, N])) == -1
+    assert lib.xdn_unet_forward(*([N] * 7 + [N, 0, 0, 30, W, N])) == -1
+    assert lib.xdn_cross_regression(*([N] * 8 + [N, 0, -1, H, W, 17, 4, 1, N])) == -1
+    # the weight gradients have no batch dimension: an empty batch still needs grad (zeroed)
+    assert lib.xdn_cnn5_backward(*([N] * 8 + [N, 0, 0, H, W, N])) == -1
+    assert lib.xdn_unet_backward(*([N] * 8 + [N, 0, 0, H, W, N])) == -1
+
+
 def test_make_slot_parity_selects_history_buffers():
     """A slot for step t takes its outputs from history buffer t % 2 and its history from the other
     (host logic only: CPU tensors stand in for device buffers)."""

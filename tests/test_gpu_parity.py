@@ -320,6 +320,23 @@ def test_reproject_many_buffers(ops, dev):
         assert rel_l2(to_np(g), rr) <= 1e-6
 
 
+@pytest.mark.parametrize("H,W,pan,dis", [(64, 80, (1.3, -0.6), 0.1), (96, 130, (-2.7, 2.2), 0.3),
+                                         (40, 40, (-0.4, 0.45), 0.6)])
+def test_reproject_three_rgb_buffers(ops, dev, H, W, pan, dis):
+    """Three 3-channel buffers (the training step's list, with its own kernel): mask bit-exact, the
+    bilinear and 3x3-fallback averages against the oracle, with many disoccluded pixels."""
+    mvec, ins, *rest = reproject_case(H, W, seed=H + W, pan=pan, dis=dis)
+    r = np.random.default_rng(H)
+    ins = [r.uniform(0, 2, (1, H, W, 3)).astype(np.float32) for _ in range(3)]
+    s_ref, o_ref = O.reproject(mvec, ins, *rest)
+    s, outs = ops.Reproject(to_dev(mvec, dev), [to_dev(a, dev) for a in ins], *[to_dev(a, dev) for a in rest])
+    s = to_np(s)
+    assert np.array_equal(s, s_ref) and 0 < s_ref.sum() < s_ref.size
+    for g, rr in zip(outs, o_ref):
+        assert rel_l2(to_np(g), rr) <= 1e-6
+        assert np.all(to_np(g)[~s[..., 0]] == 0)
+
+
 # ------------------------------------------------------------------------------------------------
 def screening_case(H, W, seed, B=1):
     ia, ib = log_images(H, W, seed=seed, B=B)
@@ -453,3 +470,28 @@ def test_config0_forward_chain(ops, dev):
         err = rel_l2(n(a), r)
         assert err <= TOL
     _report("configuration 0 chain, last spatial filter acca", rel_l2(n(acc1[0]), acc1_ref[0]))
+
+
+def test_empty_batch(ops, dev):
+    """Zero-image batches through the drop-in ops give zero-image outputs; the network backwards,
+    whose weight gradient has no batch dimension, write zeros."""
+    import ctypes
+    from xdn._lib import lib
+    H, W = 32, 48
+    e3 = torch.empty((0, H, W, 3), device=dev)
+    e1 = torch.empty((0, H, W, 1), device=dev)
+    outa, outb, va, vb = ops.cross_regression(e3, e3, e3, e3, win_size=17, sparsity=4)
+    assert outa.shape == (0, H, W, 3) and va.shape == (0, H, W, 1)
+    acc = ops.spatial_filter(e3, e3, e3, e3, torch.empty((0, H, W, 5), device=dev), win_size=11)
+    assert all(t.shape[0] == 0 for t in acc)
+    o = ops.temporal_screening(e3, e3, e3, e3, e3, e1, 7, 1)
+    assert all(t.shape[0] == 0 for t in o)
+    s, outs = ops.Reproject(e3, [e3, e3, e1], e3, e3, e3, e3, e3, e1, e1)
+    assert s.shape == (0, H, W, 1) and [t.shape[3] for t in outs] == [3, 3, 1]
+    for count, fn in ((lib.xdn_cnn5_param_count(), lib.xdn_cnn5_backward),
+                      (lib.xdn_unet_param_count(), lib.xdn_unet_backward)):
+        grad = torch.full((int(count),), float("nan"), device=dev)
+        rc = fn(*([None] * 7), ctypes.c_void_p(grad.data_ptr()), None, 0, 0, H, W, None)
+        assert rc == 0, lib.xdn_last_error().decode()
+        torch.cuda.synchronize()
+        assert torch.count_nonzero(grad).item() == 0

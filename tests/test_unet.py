@@ -192,7 +192,7 @@ def test_unet_rejects_bad_shapes_cpu():
     """Argument checks run before anything touches the device."""
     from xdn._lib import lib
     ws = int(lib.xdn_unet_workspace_size(1, 32, 32))
-    for B, H, W in ((1, 30, 32), (1, 32, 34), (0, 32, 32), (17, 32, 32)):
+    for B, H, W in ((1, 30, 32), (1, 32, 34), (-1, 32, 32), (17, 32, 32), (0, 30, 32)):
         rc = lib.xdn_unet_forward(None, None, None, None, None, None, None, None, ws, B, H, W, None)
         assert rc == -1, (B, H, W)
         msg = lib.xdn_last_error().decode()
