@@ -591,14 +591,13 @@ int xdn_cnn5_forward(const float *ols1, const float *ols2, const float *prev_out
         f.in[0] = ols1 + o3; f.in[1] = ols2 + o3; f.in[2] = prev_out + o3; f.in[3] = albedo + o3;
         f.in[4] = normal + o3;
         f.out = ws.X[0] + o32;
-        f.mask_out = ws.M[0] + (size_t)b * N;
         f.wfrag = pk + kLay.fwd_off[0];
         f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[0]);
         f.H = height;
         f.W = width;
         sf[b] = cnn5_stream_frame(ws, pk, (size_t)b * N, param + (size_t)b * N * 6, height, width);
     }
-    if (int e = hip_check(xdn::launch_conv(xdn::CONV_L1_FWD, batch, fr.data(), S(stream)), "cnn5 forward")) return e;
+    if (int e = hip_check(xdn::launch_conv_l1(batch, fr.data(), S(stream)), "cnn5 forward")) return e;
     return hip_check(xdn::launch_fwd_stream(batch, sf.data(), S(stream)), "cnn5 forward");
 }
 
@@ -617,72 +616,39 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
     Cnn5Ws ws(workspace, batch, height, width);
     const size_t N = px(height, width);
     const char *pk = static_cast<const char *>(packed);
-    std::vector<xdn::ConvFrame> cf(batch);
-    std::vector<xdn::WgFrame> wf(batch);
     std::vector<xdn::RedJob> jobs;
     std::vector<xdn::BwdFrame> bf(batch);
     int cur = 0;  // dY ping-pong index holding the gradient of the current layer's output
-    for (int l = 4; l >= 0; --l)
+    // layers 5..2: weight gradient and data gradient into layer l-1 in one kernel (the last layer's
+    // dY is the FP32 gradient of the network output); the B images' partials of a layer are reduced
+    // together into the one gradient
+    for (int l = 4; l >= 1; --l)
     {
         const int nb = ws.nblk[l];
-        if (l >= 1)
-        {
-            // layers 2..5: weight gradient and data gradient into layer l-1 in one kernel (the last
-            // layer's dY is the FP32 gradient of the network output)
-            for (int b = 0; b < batch; ++b)
-            {
-                const size_t o32 = (size_t)b * N * 32;
-                const void *dy = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6)
-                                        : static_cast<const void *>(ws.dY[cur] + o32);
-                bf[b] = xdn::BwdFrame{dy, ws.X[l - 1] + o32, ws.M[l - 1] + (size_t)b * N,
-                                      ws.dY[1 - cur] + o32, pk + kLay.dg_off[l],
-                                      ws.part[l] + (size_t)b * nb * xdn::WG_PART, height, width, nb};
-            }
-            if (int e = hip_check(xdn::launch_bwd(batch, bf.data(), l == 4, S(stream)), "cnn5 backward")) return e;
-            jobs.push_back(xdn::RedJob{ws.part[l], ws.stage[l], grad + kLay.w_off[l], batch * nb, kCnn5[l].ci, kCnn5[l].co});
-            cur = 1 - cur;
-            continue;
-        }
-        // weight gradient of layer l: x = input of layer l, dy = gradient of its pre-activation
         for (int b = 0; b < batch; ++b)
         {
-            xdn::WgFrame &f = wf[b];
-            memset(&f, 0, sizeof(f));
-            const size_t o3 = (size_t)b * N * 3, o32 = (size_t)b * N * 32;
-            if (l == 0)
-            {
-                f.x[0] = ols1 + o3; f.x[1] = ols2 + o3; f.x[2] = prev_out + o3; f.x[3] = albedo + o3; f.x[4] = normal + o3;
-            }
-            else
-                f.x[0] = ws.X[l - 1] + o32;
-            f.dy = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6) : static_cast<const void *>(ws.dY[cur] + o32);
-            f.part = ws.part[l] + (size_t)b * nb * xdn::WG_PART;
-            f.nblk = nb;
-            f.H = height;
-            f.W = width;
-        }
-        const int wk = l == 0 ? xdn::CONV_L1_WGRAD : (l == 4 ? xdn::CONV_L5_WGRAD : xdn::CONV_MID_WGRAD);
-        if (int e = hip_check(xdn::launch_wgrad(wk, batch, wf.data(), S(stream)), "cnn5 wgrad")) return e;
-        // the B images' partials of this layer are reduced together into the one gradient
-        jobs.push_back(xdn::RedJob{ws.part[l], ws.stage[l], grad + kLay.w_off[l], batch * nb, kCnn5[l].ci, kCnn5[l].co});
-        if (l == 0) break;
-        // data gradient into the output of layer l-1, masked by its ReLU
-        for (int b = 0; b < batch; ++b)
-        {
-            xdn::ConvFrame &f = cf[b];
-            memset(&f, 0, sizeof(f));
             const size_t o32 = (size_t)b * N * 32;
-            f.in[0] = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6) : static_cast<const void *>(ws.dY[cur] + o32);
-            f.out = ws.dY[1 - cur] + o32;
-            f.mask = ws.M[l - 1] + (size_t)b * N;
-            f.wfrag = pk + kLay.dg_off[l];
-            f.H = height;
-            f.W = width;
+            const void *dy = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6)
+                                    : static_cast<const void *>(ws.dY[cur] + o32);
+            bf[b] = xdn::BwdFrame{dy, ws.X[l - 1] + o32, ws.dY[1 - cur] + o32, pk + kLay.dg_off[l],
+                                  ws.part[l] + (size_t)b * nb * xdn::WG_PART, height, width, nb};
         }
-        const int dk = l == 4 ? xdn::CONV_L5_DGRAD : xdn::CONV_MID_DGRAD;
-        if (int e = hip_check(xdn::launch_conv(dk, batch, cf.data(), S(stream)), "cnn5 dgrad")) return e;
+        if (int e = hip_check(xdn::launch_bwd(batch, bf.data(), l == 4, S(stream)), "cnn5 backward")) return e;
+        jobs.push_back(xdn::RedJob{ws.part[l], ws.stage[l], grad + kLay.w_off[l], batch * nb, kCnn5[l].ci, kCnn5[l].co});
         cur = 1 - cur;
     }
+    // layer 1's weight gradient: x = the five input images, dy = the gradient of its pre-activation
+    std::vector<xdn::WgFrame> wf(batch);
+    const int nb0 = ws.nblk[0];
+    for (int b = 0; b < batch; ++b)
+    {
+        const size_t o3 = (size_t)b * N * 3;
+        wf[b] = xdn::WgFrame{{ols1 + o3, ols2 + o3, prev_out + o3, albedo + o3, normal + o3},
+                             ws.dY[cur] + (size_t)b * N * 32, ws.part[0] + (size_t)b * nb0 * xdn::WG_PART, height, width,
+                             nb0};
+    }
+    if (int e = hip_check(xdn::launch_wgrad_l1(batch, wf.data(), S(stream)), "cnn5 wgrad")) return e;
+    jobs.push_back(xdn::RedJob{ws.part[0], ws.stage[0], grad + kLay.w_off[0], batch * nb0, kCnn5[0].ci, kCnn5[0].co});
     return hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), S(stream)), "cnn5 wgrad reduce");
 }
 

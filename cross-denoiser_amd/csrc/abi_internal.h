@@ -66,12 +66,12 @@ struct Cnn5Layout {
 };
 inline const Cnn5Layout kLay;
 
-// Workspace of cnn5 for B images of H x W: hidden activations, gradient ping-pong, the hidden
-// activations' ReLU bits, and per layer the weight-gradient partials of every block and the
-// reduction stage (all five layers are reduced together after the backward pass).
+// Workspace of cnn5 for B images of H x W: hidden activations, gradient ping-pong, and per layer
+// the weight-gradient partials of every block and the reduction stage (all five layers are reduced
+// together after the backward pass). The backward takes each ReLU derivative from the activation
+// itself (x > 0), so no mask is stored.
 struct Cnn5Ws {
     uint16_t *X[4], *dY[2];
-    uint32_t *M[4];  // ReLU bits of X[l], one word per pixel
     float *part[5], *stage[5];
     int nblk[5];  // weight-gradient blocks (partials) per image and layer
     size_t bytes;
@@ -82,7 +82,6 @@ struct Cnn5Ws {
         size_t o = 0;
         for (int i = 0; i < 4; ++i) X[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
         for (int i = 0; i < 2; ++i) dY[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
-        for (int i = 0; i < 4; ++i) M[i] = reinterpret_cast<uint32_t *>(p + o), o += al256((size_t)B * H * W * 4);
         for (int l = 0; l < 5; ++l)
         {
             // the weight gradients of layers 2..5 come from the fused backward kernel
@@ -101,11 +100,7 @@ inline xdn::FwdStreamFrame cnn5_stream_frame(const Cnn5Ws &ws, const char *packe
 {
     xdn::FwdStreamFrame f{};
     f.x1 = ws.X[0] + po * 32;
-    for (int l = 1; l < 4; ++l)
-    {
-        f.x[l - 1] = ws.X[l] + po * 32;
-        f.m[l - 1] = ws.M[l] + po;
-    }
+    for (int l = 1; l < 4; ++l) f.x[l - 1] = ws.X[l] + po * 32;
     f.out = out;
     for (int l = 1; l < 5; ++l)
     {

@@ -246,13 +246,12 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
             memset(&f, 0, sizeof(f));
             f.in[0] = s.ols1; f.in[1] = s.ols2; f.in[2] = sc[i].prev_out; f.in[3] = s.albedo; f.in[4] = s.normal;
             f.out = cw[i].X[0];
-            f.mask_out = cw[i].M[0];
             f.wfrag = pk + kLay.fwd_off[0];
             f.bias = reinterpret_cast<const float *>(pk + kLay.bias_off[0]);
             f.H = s.height; f.W = s.width;
             sf[i] = cnn5_stream_frame(cw[i], pk, 0, sc[i].param, s.height, s.width);
         }
-        if (int e = hip_check(xdn::launch_conv(xdn::CONV_L1_FWD, n, fr.data(), st), "cnn forward")) return e;
+        if (int e = hip_check(xdn::launch_conv_l1(n, fr.data(), st), "cnn forward")) return e;
         if (int e = hip_check(xdn::launch_fwd_stream(n, sf.data(), st), "cnn forward")) return e;
     }
     // 5. fused spatial filter + temporal accumulation + loss + backward to the network output
@@ -291,8 +290,9 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
                               "loss"))
             return e;
     }
-    // 6. network backward: per layer one weight-gradient launch and one data-gradient launch; the
-    //    per-block weight-gradient partials of every (slot, layer) are reduced by one launch pair
+    // 6. network backward: layers 5..2 each in one fused launch (weight gradient and data gradient
+    //    into the layer below), then layer 1's weight gradient; the per-block weight-gradient
+    //    partials of every (slot, layer) are reduced by one launch pair
     if (unet)
     {
         if (int e = unet_backward_frames(uf.data(), n, st)) return e;
@@ -304,67 +304,35 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
     }
     else
     {
-        std::vector<xdn::WgFrame> wf(n);
-        std::vector<xdn::ConvFrame> cf(n);
         std::vector<xdn::RedJob> jobs;
         std::vector<xdn::BwdFrame> bf(n);
-        int cur = 0;
-        for (int l = 4; l >= 0; --l)
+        int cur = 0;  // dY ping-pong index holding the gradient of the current layer's output
+        for (int l = 4; l >= 1; --l)
         {
-            if (l >= 1)
-            {
-                // layers 2..5: weight gradient and data gradient into layer l-1 in one kernel (the
-                // last layer's dY is the FP32 gradient of the network output)
-                for (int i = 0; i < n; ++i)
-                {
-                    const xdn_slot &s = slots[i];
-                    const void *dy = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
-                    bf[i] = xdn::BwdFrame{dy, cw[i].X[l - 1], cw[i].M[l - 1], cw[i].dY[1 - cur],
-                                          static_cast<const char *>(s.packed) + kLay.dg_off[l], cw[i].part[l],
-                                          s.height, s.width, cw[i].nblk[l]};
-                    jobs.push_back(xdn::RedJob{cw[i].part[l], cw[i].stage[l], s.grad + kLay.w_off[l], cw[i].nblk[l],
-                                               kCnn5[l].ci, kCnn5[l].co});
-                }
-                if (int e = hip_check(xdn::launch_bwd(n, bf.data(), l == 4, st), "cnn backward")) return e;
-                cur = 1 - cur;
-                continue;
-            }
+            // the last layer's dY is the FP32 gradient of the network output
             for (int i = 0; i < n; ++i)
             {
                 const xdn_slot &s = slots[i];
-                xdn::WgFrame &f = wf[i];
-                memset(&f, 0, sizeof(f));
-                if (l == 0)
-                {
-                    f.x[0] = s.ols1; f.x[1] = s.ols2; f.x[2] = sc[i].prev_out; f.x[3] = s.albedo; f.x[4] = s.normal;
-                }
-                else
-                    f.x[0] = cw[i].X[l - 1];
-                f.dy = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
-                f.part = cw[i].part[l];
-                f.nblk = cw[i].nblk[l];
-                f.H = s.height; f.W = s.width;
+                const void *dy = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
+                bf[i] = xdn::BwdFrame{dy, cw[i].X[l - 1], cw[i].dY[1 - cur],
+                                      static_cast<const char *>(s.packed) + kLay.dg_off[l], cw[i].part[l],
+                                      s.height, s.width, cw[i].nblk[l]};
                 jobs.push_back(xdn::RedJob{cw[i].part[l], cw[i].stage[l], s.grad + kLay.w_off[l], cw[i].nblk[l],
                                            kCnn5[l].ci, kCnn5[l].co});
             }
-            const int wk = l == 0 ? xdn::CONV_L1_WGRAD : (l == 4 ? xdn::CONV_L5_WGRAD : xdn::CONV_MID_WGRAD);
-            if (int e = hip_check(xdn::launch_wgrad(wk, n, wf.data(), st), "cnn wgrad")) return e;
-            if (l == 0) break;
-            for (int i = 0; i < n; ++i)
-            {
-                const xdn_slot &s = slots[i];
-                xdn::ConvFrame &f = cf[i];
-                memset(&f, 0, sizeof(f));
-                f.in[0] = l == 4 ? static_cast<const void *>(sc[i].dparam) : static_cast<const void *>(cw[i].dY[cur]);
-                f.out = cw[i].dY[1 - cur];
-                f.mask = cw[i].M[l - 1];
-                f.wfrag = static_cast<const char *>(s.packed) + kLay.dg_off[l];
-                f.H = s.height; f.W = s.width;
-            }
-            const int dk = l == 4 ? xdn::CONV_L5_DGRAD : xdn::CONV_MID_DGRAD;
-            if (int e = hip_check(xdn::launch_conv(dk, n, cf.data(), st), "cnn dgrad")) return e;
+            if (int e = hip_check(xdn::launch_bwd(n, bf.data(), l == 4, st), "cnn backward")) return e;
             cur = 1 - cur;
         }
+        std::vector<xdn::WgFrame> wf(n);
+        for (int i = 0; i < n; ++i)
+        {
+            const xdn_slot &s = slots[i];
+            wf[i] = xdn::WgFrame{{s.ols1, s.ols2, sc[i].prev_out, s.albedo, s.normal}, cw[i].dY[cur], cw[i].part[0],
+                                 s.height, s.width, cw[i].nblk[0]};
+            jobs.push_back(xdn::RedJob{cw[i].part[0], cw[i].stage[0], s.grad + kLay.w_off[0], cw[i].nblk[0],
+                                       kCnn5[0].ci, kCnn5[0].co});
+        }
+        if (int e = hip_check(xdn::launch_wgrad_l1(n, wf.data(), st), "cnn wgrad")) return e;
         if (int e = hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), st), "cnn wgrad reduce")) return e;
     }
     // 7. Keras Adam on every slot's parameters and the bf16 fragments: the cnn5 update writes each

@@ -1,30 +1,28 @@
 // Bandwidth CNN of the online training step on gfx950 MFMA: five 3x3 'same' convolutions
 // 15 -> 32 -> 32 -> 32 -> 32 -> 6 with ReLU between them, BF16 operands, FP32 accumulation.
 //
-// All convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16 (M = 32 pixels of one row,
-// N = 32 output channels, K = 16 input channels):
-//   conv3x3<IN, CIN, EPI>  forward and data-gradient (the data gradient is a convolution of dY
-//     with the flipped, transposed kernel). Tile: 16 rows x 32 px per 256-lane workgroup, the
-//     tile plus a 1-px halo staged in LDS as NHWC bf16 with a padded pixel stride (conflict-free
-//     ds_read_b128). Each wave owns 4 output rows; every A fragment loaded from LDS feeds the
-//     MFMAs of up to three output rows (the three kernel rows), so LDS traffic is ~0.5 fragments
-//     per MFMA. Persistent, software-pipelined blocks; the kernel fragments sit in LDS.
-//     Inputs: bf16 activations, or the five FP32 3-channel images of the network input (the
-//     concatenation and the bf16 rounding are fused into the staging), or the FP32 6-channel
-//     gradient of the network output. Epilogues: bias + ReLU -> bf16 (plus one 32-bit word per
-//     pixel of ReLU bits, bit c = channel c > 0), the backward's ReLU mask of the layer input
-//     (read as those bits: 4 B/px instead of the 64 B/px activation) -> bf16, or bias -> FP32 6
-//     channels (network output).
-//   conv3x3_wgrad<X, CI, DY>  weight gradient: M = input channel, N = output channel, K = 16
-//     pixels (2 rows x 8 px). Both operands are staged pixel-major in LDS as they lie in HBM and
-//     read channel-major with the transposing ds_read_b64_tr_b16. A wave keeps all 9 taps' 32x32
-//     FP32 accumulators (AGPRs) and loops over many 8x32-px strips; per-block partial sums are
-//     reduced by wgrad_reduce in two deterministic stages.
-//   conv3x3_bwd  backward of a 32 -> 32 layer: the data gradient (conv3x3 scheme) and the weight
-//     gradient (conv3x3_wgrad scheme) of one 16x32-px tile from one staging of dY and X, eight
-//     waves per block; used for the three middle layers.
-//   pack_weights  FP32 master weights -> bf16 MFMA B fragments for the forward and the data
-//     gradient (flipped/transposed) convolutions.
+// The convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16 (M = 32 pixels of one row,
+// N = 32 output channels, K = 16 input channels; 16x16x32 where a side has only 16 channels):
+//   conv3x3_l1        layer 1 forward: the five FP32 3-channel input images (the concatenation
+//                     and the bf16 rounding fused into the staging) -> bias + ReLU -> bf16.
+//                     Tile: 16 rows x 32 px per 256-lane workgroup plus a 1-px halo in LDS;
+//                     each B fragment read from LDS feeds up to three output rows. Persistent,
+//                     software-pipelined blocks; the kernel fragments sit in LDS.
+//   cnn5_fwd_stream   layers 2..5 forward in one launch: a block walks a strip of rows, each of
+//                     its four waves computing one layer's row per step from 4-row LDS rings
+//                     of the layer below (the hidden activations are written once and never
+//                     read back); kernel fragments in registers.
+//   conv3x3_bwd       backward of layers 5..2: the data gradient into the layer below (conv3x3
+//                     scheme on the flipped, transposed kernel) and the weight gradient of one
+//                     16x32-px tile from one staging of dY and the layer input X, eight waves per
+//                     block. The ReLU derivative of the data gradient is read off X itself
+//                     (x > 0 in bf16), so the forward stores no mask.
+//   conv3x3_wgrad_l1  layer 1 weight gradient: M = input channel, N = output channel, K = pixels;
+//                     both operands staged pixel-major in LDS as they lie in HBM and read
+//                     channel-major with the transposing ds_read_b64_tr_b16.
+//   wgrad_reduce1/2   deterministic two-stage reduction of the per-block weight-gradient partials.
+//   pack_weights      FP32 master weights -> bf16 MFMA B fragments for the forward and the data
+//                     gradient (flipped/transposed) convolutions.
 #include "xdn_common.h"
 #include "xdn_cnn.h"
 
@@ -72,7 +70,7 @@ __device__ __forceinline__ void bst8(u32x2 v, rsrc_t r, uint32_t off) { __builti
 __device__ __forceinline__ void bst4(uint32_t v, rsrc_t r, uint32_t off) { __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0); }
 
 // ---------------------------------------------------------------------------------------------
-// Forward / data-gradient convolution
+// First layer forward
 // ---------------------------------------------------------------------------------------------
 struct ConvBatch {
     TileMap map;
@@ -82,36 +80,33 @@ static_assert(sizeof(ConvBatch) <= 4096, "kernel argument too large");
 
 constexpr int CT_W = 32, CT_H = 16;
 
-// Smallest float whose bf16 rounding (nearest even) is nonzero is just above 2^-134: a ReLU output
-// r >= 0 is stored as a positive bf16 exactly when r > 2^-134.
-constexpr float kBf16Tiny = 0x1p-134f;
-
+// Layer 1: the five FP32 3-channel images of the network input (15 channels; the concatenation and
+// the bf16 rounding are fused into the staging, channel 15 is zero) -> bias + ReLU -> bf16 32
+// channels. Tile: 16 rows x 32 px per 256-lane workgroup, the tile plus a 1-px halo staged in LDS as
+// NHWC bf16 with a padded pixel stride (conflict-free ds_read_b128).
 // Persistent and software-pipelined: a block walks the tiles it, it + gridDim.x, ... (gridDim.x a
 // multiple of 8, so a block stays on one XCD and xcd_tile keeps each XCD on a contiguous run of
 // tiles). The global loads of the next tile are issued into registers before the MFMAs of the
-// current one and land in LDS at the top of the next iteration. The kernel fragments of the tile's
+// current one and land in LDS at the end of the iteration. The kernel fragments of the tile's
 // frame sit in LDS (reloaded when the frame changes) and are the MFMA A operands; the B operands
 // are 32 pixels x 16 input channels read from the LDS tile, so the accumulators hold C[channel][pixel]
 // and a lane owns 16 channels c(k) = 8(k/4) + 4h + k%4 of one pixel: the epilogue works in
 // registers, and a permlane32 swap between the two half-waves gives each lane 2 x 8 consecutive
-// channels for 16-B stores.
-template <int IN, int CIN, int EPI>
-__global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
+// channels for 16-B stores. Each wave owns 4 output rows; every B fragment loaded from LDS feeds
+// the MFMAs of up to three output rows (the three kernel rows).
+__global__ __launch_bounds__(256, 2) void conv3x3_l1(ConvBatch b)
 {
-    constexpr int PS = CIN + 8;  // LDS pixel stride in bf16 (20 or 12 dwords: conflict-free b128 reads)
+    constexpr int PS = 16 + 8;  // LDS pixel stride in bf16 (12 dwords: conflict-free b128 reads)
     constexpr int LW = CT_W + 2, LH = CT_H + 2;
-    constexpr int KC = CIN / 16;
     __shared__ __attribute__((aligned(16))) uint16_t lds[LH * LW * PS];
-    __shared__ u32x4 wl[9 * KC * 64];  // kernel fragments of the current frame: [tap][kc][lane]
+    __shared__ u32x4 wl[9 * 64];  // kernel fragments of the current frame: [tap][lane]
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6, px = lane & 31, h = lane >> 5;
     const int total = b.map.total;
 
     // ---- input tile (+1 px halo, zero outside the image): fetch to registers, commit to LDS ----
-    constexpr int NV = IN == CONV_IN_BF16 ? (LH * LW * (CIN / 8) + 255) / 256 : (LH * LW + 255) / 256;
-    constexpr int NF = IN == CONV_IN_IMAGES ? 15 : 6;
-    u32x4 rv[IN == CONV_IN_BF16 ? NV : 1];
-    float fv[IN == CONV_IN_BF16 ? 1 : NV][NF];
+    constexpr int NV = (LH * LW + 255) / 256;
+    float fv[NV][15];
     // every load is issued (zero outside the image: offset kOob), so the loop's memory accesses
     // carry no branches
     auto fetch = [&](int it) {
@@ -119,50 +114,24 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
         const ConvFrame &G = b.f[T.f];
         const int gH = G.H, gW = G.W, tx0 = T.tx * CT_W, ty0 = T.ty * CT_H;
         const size_t npx = (size_t)gH * gW;
-        rsrc_t rin[IN == CONV_IN_IMAGES ? 5 : 1];
+        rsrc_t rin[5];
 #pragma unroll
-        for (int im = 0; im < (IN == CONV_IN_IMAGES ? 5 : 1); ++im)
-            rin[im] = make_rsrc(G.in[im], npx * (IN == CONV_IN_BF16 ? CIN * 2 : (IN == CONV_IN_IMAGES ? 12 : 24)));
+        for (int im = 0; im < 5; ++im) rin[im] = make_rsrc(G.in[im], npx * 12);
 #pragma unroll
         for (int u = 0; u < NV; ++u)
         {
-            if constexpr (IN == CONV_IN_BF16)
-            {
-                const int i = tid + 256 * u;
-                const int pix = i / (CIN / 8), q = i - pix * (CIN / 8);
-                const int ly = pix / LW, lx = pix - ly * LW;
-                const int gy = ty0 + ly - 1, gx = tx0 + lx - 1;
-                const bool ok = i < LH * LW * (CIN / 8) && gy >= 0 && gy < gH && gx >= 0 && gx < gW;
-                rv[u] = bld16(rin[0], ok ? (uint32_t)((gy * gW + gx) * CIN + q * 8) * 2u : kOob);
-            }
-            else
-            {
-                const int pix = tid + 256 * u;
-                const int ly = pix / LW, lx = pix - ly * LW;
-                const int gy = ty0 + ly - 1, gx = tx0 + lx - 1;
-                const bool ok = pix < LH * LW && gy >= 0 && gy < gH && gx >= 0 && gx < gW;
-                const uint32_t p = (uint32_t)(gy * gW + gx);
-                if constexpr (IN == CONV_IN_IMAGES)
-                {
+            const int pix = tid + 256 * u;
+            const int ly = pix / LW, lx = pix - ly * LW;
+            const int gy = ty0 + ly - 1, gx = tx0 + lx - 1;
+            const bool ok = pix < LH * LW && gy >= 0 && gy < gH && gx >= 0 && gx < gW;
+            const uint32_t p = (uint32_t)(gy * gW + gx);
 #pragma unroll
-                    for (int im = 0; im < 5; ++im)
-                    {
-                        const u32x3 v = bld12(rin[im], ok ? p * 12u : kOob);
-                        fv[u][3 * im] = __uint_as_float(v.x);
-                        fv[u][3 * im + 1] = __uint_as_float(v.y);
-                        fv[u][3 * im + 2] = __uint_as_float(v.z);
-                    }
-                }
-                else
-                {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k)
-                    {
-                        const u32x2 v = bld8(rin[0], ok ? p * 24u + 8u * k : kOob);
-                        fv[u][2 * k] = __uint_as_float(v.x);
-                        fv[u][2 * k + 1] = __uint_as_float(v.y);
-                    }
-                }
+            for (int im = 0; im < 5; ++im)
+            {
+                const u32x3 v = bld12(rin[im], ok ? p * 12u : kOob);
+                fv[u][3 * im] = __uint_as_float(v.x);
+                fv[u][3 * im + 1] = __uint_as_float(v.y);
+                fv[u][3 * im + 2] = __uint_as_float(v.z);
             }
         }
     };
@@ -170,27 +139,16 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
 #pragma unroll
         for (int u = 0; u < NV; ++u)
         {
-            if constexpr (IN == CONV_IN_BF16)
+            const int pix = tid + 256 * u;
+            if (pix < LH * LW)
             {
-                const int i = tid + 256 * u;
-                const int pix = i / (CIN / 8), q = i - pix * (CIN / 8);
-                if (i < LH * LW * (CIN / 8)) *reinterpret_cast<u32x4 *>(&lds[pix * PS + q * 8]) = rv[u];
-            }
-            else
-            {
-                const int pix = tid + 256 * u;
-                if (pix < LH * LW)
-                {
-                    float v[16];
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) v[k] = k < NF ? fv[u][k] : 0.f;
-                    u32x4 a, c;
-                    a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = pack2(v[6], v[7]);
-                    c.x = pack2(v[8], v[9]); c.y = pack2(v[10], v[11]); c.z = pack2(v[12], v[13]);
-                    c.w = pack2(v[14], v[15]);
-                    *reinterpret_cast<u32x4 *>(&lds[pix * PS]) = a;
-                    *reinterpret_cast<u32x4 *>(&lds[pix * PS + 8]) = c;
-                }
+                const float *v = fv[u];
+                u32x4 a, c;
+                a.x = pack2(v[0], v[1]); a.y = pack2(v[2], v[3]); a.z = pack2(v[4], v[5]); a.w = pack2(v[6], v[7]);
+                c.x = pack2(v[8], v[9]); c.y = pack2(v[10], v[11]); c.z = pack2(v[12], v[13]);
+                c.w = pack2(v[14], 0.f);
+                *reinterpret_cast<u32x4 *>(&lds[pix * PS]) = a;
+                *reinterpret_cast<u32x4 *>(&lds[pix * PS + 8]) = c;
             }
         }
     };
@@ -201,8 +159,8 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
     __shared__ float4 wb[8];  // bias of the current frame, [32]
     auto frame_setup = [&](int fi) {
         const ConvFrame &G = b.f[fi];
-        for (int e = tid; e < 9 * KC * 64; e += 256) wl[e] = static_cast<const u32x4 *>(G.wfrag)[e];
-        if (tid < 32) reinterpret_cast<float *>(wb)[tid] = (EPI == CONV_EPI_MASK) ? 0.f : G.bias[tid];
+        for (int e = tid; e < 9 * 64; e += 256) wl[e] = static_cast<const u32x4 *>(G.wfrag)[e];
+        if (tid < 32) reinterpret_cast<float *>(wb)[tid] = G.bias[tid];
     };
     int cur_f = -1;
     if ((int)blockIdx.x < total)
@@ -219,20 +177,6 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
     const int H = F.H, W = F.W;
     const int x0 = L.tx * CT_W, y0 = L.ty * CT_H;
     __syncthreads();  // tile `it` and its frame's fragments are in LDS
-    // the backward's ReLU bits of the lane's pixel in each of its 4 output rows, loaded now so their
-    // latency hides behind the MFMAs, and before the prefetch so that waiting for them does not wait
-    // for it
-    uint32_t mrow[4] = {0u, 0u, 0u, 0u};
-    if constexpr (EPI == CONV_EPI_MASK)
-    {
-        const rsrc_t rm = make_rsrc(F.mask, (size_t)H * W * 4);
-#pragma unroll
-        for (int orow = 0; orow < 4; ++orow)
-        {
-            const int gy = y0 + 4 * wv + orow, gx = x0 + px;
-            mrow[orow] = bld4(rm, gy < H && gx < W ? (uint32_t)(gy * W + gx) * 4u : kOob);
-        }
-    }
     const int nxt = it + (int)gridDim.x;
     // the next tile's loads (the last iteration reloads its own tile: no branch around the loads)
     fetch(nxt < total ? nxt : it);
@@ -250,97 +194,53 @@ __global__ __launch_bounds__(256, 2) void conv3x3(ConvBatch b)
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc[r][k] = 0.f;
     // wave wv owns output rows 4wv..4wv+3, which read LDS rows 4wv..4wv+5: each B fragment (LDS row
-    // lr, column shift dx, channel chunk kc) is read once and feeds the kernel rows dy with
-    // output row lr - dy in 0..3
+    // lr, column shift dx) is read once and feeds the kernel rows dy with output row lr - dy in 0..3
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx)
+    {
+        u32x4 wk[3], bx[6];
 #pragma unroll
-        for (int kc = 0; kc < KC; ++kc)
-        {
-            u32x4 wk[3], bx[6];
+        for (int dy = 0; dy < 3; ++dy) wk[dy] = wl[(dy * 3 + dx) * 64 + lane];
 #pragma unroll
-            for (int dy = 0; dy < 3; ++dy) wk[dy] = wl[((dy * 3 + dx) * KC + kc) * 64 + lane];
+        for (int lr = 0; lr < 6; ++lr)
+            bx[lr] = *reinterpret_cast<const u32x4 *>(&lds[((4 * wv + lr) * LW + px + dx) * PS + 8 * h]);
 #pragma unroll
-            for (int lr = 0; lr < 6; ++lr)
-                bx[lr] = *reinterpret_cast<const u32x4 *>(&lds[((4 * wv + lr) * LW + px + dx) * PS + kc * 16 + 8 * h]);
+        for (int lr = 0; lr < 6; ++lr)
 #pragma unroll
-            for (int lr = 0; lr < 6; ++lr)
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
-                {
-                    const int orow = lr - dy;
-                    if (orow >= 0 && orow < 4) acc[orow] = mfma(wk[dy], bx[lr], acc[orow]);
-                }
-        }
+            for (int dy = 0; dy < 3; ++dy)
+            {
+                const int orow = lr - dy;
+                if (orow >= 0 && orow < 4) acc[orow] = mfma(wk[dy], bx[lr], acc[orow]);
+            }
+    }
 
     // ---- epilogue, per output row: lane (px, h) holds channels c(k) of pixel (row, x0 + px) ----
-    const size_t npx = (size_t)H * W;
-    const rsrc_t rout = make_rsrc(F.out, npx * (EPI == CONV_EPI_F32_6 ? 24 : 64));
-    const rsrc_t rmo = make_rsrc(EPI == CONV_EPI_RELU ? F.mask_out : nullptr, npx * 4);
+    const rsrc_t rout = make_rsrc(F.out, (size_t)H * W * 64);
 #pragma unroll
     for (int orow = 0; orow < 4; ++orow)
     {
         const int gy = y0 + 4 * wv + orow, gx = x0 + px;
         const bool in = gy < H && gx < W;
         const uint32_t p = (uint32_t)(gy * W + gx);
-        if constexpr (EPI == CONV_EPI_F32_6)
-        {
-            // network output: channels 0..3 on the h = 0 lane, 4..5 on the h = 1 lane
-            const float v0 = acc[orow][0] + bias[0], v1 = acc[orow][1] + bias[1];
-            const float v2 = acc[orow][2] + bias[2], v3 = acc[orow][3] + bias[3];
-            const uint32_t o = in ? p * 24u + 16u * h : kOob;
-            bst8(u32x2{__float_as_uint(v0), __float_as_uint(v1)}, rout, o);
-            bst8(u32x2{__float_as_uint(v2), __float_as_uint(v3)}, rout, h == 0 ? o + 8u : kOob);
-        }
-        else
-        {
-            float r[16];
+        uint32_t d[8];  // d[2j + e] = channels 8j + 4h + 2e, +1
 #pragma unroll
-            for (int k = 0; k < 16; ++k)
-                r[k] = EPI == CONV_EPI_RELU ? fmaxf(acc[orow][k] + bias[k], 0.f) : acc[orow][k];
-            uint32_t d[8];  // d[2j + e] = channels 8j + 4h + 2e, +1
+        for (int q = 0; q < 8; ++q)
+            d[q] = pack2(fmaxf(acc[orow][2 * q] + bias[2 * q], 0.f), fmaxf(acc[orow][2 * q + 1] + bias[2 * q + 1], 0.f));
+        // lane h gathers channels 16g + 8h .. 16g + 8h + 7 (g = 0, 1): the low half-wave keeps its
+        // group 2g and takes the high half's; the high half keeps group 2g + 1 and takes the low's
+        u32x4 v[2];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) d[q] = pack2(r[2 * q], r[2 * q + 1]);
-            if constexpr (EPI == CONV_EPI_RELU)
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
             {
-                // ReLU bits of the pixel, bit c = (bf16 channel c > 0); this lane's 16 channels, then
-                // the other half-wave's 16 through a permlane32 swap
-                uint32_t bits = 0u;
-#pragma unroll
-                for (int k = 0; k < 16; ++k) bits |= (r[k] > kBf16Tiny ? 1u : 0u) << (8 * (k >> 2) + (k & 3));
-                bits <<= 4 * h;
-                const auto pb = __builtin_amdgcn_permlane32_swap(bits, bits, false, false);
-                bst4(bits | (uint32_t)pb[1], rmo, in && h == 0 ? p * 4u : kOob);
+                const auto sw = __builtin_amdgcn_permlane32_swap(d[4 * g + e], d[4 * g + 2 + e], false, false);
+                v[g][e] = (uint32_t)sw[0];
+                v[g][2 + e] = (uint32_t)sw[1];
             }
-            else
-            {
-                // keep the gradient where the layer's bf16 activation is > 0 (ReLU derivative)
-                const uint32_t m = mrow[orow] >> (4 * h);
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                {
-                    const int pos = 8 * (q >> 1) + 2 * (q & 1);
-                    const uint32_t lo = (m >> pos) & 1u ? 0x0000FFFFu : 0u;
-                    const uint32_t hi = (m >> (pos + 1)) & 1u ? 0xFFFF0000u : 0u;
-                    d[q] &= lo | hi;
-                }
-            }
-            // lane h gathers channels 16g + 8h .. 16g + 8h + 7 (g = 0, 1): the low half-wave keeps its
-            // group 2g and takes the high half's; the high half keeps group 2g + 1 and takes the low's
-            u32x4 v[2];
-#pragma unroll
-            for (int g = 0; g < 2; ++g)
-#pragma unroll
-                for (int e = 0; e < 2; ++e)
-                {
-                    const auto sw = __builtin_amdgcn_permlane32_swap(d[4 * g + e], d[4 * g + 2 + e], false, false);
-                    v[g][e] = (uint32_t)sw[0];
-                    v[g][2 + e] = (uint32_t)sw[1];
-                }
-            const uint32_t o = in ? p * 64u + 16u * h : kOob;
-            bst16(v[0], rout, o);
-            bst16(v[1], rout, in ? o + 32u : kOob);
-        }
+        const uint32_t o = in ? p * 64u + 16u * h : kOob;
+        bst16(v[0], rout, o);
+        bst16(v[1], rout, in ? o + 32u : kOob);
     }
     __syncthreads();  // every wave is done reading tile `it`
     if (nxt < total)
@@ -379,8 +279,8 @@ static_assert(sizeof(FwdStreamBatch) <= 4096, "kernel argument too large");
 // 4-row LDS ring (written in earlier steps; the previous layer writes row r + 2 in this step), and
 // writes it to its own ring (zero outside the image: the next layer's 'same' padding). Layer 1's
 // output rows stream in from HBM two steps ahead through registers. Each layer stores the rows of
-// [ys, ye) and the columns of the strip, exactly as the one-layer conv3x3 kernel computes them
-// (same MFMA order: column shift, channel chunk, kernel row), so the outputs are bit-identical.
+// [ys, ye) and the columns of the strip, with the MFMAs of an output row in a fixed order (column
+// shift, channel chunk, kernel row), so a row is the same whichever strip or segment computes it.
 // One barrier per step. Kernel fragments and bias of the wave's layer sit in registers.
 __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
 {
@@ -415,7 +315,6 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
     const rsrc_t rx1 = make_rsrc(F.x1, npx * 64);
     const rsrc_t rout = make_rsrc(wv < 3 ? static_cast<const void *>(F.x[wv < 3 ? wv : 0]) : F.out,
                                   npx * (wv < 3 ? 64 : 24));
-    const rsrc_t rmo = make_rsrc(wv < 3 ? F.m[wv < 3 ? wv : 0] : nullptr, npx * 4);
     // layer 1's rows: chunk tid (< 136) = pixel tid / 4 of the ring (column xw - 1 + tid / 4), quarter tid % 4
     const int lpx = tid >> 2, lq = tid & 3, lcol = xw - 1 + lpx;
     const bool lrow = tid < FS_PX * 4 && lcol >= 0 && lcol < W;
@@ -436,10 +335,9 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
     // prefetch then count the same accesses behind it whether it was issued here or in a step.
     u32x4 pf0 = bld16(rx1, xoff(ys - 1));
     const u32x4 zero4 = {0u, 0u, 0u, 0u};
-    bst16(zero4, rout, kOob); bst16(zero4, rout, kOob + 32u); bst4(0u, rmo, kOob); bst8(u32x2{0u, 0u}, rout, kOob + 64u);
+    bst16(zero4, rout, kOob); bst16(zero4, rout, kOob + 32u); bst8(u32x2{0u, 0u}, rout, kOob + 64u);
     u32x4 pf1 = bld16(rx1, xoff(ys));
-    bst16(zero4, rout, kOob + 96u); bst16(zero4, rout, kOob + 128u); bst4(0u, rmo, kOob + 4u);
-    bst8(u32x2{0u, 0u}, rout, kOob + 160u);
+    bst16(zero4, rout, kOob + 96u); bst16(zero4, rout, kOob + 128u); bst8(u32x2{0u, 0u}, rout, kOob + 160u);
     __syncthreads();
 
     const uint16_t *in = &ring[wv][0][0];
@@ -476,11 +374,6 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
         uint32_t d[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) d[q] = pack2(v[2 * q], v[2 * q + 1]);
-        uint32_t bits = 0u;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) bits |= (v[k] > kBf16Tiny ? 1u : 0u) << (8 * (k >> 2) + (k & 3));
-        bits <<= 4 * h;
-        const auto pb = __builtin_amdgcn_permlane32_swap(bits, bits, false, false);
         u32x4 o2[2];
 #pragma unroll
         for (int g = 0; g < 2; ++g)
@@ -498,11 +391,9 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
                             : o2[0];
         const uint32_t oA = !st ? kOob : (l5 ? (h == 0 ? p * 24u : kOob) : p * 64u + 16u * h);
         const uint32_t oB = st && !l5 ? p * 64u + 16u * h + 32u : kOob;
-        const uint32_t oM = st && !l5 && h == 0 ? p * 4u : kOob;
         const uint32_t oC = st && l5 && h == 1 ? p * 24u + 16u : kOob;
         bst16(sA, rout, oA);
         bst16(o2[1], rout, oB);
-        bst4(bits | (uint32_t)pb[1], rmo, oM);
         bst8(u32x2{__float_as_uint(f0), __float_as_uint(f1)}, rout, oC);
         if (!l5)
         {
@@ -547,20 +438,20 @@ __device__ __forceinline__ uint2 ld_tr(const uint16_t *p)
     return __builtin_bit_cast(uint2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)p));
 }
 
-// Weight gradient dW[t][ci][co] = sum_p X[p + shift_t][ci] dY[p][co] over 8x32-px strips.
-// X (with a 1-px halo) and dY are staged pixel-major exactly as they sit in HBM (16-B stores);
-// the channel-major MFMA operands (M = ci, N = co, K = 16 px: 8 px of strip row 2wv + h per lane
-// half h) come from transposing reads, the +-1-px shifts of the taps being address offsets.
-// Wave wv accumulates strip rows 2wv and 2wv+1; the four waves' accumulators are summed through
-// LDS before the block writes its partial.
-template <int XK, int CI, int DYK>
-__global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
+// Weight gradient of layer 1, dW[t][ci][co] = sum_p X[p + shift_t][ci] dY[p][co] over 8x32-px
+// strips, X = the five FP32 3-channel input images (15 channels, channel 15 zero), dY = the bf16
+// gradient of the layer's 32-channel pre-activation. X (with a 1-px halo) and dY are staged
+// pixel-major (16-B stores); the channel-major MFMA operands come from transposing reads, the
+// +-1-px shifts of the taps being address offsets. The product runs on v_mfma_f32_16x16x32_bf16:
+// M = the 16 input channels, N = 16 output channels (two tiles), K = 32 px = one strip row (8 px
+// per lane group kg = lane / 16). Wave wv accumulates strip rows 2wv and 2wv+1; the four waves'
+// accumulators are summed through LDS before the block writes its partial.
+__global__ __launch_bounds__(256, 2) void conv3x3_wgrad_l1(WgBatch b)
 {
-    constexpr int XS = CI > 16 ? 32 : 16;                // staged X channels (pixel stride)
-    constexpr int DS = DYK == CONV_IN_BF16 ? 32 : 16;    // staged dY channels
+    constexpr int XS = 16, DS = 32;  // staged X / dY channels (pixel strides)
     constexpr int X_ELEMS = WX_R * WX_C * XS, D_ELEMS = WS_H * WS_W * DS;
-    constexpr int LDS_ELEMS = (X_ELEMS + D_ELEMS) > 8192 ? (X_ELEMS + D_ELEMS) : 8192;  // >= 4 x 1024 floats
-    __shared__ __attribute__((aligned(16))) uint16_t lds[LDS_ELEMS];
+    static_assert(X_ELEMS + D_ELEMS >= 4 * 1024 * 2, "partial reduction space");
+    __shared__ __attribute__((aligned(16))) uint16_t lds[X_ELEMS + D_ELEMS];
     __shared__ float dbs[4][32];
     uint16_t *xs = lds, *ds = lds + X_ELEMS;
     int f = 0;
@@ -569,82 +460,45 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
     const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
     const int H = F.H, W = F.W;
     const int sx = (W + WS_W - 1) / WS_W, sy = (H + WS_H - 1) / WS_H;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, h = lane >> 5, ch = lane & 31;
-    const int rr = 2 * wv + h;                           // strip row of this lane half
-    const int grp = (lane >> 4) & 1, s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3;
-    const int cbx = (XS > 16 ? 16 * grp : 0) + 4 * qq;   // channel offset addressed by this lane
-    const int cbd = (DS > 16 ? 16 * grp : 0) + 4 * qq;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3, kg = lane >> 4;
 
-    // With fewer than four 16x16 channel tiles (the first layer: 16 inputs, the last: 6 outputs)
-    // the product runs on v_mfma_f32_16x16x32_bf16 instead: M = 16 input channels (MTI tiles),
-    // N = 16 output channels (NTI tiles), K = 32 px = one strip row (8 px per lane group
-    // kg = lane / 16), which skips the padded half or three quarters of the 32x32 tiles.
-    constexpr int MTI = XS / 16, NTI = DS / 16;
-    constexpr bool T16 = MTI * NTI < 4;
     typedef __attribute__((ext_vector_type(4))) float f32x4;
-    f32x16 acc[T16 ? 1 : 9];
-    f32x4 acc16[T16 ? 9 : 1][T16 ? MTI : 1][T16 ? NTI : 1];
-    float db16[T16 ? NTI : 1] = {};
-    if constexpr (T16)
-    {
+    f32x4 acc16[9][2];  // [tap][output-channel tile]
 #pragma unroll
-        for (int t = 0; t < 9; ++t)
+    for (int t = 0; t < 9; ++t)
 #pragma unroll
-            for (int m = 0; m < MTI; ++m)
+        for (int n = 0; n < 2; ++n)
 #pragma unroll
-                for (int n = 0; n < NTI; ++n)
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) acc16[t][m][n][k] = 0.f;
-    }
-    else
-#pragma unroll
-        for (int t = 0; t < 9; ++t)
-#pragma unroll
-            for (int k = 0; k < 16; ++k) acc[t][k] = 0.f;
-    float db = 0.f;
-    const int kg = lane >> 4;
+            for (int k = 0; k < 4; ++k) acc16[t][n][k] = 0.f;
+    float db16[2] = {0.f, 0.f};
 
     // Software pipeline: the global loads of strip s + nblk are issued into registers before the
     // MFMAs of strip s and land in LDS after them.
-    constexpr int NX = XK == CONV_IN_BF16 ? (WX_R * WX_C * 4 + 255) / 256 : (WX_R * WX_C + 255) / 256;
-    constexpr int ND = DYK == CONV_IN_BF16 ? WS_H * WS_W * 4 / 256 : 1;
-    u32x4 rx[NX], rd[ND];
-    float fx[XK == CONV_IN_BF16 ? 1 : NX][15];
-    float fd[6];
+    constexpr int NX = (WX_R * WX_C + 255) / 256;
+    constexpr int ND = WS_H * WS_W * 4 / 256;
+    u32x4 rd[ND];
+    float fx[NX][15];
     auto fetch = [&](int s) {
         const int y0 = (s / sx) * WS_H, x0 = (s % sx) * WS_W;
-        // X: LDS pixel (r, col) <-> (y0 - 1 + r, x0 - 1 + col), zero outside the image
+        // X: LDS pixel (r, col) <-> (y0 - 1 + r, x0 - 1 + col), zero outside the image; one pixel per
+        // item: the 15 channels of the five 3-channel images
 #pragma unroll
         for (int u = 0; u < NX; ++u)
         {
-            if constexpr (XK == CONV_IN_BF16)
-            {
-                const int i = tid + 256 * u;
-                const int pix = i >> 2, q = i & 3;
-                const int r = pix / WX_C, col = pix - r * WX_C;
-                const int gy = y0 - 1 + r, gx = x0 - 1 + col;
-                rx[u] = u32x4{0u, 0u, 0u, 0u};
-                if (i < WX_R * WX_C * 4 && gy >= 0 && gy < H && gx >= 0 && gx < W)
-                    rx[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.x[0]) +
-                                                             ((size_t)gy * W + gx) * 32 + q * 8);
-            }
-            else
-            {
-                // one pixel per item: the 15 channels of the five 3-channel images
-                const int pix = tid + 256 * u;
-                const int r = pix / WX_C, col = pix - r * WX_C;
-                const int gy = y0 - 1 + r, gx = x0 - 1 + col;
+            const int pix = tid + 256 * u;
+            const int r = pix / WX_C, col = pix - r * WX_C;
+            const int gy = y0 - 1 + r, gx = x0 - 1 + col;
 #pragma unroll
-                for (int j = 0; j < 15; ++j) fx[u][j] = 0.f;
-                if (pix < WX_R * WX_C && gy >= 0 && gy < H && gx >= 0 && gx < W)
+            for (int j = 0; j < 15; ++j) fx[u][j] = 0.f;
+            if (pix < WX_R * WX_C && gy >= 0 && gy < H && gx >= 0 && gx < W)
+            {
+                const size_t p = (size_t)gy * W + gx;
+#pragma unroll
+                for (int im = 0; im < 5; ++im)
                 {
-                    const size_t p = (size_t)gy * W + gx;
-#pragma unroll
-                    for (int im = 0; im < 5; ++im)
-                    {
-                        const float *sp = static_cast<const float *>(F.x[im]) + 3 * p;
-                        fx[u][3 * im] = sp[0]; fx[u][3 * im + 1] = sp[1]; fx[u][3 * im + 2] = sp[2];
-                    }
+                    const float *sp = static_cast<const float *>(F.x[im]) + 3 * p;
+                    fx[u][3 * im] = sp[0]; fx[u][3 * im + 1] = sp[1]; fx[u][3 * im + 2] = sp[2];
                 }
             }
         }
@@ -652,71 +506,37 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
 #pragma unroll
         for (int u = 0; u < ND; ++u)
         {
-            if constexpr (DYK == CONV_IN_BF16)
-            {
-                const int i = tid + 256 * u;
-                const int pix = i >> 2, q = i & 3;
-                const int r = pix / WS_W, col = pix - r * WS_W;
-                const int gy = y0 + r, gx = x0 + col;
-                rd[u] = u32x4{0u, 0u, 0u, 0u};
-                if (gy < H && gx < W)
-                    rd[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.dy) +
-                                                             ((size_t)gy * W + gx) * 32 + q * 8);
-            }
-            else
-            {
-                const int r = tid / WS_W, col = tid - r * WS_W;
-                const int gy = y0 + r, gx = x0 + col;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) fd[k] = 0.f;
-                if (gy < H && gx < W)
-                {
-                    const float *sp = static_cast<const float *>(F.dy) + 6 * ((size_t)gy * W + gx);
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) fd[k] = sp[k];
-                }
-            }
+            const int i = tid + 256 * u;
+            const int pix = i >> 2, q = i & 3;
+            const int r = pix / WS_W, col = pix - r * WS_W;
+            const int gy = y0 + r, gx = x0 + col;
+            rd[u] = u32x4{0u, 0u, 0u, 0u};
+            if (gy < H && gx < W)
+                rd[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.dy) + ((size_t)gy * W + gx) * 32 +
+                                                         q * 8);
         }
     };
     auto commit = [&]() {
 #pragma unroll
         for (int u = 0; u < NX; ++u)
         {
-            if constexpr (XK == CONV_IN_BF16)
+            const int pix = tid + 256 * u;
+            if (pix < WX_R * WX_C)
             {
-                const int i = tid + 256 * u;
-                if (i < WX_R * WX_C * 4) *reinterpret_cast<u32x4 *>(&xs[(i >> 2) * XS + (i & 3) * 8]) = rx[u];
-            }
-            else
-            {
-                const int pix = tid + 256 * u;
-                if (pix < WX_R * WX_C)
-                {
-                    u32x4 a, c;
-                    a.x = pack2(fx[u][0], fx[u][1]); a.y = pack2(fx[u][2], fx[u][3]);
-                    a.z = pack2(fx[u][4], fx[u][5]); a.w = pack2(fx[u][6], fx[u][7]);
-                    c.x = pack2(fx[u][8], fx[u][9]); c.y = pack2(fx[u][10], fx[u][11]);
-                    c.z = pack2(fx[u][12], fx[u][13]); c.w = pack2(fx[u][14], 0.f);
-                    *reinterpret_cast<u32x4 *>(&xs[pix * XS]) = a;
-                    *reinterpret_cast<u32x4 *>(&xs[pix * XS + 8]) = c;
-                }
+                u32x4 a, c;
+                a.x = pack2(fx[u][0], fx[u][1]); a.y = pack2(fx[u][2], fx[u][3]);
+                a.z = pack2(fx[u][4], fx[u][5]); a.w = pack2(fx[u][6], fx[u][7]);
+                c.x = pack2(fx[u][8], fx[u][9]); c.y = pack2(fx[u][10], fx[u][11]);
+                c.z = pack2(fx[u][12], fx[u][13]); c.w = pack2(fx[u][14], 0.f);
+                *reinterpret_cast<u32x4 *>(&xs[pix * XS]) = a;
+                *reinterpret_cast<u32x4 *>(&xs[pix * XS + 8]) = c;
             }
         }
 #pragma unroll
         for (int u = 0; u < ND; ++u)
         {
-            if constexpr (DYK == CONV_IN_BF16)
-            {
-                const int i = tid + 256 * u;
-                *reinterpret_cast<u32x4 *>(&ds[(i >> 2) * DS + (i & 3) * 8]) = rd[u];
-            }
-            else
-            {
-                u32x4 a, z = {0u, 0u, 0u, 0u};
-                a.x = pack2(fd[0], fd[1]); a.y = pack2(fd[2], fd[3]); a.z = pack2(fd[4], fd[5]); a.w = 0u;
-                *reinterpret_cast<u32x4 *>(&ds[tid * DS]) = a;
-                *reinterpret_cast<u32x4 *>(&ds[tid * DS + 8]) = z;
-            }
+            const int i = tid + 256 * u;
+            *reinterpret_cast<u32x4 *>(&ds[(i >> 2) * DS + (i & 3) * 8]) = rd[u];
         }
     };
 
@@ -729,57 +549,32 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
         commit();
         __syncthreads();
         if (s + sw.step < sw.end) fetch(s + sw.step);
-        if constexpr (T16)
-        {
 #pragma unroll
-            for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < 2; ++r)
+        {
+            const int row = 2 * wv + r;
+            u32x4 Bn[2];
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
             {
-                const int row = 2 * wv + r;
-                u32x4 Bn[NTI];
-#pragma unroll
-                for (int n = 0; n < NTI; ++n)
-                {
-                    const uint16_t *dp = &ds[(row * WS_W + 8 * kg + jq) * DS + 16 * n + 4 * qq];
-                    const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * DS);
-                    Bn[n] = u32x4{b0.x, b0.y, b1.x, b1.y};
-                    db16[n] += (bf_lo(Bn[n].x) + bf_hi(Bn[n].x)) + (bf_lo(Bn[n].y) + bf_hi(Bn[n].y)) +
-                               (bf_lo(Bn[n].z) + bf_hi(Bn[n].z)) + (bf_lo(Bn[n].w) + bf_hi(Bn[n].w));
-                }
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                    for (int sh = 0; sh < 3; ++sh)
-#pragma unroll
-                        for (int m = 0; m < MTI; ++m)
-                        {
-                            const uint16_t *xp = &xs[((row + dy) * WX_C + 8 * kg + sh + jq) * XS + 16 * m + 4 * qq];
-                            const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
-                            const bf16x8 A = __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y});
-#pragma unroll
-                            for (int n = 0; n < NTI; ++n)
-                                acc16[dy * 3 + sh][m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                    A, __builtin_bit_cast(bf16x8, Bn[n]), acc16[dy * 3 + sh][m][n], 0, 0, 0);
-                        }
+                const uint16_t *dp = &ds[(row * WS_W + 8 * kg + jq) * DS + 16 * n + 4 * qq];
+                const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * DS);
+                Bn[n] = u32x4{b0.x, b0.y, b1.x, b1.y};
+                db16[n] += (bf_lo(Bn[n].x) + bf_hi(Bn[n].x)) + (bf_lo(Bn[n].y) + bf_hi(Bn[n].y)) +
+                           (bf_lo(Bn[n].z) + bf_hi(Bn[n].z)) + (bf_lo(Bn[n].w) + bf_hi(Bn[n].w));
             }
-            continue;
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-        {
-            const uint16_t *dp = &ds[(rr * WS_W + 8 * c + jq) * DS + cbd];
-            const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * DS);
-            const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
-            db += (bf_lo(B.x) + bf_hi(B.x)) + (bf_lo(B.y) + bf_hi(B.y)) + (bf_lo(B.z) + bf_hi(B.z)) +
-                  (bf_lo(B.w) + bf_hi(B.w));
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
                 for (int sh = 0; sh < 3; ++sh)
                 {
-                    const uint16_t *xp = &xs[((rr + dy) * WX_C + 8 * c + sh + jq) * XS + cbx];
+                    const uint16_t *xp = &xs[((row + dy) * WX_C + 8 * kg + sh + jq) * XS + 4 * qq];
                     const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
-                    const u32x4 A = {a0.x, a0.y, a1.x, a1.y};
-                    acc[dy * 3 + sh] = mfma(A, B, acc[dy * 3 + sh]);
+                    const bf16x8 A = __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y});
+#pragma unroll
+                    for (int n = 0; n < 2; ++n)
+                        acc16[dy * 3 + sh][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            A, __builtin_bit_cast(bf16x8, Bn[n]), acc16[dy * 3 + sh][n], 0, 0, 0);
                 }
         }
     }
@@ -790,47 +585,23 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad(WgBatch b)
     for (int t = 0; t < 9; ++t)
     {
         __syncthreads();
-        if constexpr (T16)
-        {
-            // 16x16 accumulators: column co = 16n + lane % 16, rows ci = 16m + 4kg + i
+        // 16x16 accumulators: column co = 16n + lane % 16, rows ci = 4kg + i
 #pragma unroll
-            for (int m = 0; m < MTI; ++m)
+        for (int n = 0; n < 2; ++n)
 #pragma unroll
-                for (int n = 0; n < NTI; ++n)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        red[wv * 1024 + (16 * m + 4 * kg + i) * 32 + 16 * n + (lane & 15)] = acc16[t][m][n][i];
-        }
-        else
-#pragma unroll
-            for (int k = 0; k < 16; ++k)
-            {
-                const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
-                red[wv * 1024 + ci * 32 + ch] = acc[t][k];
-            }
+            for (int i = 0; i < 4; ++i) red[wv * 1024 + (4 * kg + i) * 32 + 16 * n + (lane & 15)] = acc16[t][n][i];
         __syncthreads();
-        // entries outside the computed channel tiles (padding) are zero
+        // input channels 16..31 are padding (zero)
         for (int e = tid; e < 1024; e += 256)
-            out[t * 1024 + e] = !T16 || ((e >> 5) < 16 * MTI && (e & 31) < 16 * NTI)
-                                    ? (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e])
-                                    : 0.f;
+            out[t * 1024 + e] = (e >> 5) < 16 ? (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]) : 0.f;
     }
-    if constexpr (T16)
-    {
 #pragma unroll
-        for (int n = 0; n < NTI; ++n)
-        {
-            float v = db16[n];
-            v += __shfl_xor(v, 16);
-            v += __shfl_xor(v, 32);
-            if (lane < 16) dbs[wv][16 * n + lane] = v;
-        }
-        if (NTI == 1 && lane >= 16 && lane < 32) dbs[wv][lane] = 0.f;
-    }
-    else
+    for (int n = 0; n < 2; ++n)
     {
-        db += __shfl_xor(db, 32);
-        if (h == 0) dbs[wv][ch] = db;
+        float v = db16[n];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (lane < 16) dbs[wv][16 * n + lane] = v;
     }
     __syncthreads();
     if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
@@ -848,27 +619,29 @@ static_assert(sizeof(BwdBatch) <= 4096, "kernel argument too large");
 
 // One staging of a 16x32-px tile of dY and of the layer input X (both with their 1-px halo, zero
 // outside the image) feeds both products of the layer's backward pass:
-//   data gradient  dX = conv(dY, flipped/transposed W), ReLU-masked by X's bits: the conv3x3
-//                  scheme (B operands read from the dY tile with ds_read_b128);
-//   weight gradient dW[t] += X(p + shift_t)^T dY(p) over the tile's pixels: the conv3x3_wgrad
-//                  scheme (transposing reads of both tiles), kept in registers across the block's
-//                  tiles.
+//   data gradient  dX = conv(dY, flipped/transposed W), kept where X's bf16 value is nonzero (the
+//                  ReLU derivative, read from the staged X tile); B operands read from the dY tile
+//                  with ds_read_b128, as conv3x3_l1 reads its input tile;
+//   weight gradient dW[t] += X(p + shift_t)^T dY(p) over the tile's pixels: transposing reads of
+//                  both tiles, as in conv3x3_wgrad_l1, accumulators kept in registers across the
+//                  block's tiles.
 // dY is read from HBM once instead of once per product. Eight waves share the work of a tile:
 // wave w computes the data gradient of tile rows 2w, 2w+1 and the weight gradient of taps
 // 0..4 (w < 4) or 5..8 (w >= 4) over tile rows 4(w%4) .. 4(w%4)+3, so each wave holds 80 + 32
 // accumulators and the next tile's prefetch, which fits two waves per SIMD (one 512-lane block per
-// CU: the two staged tiles and the kernel fragments take 107 KB of LDS). The block walks a fixed
+// CU: the two staged tiles and the kernel fragments take 117 KB of LDS; both tiles have an 80-B
+// pixel stride, so a lane row reading one pixel each is free of bank conflicts beyond 2-way). The block walks a fixed
 // set of its frame's tiles (xcd_strips over tiles) and writes one partial, so a frame's gradient is
 // the same alone and in a batch.
 // L5 (the last layer, 32 -> 6): dY is the FP32 6-channel gradient of the network output, staged as
-// bf16 16 channels (zero padded) like the one-layer kernels do; the data gradient has one 16-channel
-// K chunk, and the weight gradient runs on v_mfma_f32_16x16x32_bf16 (M = 16 input channels, two
-// tiles; N = the 16 output channels; K = the 32 px of one tile row), as conv3x3_wgrad does for it.
+// bf16 16 channels (zero padded); the data gradient has one 16-channel K chunk, and the weight
+// gradient runs on v_mfma_f32_16x16x32_bf16 (M = 16 input channels, two tiles; N = the 16 output
+// channels; K = the 32 px of one tile row).
 template <bool L5>
 __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
 {
     constexpr int PS = L5 ? 24 : 40;       // dY pixel stride in bf16 (conflict-free b128 reads)
-    constexpr int XS = 32;                 // X pixel stride
+    constexpr int XS = 40;                 // X pixel stride in bf16 (conflict-free reads of a pixel row)
     constexpr int KC = L5 ? 1 : 2;
     constexpr int LW = CT_W + 2, LH = CT_H + 2;
     constexpr int NCH = LH * LW * 4;       // 16-B chunks of one staged X tile (and dY tile, middle layers)
@@ -914,7 +687,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
     // at its commit, placed behind the tile's stores, does not wait for those stores.
     const size_t npx = (size_t)H * W;
     const rsrc_t rdy = make_rsrc(F.dy, npx * (L5 ? 24 : 64)), rxx = make_rsrc(gx, npx * 64);
-    const rsrc_t rmask = make_rsrc(F.mask, npx * 4), rdx = make_rsrc(F.dx, npx * 64);
+    const rsrc_t rdx = make_rsrc(F.dx, npx * 64);
     u32x4 rd[L5 ? 1 : NV], rx[NV];
     float fd[L5 ? NVD : 1][6];
     auto fetch = [&](int t) {
@@ -985,14 +758,6 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
     {
         const int y0 = (t / ntx) * CT_H, x0 = (t % ntx) * CT_W;
         __syncthreads();  // tile t is in LDS
-        // the ReLU bits first: waiting for them in the epilogue then does not wait for the prefetch
-        uint32_t mrow[2];
-#pragma unroll
-        for (int orow = 0; orow < 2; ++orow)
-        {
-            const int gy = y0 + 2 * wv + orow, gxx = x0 + px;
-            mrow[orow] = bld4(rmask, gy < H && gxx < W ? (uint32_t)(gy * W + gxx) * 4u : kOob);
-        }
         const int nxt = t + sw.step;
         fetch(nxt < sw.end ? nxt : t);  // the last tile reloads itself: no branch around the loads
 
@@ -1087,17 +852,23 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         for (int orow = 0; orow < 2; ++orow)
         {
             const int gy = y0 + 2 * wv + orow, gxx = x0 + px;
-            uint32_t d[8];
+            uint32_t d[8];  // d[2j + e] = channels 8j + 4h + 2e, +1
 #pragma unroll
             for (int q = 0; q < 8; ++q) d[q] = pack2(acc[orow][2 * q], acc[orow][2 * q + 1]);
-            const uint32_t m = mrow[orow] >> (4 * h);
+            // ReLU derivative from the staged X tile: keep a channel where X's bf16 value is nonzero
+            // (X >= 0; +-0 both count as zero)
+            const uint16_t *xp = &xs[((2 * wv + orow + 1) * LW + px + 1) * XS + 4 * h];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
+            for (int j = 0; j < 4; ++j)
             {
-                const int pos = 8 * (q >> 1) + 2 * (q & 1);
-                const uint32_t lo = (m >> pos) & 1u ? 0x0000FFFFu : 0u;
-                const uint32_t hi = (m >> (pos + 1)) & 1u ? 0xFFFF0000u : 0u;
-                d[q] &= lo | hi;
+                const u32x2 xw = *reinterpret_cast<const u32x2 *>(xp + 8 * j);
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                {
+                    const uint32_t lo = xw[e] & 0x00007FFFu ? 0x0000FFFFu : 0u;
+                    const uint32_t hi = xw[e] & 0x7FFF0000u ? 0xFFFF0000u : 0u;
+                    d[2 * j + e] &= lo | hi;
+                }
             }
             u32x4 v[2];
 #pragma unroll
@@ -1286,8 +1057,7 @@ __global__ __launch_bounds__(64) void pack_weights(PackBatch b)
 // ---------------------------------------------------------------------------------------------
 static int div_up(int a, int b) { return (a + b - 1) / b; }
 
-template <int IN, int CIN, int EPI>
-static hipError_t launch_conv_t(int n, const ConvFrame *fr, hipStream_t st)
+hipError_t launch_conv_l1(int n, const ConvFrame *fr, hipStream_t st)
 {
     ConvBatch b{};
     int tx[XDN_MAX_FRAMES], ty[XDN_MAX_FRAMES];
@@ -1304,21 +1074,8 @@ static hipError_t launch_conv_t(int n, const ConvFrame *fr, hipStream_t st)
     // persistent grid: two resident blocks per CU, a multiple of the 8 XCDs
     constexpr int kGrid = 256 * 2;
     const int grid = b.map.total < kGrid ? b.map.total : kGrid;
-    hipLaunchKernelGGL((conv3x3<IN, CIN, EPI>), dim3(grid), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(conv3x3_l1, dim3(grid), dim3(256), 0, st, b);
     return hipGetLastError();
-}
-
-hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st)
-{
-    switch (kind)
-    {
-    case CONV_L1_FWD: return launch_conv_t<CONV_IN_IMAGES, 16, CONV_EPI_RELU>(n, fr, st);
-    case CONV_MID_FWD: return launch_conv_t<CONV_IN_BF16, 32, CONV_EPI_RELU>(n, fr, st);
-    case CONV_L5_FWD: return launch_conv_t<CONV_IN_BF16, 32, CONV_EPI_F32_6>(n, fr, st);
-    case CONV_MID_DGRAD: return launch_conv_t<CONV_IN_BF16, 32, CONV_EPI_MASK>(n, fr, st);
-    case CONV_L5_DGRAD: return launch_conv_t<CONV_IN_F32_6, 16, CONV_EPI_MASK>(n, fr, st);
-    default: return hipErrorInvalidValue;
-    }
 }
 
 int wgrad_blocks(int H, int W)
@@ -1333,8 +1090,7 @@ int wgrad_blocks(int H, int W)
     return nb < 1 ? 1 : nb;
 }
 
-template <int XK, int CI, int DYK>
-static hipError_t launch_wgrad_t(int n, const WgFrame *fr, hipStream_t st)
+hipError_t launch_wgrad_l1(int n, const WgFrame *fr, hipStream_t st)
 {
     WgBatch b{};
     b.n = n;
@@ -1347,19 +1103,8 @@ static hipError_t launch_wgrad_t(int n, const WgFrame *fr, hipStream_t st)
     }
     b.blk_begin[n] = acc;
     if (acc == 0) return hipSuccess;
-    hipLaunchKernelGGL((conv3x3_wgrad<XK, CI, DYK>), dim3(acc), dim3(256), 0, st, b);
+    hipLaunchKernelGGL(conv3x3_wgrad_l1, dim3(acc), dim3(256), 0, st, b);
     return hipGetLastError();
-}
-
-hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st)
-{
-    switch (kind)
-    {
-    case CONV_L1_WGRAD: return launch_wgrad_t<CONV_IN_IMAGES, 16, CONV_IN_BF16>(n, fr, st);
-    case CONV_MID_WGRAD: return launch_wgrad_t<CONV_IN_BF16, 32, CONV_IN_BF16>(n, fr, st);
-    case CONV_L5_WGRAD: return launch_wgrad_t<CONV_IN_BF16, 32, CONV_IN_F32_6>(n, fr, st);
-    default: return hipErrorInvalidValue;
-    }
 }
 
 int bwd_blocks(int H, int W)

@@ -7,44 +7,32 @@
 
 namespace xdn {
 
-enum { CONV_IN_BF16 = 0, CONV_IN_IMAGES = 1, CONV_IN_F32_6 = 2 };
-enum { CONV_EPI_RELU = 0, CONV_EPI_MASK = 1, CONV_EPI_F32_6 = 2 };
-enum {
-    CONV_L1_FWD = 0,    // 5 FP32 3-ch images (15 ch) -> bf16 32 ch, bias + ReLU
-    CONV_MID_FWD = 1,   // bf16 32 -> bf16 32, bias + ReLU
-    CONV_L5_FWD = 2,    // bf16 32 -> FP32 6, bias
-    CONV_MID_DGRAD = 3, // bf16 32 (dY) -> bf16 32, ReLU mask of the layer input
-    CONV_L5_DGRAD = 4,  // FP32 6 (dY) -> bf16 32, ReLU mask
-};
-enum { CONV_L1_WGRAD = 0, CONV_MID_WGRAD = 1, CONV_L5_WGRAD = 2 };
-
 constexpr int WG_PART = 9 * 1024 + 32;  // one block's partial: [9][32][32] weights + [32] bias
 constexpr int WG_SPLITS = 32;
 
+// Layer 1 forward: the five FP32 3-channel images -> bf16 32 channels, bias + ReLU.
 struct ConvFrame {
-    const void *in[5];       // see the kernel kinds
-    void *out;
-    const uint32_t *mask;    // [H][W] ReLU bits of the layer input, bit c = channel c > 0 (mask epilogue)
-    uint32_t *mask_out;      // [H][W] ReLU bits of the output (bias + ReLU epilogue; may be null)
-    const void *wfrag;       // 16-byte B fragments [9][KC][64]
+    const float *in[5];      // FP32 [H][W][3] each: ols1, ols2, prev_out, albedo, normal
+    void *out;               // bf16 [H][W][32]
+    const void *wfrag;       // 16-byte B fragments [9][64]
     const float *bias;       // [32]
     int H, W;
 };
 
+// Layer 1 weight gradient.
 struct WgFrame {
-    const void *x[5];        // bf16 [H][W][32] activation, or the 5 FP32 input images
-    const void *dy;          // bf16 [H][W][32] or FP32 [H][W][6]
+    const float *x[5];       // the five FP32 input images
+    const void *dy;          // bf16 [H][W][32]: gradient of the layer's pre-activation
     float *part;             // [nblk][WG_PART]
     int H, W, nblk;
 };
 
 // Fused backward of one 32 -> 32 layer: its weight gradient and its data gradient into the layer
-// below, both from one staging of dY.
+// below, both from one staging of dY and x.
 struct BwdFrame {
     const void *dy;          // bf16 [H][W][32] (last layer: FP32 [H][W][6]): gradient of the layer's output
-    const void *x;           // bf16 [H][W][32]: the layer's input
-    const uint32_t *mask;    // [H][W] ReLU bits of x
-    void *dx;                // bf16 [H][W][32]: gradient of x, ReLU-masked (dY of the layer below)
+    const void *x;           // bf16 [H][W][32]: the layer's input (a ReLU output)
+    void *dx;                // bf16 [H][W][32]: gradient of x where x > 0, else 0 (dY of the layer below)
     const void *wfrag;       // data-gradient B fragments [9][KC][64] (16 B each; KC 2, last layer 1)
     float *part;             // [nblk][WG_PART] weight-gradient partials
     int H, W, nblk;
@@ -56,7 +44,6 @@ struct BwdFrame {
 struct FwdStreamFrame {
     const void *x1;          // bf16 [H][W][32]: output of layer 1
     void *x[3];              // bf16 [H][W][32]: outputs of layers 2, 3, 4
-    uint32_t *m[3];          // their ReLU bits
     float *out;              // FP32 [H][W][6]: network output (layer 5)
     const void *wfrag[4];    // forward B fragments [9][2][64] (16 B each) of layers 2..5
     const float *bias[4];    // [32] bias of layers 2..5 (zero padded)
@@ -70,8 +57,8 @@ struct PackJob {
     int ci, co;
 };
 
-hipError_t launch_conv(int kind, int n, const ConvFrame *fr, hipStream_t st);
-hipError_t launch_wgrad(int kind, int n, const WgFrame *fr, hipStream_t st);
+hipError_t launch_conv_l1(int n, const ConvFrame *fr, hipStream_t st);
+hipError_t launch_wgrad_l1(int n, const WgFrame *fr, hipStream_t st);
 // One reduction job: the nblk partials of one layer of one frame -> its flat gradient.
 // The partials hold an input-channel slice ci_off .. ci_off + ci_cnt - 1 (ci_cnt < 0: all ci),
 // [taps][32 ci][32 co] + [32 co] in WG_PART floats each, or when compact [taps][ci_cnt][co] +

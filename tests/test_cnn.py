@@ -93,23 +93,6 @@ def test_cnn5_forward_backward(dev, B, H, W):
     for l, (ew, eb) in enumerate(errs):
         assert ew <= 5e-3 and eb <= 5e-3, l
     check_kernels_on_own_operands(net, imgs, layers_cpu, B, H, W)
-    check_relu_bits(net, B, H, W)
-
-
-def check_relu_bits(net, B, H, W):
-    """The forward writes one 32-bit word per pixel and hidden layer, bit c = (bf16 channel c > 0);
-    the data gradient masks with it, so it must equal the stored activations' signs exactly."""
-    act = (B * H * W * 32 * 2 + 255) // 256 * 256
-    words = (B * H * W * 4 + 255) // 256 * 256
-    weights = torch.tensor([1 << c for c in range(32)], dtype=torch.int64)
-    for i in range(4):
-        raw = net.ws[6 * act + i * words:6 * act + i * words + B * H * W * 4].cpu().view(torch.int32)
-        got = raw.to(torch.int64) & 0xFFFFFFFF
-        a = _ws_act(net, i, B, H, W).permute(0, 2, 3, 1).reshape(-1, 32)
-        want = ((a > 0).to(torch.int64) * weights).sum(1)
-        bad = int((got != want).sum())
-        print(f"  relu bits of layer {i + 1}: {bad} of {got.numel()} words differ", flush=True)
-        assert bad == 0, i
 
 
 def _ws_act(net, i, B, H, W):
