@@ -631,10 +631,10 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
             const void *dy = l == 4 ? static_cast<const void *>(dparam + (size_t)b * N * 6)
                                     : static_cast<const void *>(ws.dY[cur] + o32);
             bf[b] = xdn::BwdFrame{dy, ws.X[l - 1] + o32, ws.dY[1 - cur] + o32, pk + kLay.dg_off[l],
-                                  ws.part[l] + (size_t)b * nb * xdn::WG_PART, height, width, nb};
+                                  ws.part[l] + (size_t)b * nb * ws.pstride[l], height, width, nb};
         }
         if (int e = hip_check(xdn::launch_bwd(batch, bf.data(), l == 4, S(stream)), "cnn5 backward")) return e;
-        jobs.push_back(xdn::RedJob{ws.part[l], ws.stage[l], grad + kLay.w_off[l], batch * nb, kCnn5[l].ci, kCnn5[l].co});
+        jobs.push_back(cnn5_red_job(ws, l, grad, batch * nb));
         cur = 1 - cur;
     }
     // layer 1's weight gradient: x = the five input images, dy = the gradient of its pre-activation
@@ -644,11 +644,11 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
     {
         const size_t o3 = (size_t)b * N * 3;
         wf[b] = xdn::WgFrame{{ols1 + o3, ols2 + o3, prev_out + o3, albedo + o3, normal + o3},
-                             ws.dY[cur] + (size_t)b * N * 32, ws.part[0] + (size_t)b * nb0 * xdn::WG_PART, height, width,
-                             nb0};
+                             ws.dY[cur] + (size_t)b * N * 32, ws.part[0] + (size_t)b * nb0 * ws.pstride[0], height,
+                             width, nb0};
     }
     if (int e = hip_check(xdn::launch_wgrad_l1(batch, wf.data(), S(stream)), "cnn5 wgrad")) return e;
-    jobs.push_back(xdn::RedJob{ws.part[0], ws.stage[0], grad + kLay.w_off[0], batch * nb0, kCnn5[0].ci, kCnn5[0].co});
+    jobs.push_back(cnn5_red_job(ws, 0, grad, batch * nb0));
     return hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), S(stream)), "cnn5 wgrad reduce");
 }
 

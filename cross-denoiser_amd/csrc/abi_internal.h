@@ -73,7 +73,8 @@ inline const Cnn5Layout kLay;
 struct Cnn5Ws {
     uint16_t *X[4], *dY[2];
     float *part[5], *stage[5];
-    int nblk[5];  // weight-gradient blocks (partials) per image and layer
+    int nblk[5];     // weight-gradient blocks (partials) per image and layer
+    int pstride[5];  // floats per partial (the first and last layers' partials are compact)
     size_t bytes;
     Cnn5Ws(void *base, int B, int H, int W)
     {
@@ -86,14 +87,29 @@ struct Cnn5Ws {
         {
             // the weight gradients of layers 2..5 come from the fused backward kernel
             nblk[l] = l >= 1 ? xdn::bwd_blocks(H, W) : xdn::wgrad_blocks(H, W);
+            pstride[l] = l == 0 ? xdn::WG_PART_L1 : (l == 4 ? xdn::WG_PART_L5 : xdn::WG_PART);
             part[l] = reinterpret_cast<float *>(p + o);
-            o += al256((size_t)B * nblk[l] * xdn::WG_PART * 4);
+            o += al256((size_t)B * nblk[l] * pstride[l] * 4);
             stage[l] = reinterpret_cast<float *>(p + o);
-            o += al256((size_t)xdn::WG_SPLITS * xdn::WG_PART * 4);
+            o += al256((size_t)xdn::WG_SPLITS * pstride[l] * 4);
         }
         bytes = o;
     }
 };
+
+// Reduction of layer l's nblk_total partials (the B images' of one workspace, or one slot's) into
+// its flat gradient at grad + the layer's weight offset.
+inline xdn::RedJob cnn5_red_job(const Cnn5Ws &ws, int l, float *grad, int nblk_total)
+{
+    xdn::RedJob r{ws.part[l], ws.stage[l], grad + kLay.w_off[l], nblk_total, kCnn5[l].ci, kCnn5[l].co};
+    if (l == 0 || l == 4)
+    {
+        r.compact = 1;
+        r.ci_cnt = kCnn5[l].ci;
+        r.pstride = ws.pstride[l];
+    }
+    return r;
+}
 
 // Layers 2..5 of a cnn5 forward for the image at pixel offset `po` of the workspace's buffers.
 inline xdn::FwdStreamFrame cnn5_stream_frame(const Cnn5Ws &ws, const char *packed, size_t po, float *out, int H, int W)

@@ -317,8 +317,7 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
                 bf[i] = xdn::BwdFrame{dy, cw[i].X[l - 1], cw[i].dY[1 - cur],
                                       static_cast<const char *>(s.packed) + kLay.dg_off[l], cw[i].part[l],
                                       s.height, s.width, cw[i].nblk[l]};
-                jobs.push_back(xdn::RedJob{cw[i].part[l], cw[i].stage[l], s.grad + kLay.w_off[l], cw[i].nblk[l],
-                                           kCnn5[l].ci, kCnn5[l].co});
+                jobs.push_back(cnn5_red_job(cw[i], l, s.grad, cw[i].nblk[l]));
             }
             if (int e = hip_check(xdn::launch_bwd(n, bf.data(), l == 4, st), "cnn backward")) return e;
             cur = 1 - cur;
@@ -329,8 +328,7 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
             const xdn_slot &s = slots[i];
             wf[i] = xdn::WgFrame{{s.ols1, s.ols2, sc[i].prev_out, s.albedo, s.normal}, cw[i].dY[cur], cw[i].part[0],
                                  s.height, s.width, cw[i].nblk[0]};
-            jobs.push_back(xdn::RedJob{cw[i].part[0], cw[i].stage[0], s.grad + kLay.w_off[0], cw[i].nblk[0],
-                                       kCnn5[0].ci, kCnn5[0].co});
+            jobs.push_back(cnn5_red_job(cw[i], 0, s.grad, cw[i].nblk[0]));
         }
         if (int e = hip_check(xdn::launch_wgrad_l1(n, wf.data(), st), "cnn wgrad")) return e;
         if (int e = hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), st), "cnn wgrad reduce")) return e;

@@ -578,8 +578,9 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad_l1(WgBatch b)
                 }
         }
     }
-    // partials: [t][ci][co] then [co]; the four waves' sums are added in LDS one tap at a time
-    float *out = F.part + (size_t)li * WG_PART;
+    // compact partial: [t][15 ci][32 co] then [32 co]; the four waves' sums are added in LDS one tap
+    // at a time
+    float *out = F.part + (size_t)li * WG_PART_L1;
     float *red = reinterpret_cast<float *>(lds);  // [4 waves][1024]
 #pragma unroll
     for (int t = 0; t < 9; ++t)
@@ -591,9 +592,9 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad_l1(WgBatch b)
 #pragma unroll
             for (int i = 0; i < 4; ++i) red[wv * 1024 + (4 * kg + i) * 32 + 16 * n + (lane & 15)] = acc16[t][n][i];
         __syncthreads();
-        // input channels 16..31 are padding (zero)
-        for (int e = tid; e < 1024; e += 256)
-            out[t * 1024 + e] = (e >> 5) < 16 ? (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]) : 0.f;
+        // input channel 15 is padding
+        for (int e = tid; e < 15 * 32; e += 256)
+            out[t * 15 * 32 + e] = (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]);
     }
 #pragma unroll
     for (int n = 0; n < 2; ++n)
@@ -604,7 +605,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad_l1(WgBatch b)
         if (lane < 16) dbs[wv][16 * n + lane] = v;
     }
     __syncthreads();
-    if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
+    if (tid < 32) out[9 * 15 * 32 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -889,8 +890,10 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         if (nxt < sw.end) commit();
     }
 
-    // partial: [t][ci][co] then [co]; tap t's four row-group waves are added in LDS
-    float *out = F.part + (size_t)li * WG_PART;
+    // partial: [t][ci][co] then [co] (L5: compact, [t][32 ci][6 co] then [6]); tap t's four row-group
+    // waves are added in LDS
+    float *out = F.part + (size_t)li * (L5 ? WG_PART_L5 : WG_PART);
+    constexpr int CO = L5 ? 6 : 32;
     float *red = reinterpret_cast<float *>(dys);  // [4 row groups][1024]
 #pragma unroll
     for (int t = 0; t < 9; ++t)
@@ -915,9 +918,11 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
                         red[rg * 1024 + (16 * m + 4 * kg + i) * 32 + (lane & 15)] = wacc16[j][m][i];
         }
         __syncthreads();
-        // L5: output channels 16..31 are padding (zero)
-        for (int e = tid; e < 1024; e += NT)
-            out[t * 1024 + e] = !L5 || (e & 31) < 16 ? (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]) : 0.f;
+        for (int e = tid; e < 32 * CO; e += NT)
+        {
+            const int r = L5 ? (e / CO) * 32 + e % CO : e;  // [ci][32] slot of entry (ci, co)
+            out[t * 32 * CO + e] = (red[r] + red[1024 + r]) + (red[2048 + r] + red[3072 + r]);
+        }
     }
     if constexpr (!L5)
     {
@@ -931,7 +936,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         if (tg == 0 && lane < 32) dbs[rg][lane] = lane < 16 ? db : 0.f;
     }
     __syncthreads();
-    if (tid < 32) out[9 * 1024 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
+    if (tid < CO) out[9 * 32 * CO + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
 }
 
 // Two-stage deterministic reduction of the per-block partials, batched over jobs (a job is one

@@ -23,7 +23,7 @@ struct ConvFrame {
 struct WgFrame {
     const float *x[5];       // the five FP32 input images
     const void *dy;          // bf16 [H][W][32]: gradient of the layer's pre-activation
-    float *part;             // [nblk][WG_PART]
+    float *part;             // [nblk][WG_PART_L1], compact
     int H, W, nblk;
 };
 
@@ -34,7 +34,7 @@ struct BwdFrame {
     const void *x;           // bf16 [H][W][32]: the layer's input (a ReLU output)
     void *dx;                // bf16 [H][W][32]: gradient of x where x > 0, else 0 (dY of the layer below)
     const void *wfrag;       // data-gradient B fragments [9][KC][64] (16 B each; KC 2, last layer 1)
-    float *part;             // [nblk][WG_PART] weight-gradient partials
+    float *part;             // [nblk][WG_PART] weight-gradient partials (last layer: WG_PART_L5, compact)
     int H, W, nblk;
 };
 
@@ -74,6 +74,9 @@ struct RedJob {
 };
 // floats of one compact partial: [taps][cc][co] + [co]
 constexpr int compact_part(int taps, int cc, int co) { return taps * cc * co + co; }
+// the first layer's (15 -> 32) and the last layer's (32 -> 6) partials are compact
+constexpr int WG_PART_L1 = compact_part(9, 15, 32);
+constexpr int WG_PART_L5 = compact_part(9, 32, 6);
 constexpr int WG_MAX_JOBS = 48;  // per launch; longer job lists are split
 hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st);
 hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st);
