@@ -116,17 +116,13 @@ __device__ __forceinline__ void sf_center_bands(SfCenter &C, const float *band, 
 // Epilogue of the fused training kernel for one pixel; returns the pixel's loss sum.
 // Follows aggregation_loss (scripts/main.py:214-248), its analytic gradient, the band-gradient
 // contraction (regression.cu.cc:58-80) and the band/alpha transform (main.py:111-113).
-__device__ __forceinline__ float train_epilogue(const TrainFrame &F, size_t c, const SfAcc &acc, float inv_count,
+__device__ __forceinline__ float train_epilogue(const TrainFrame &F, uint32_t c, const SfAcc &acc, float inv_count,
                                                 float3 lab2, float3 lab1)
 {
-    const float *pr = F.param + 6 * c;
-    float x[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) x[k] = pr[k];
-    float band[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) band[k] = 1.f / (x[k] * x[k] + 1e-4f);
-    const float alpha = 1.f / (1.f + __expf(-x[5]));
+    // the five bandwidth outputs are read again at the end, where the band transform needs them:
+    // kept live from here they push the epilogue past the registers of five resident blocks
+    const float *pr = F.param + 6 * (size_t)c;
+    const float alpha = 1.f / (1.f + __expf(-pr[5]));
     const float d1 = acc.sA + 1e-4f, d2 = acc.sB + 1e-4f;
     const float i1 = 1.f / d1, i2 = 1.f / d2, i12 = 1.f / (acc.sA + acc.sB + 1e-4f);
     const float op = F.opacity[c] > 0.f ? 1.f : 0.f;
@@ -179,9 +175,16 @@ __device__ __forceinline__ float train_epilogue(const TrainFrame &F, size_t c, c
         oB[k] = -(gB[0] * acc.MB[k].x + gB[1] * acc.MB[k].y + gB[2] * acc.MB[k].z + hB * acc.MB[k].w);
     }
     const float gb[5] = {oA[0], oB[0], oA[1] + oB[1], oA[2] + oB[2], oA[3] + oB[3]};
-    float *dp = F.dparam + 6 * c;
+    float x[5];  // all five loaded before the first store (the stores may alias them as far as the
+#pragma unroll  // compiler knows)
+    for (int k = 0; k < 5; ++k) x[k] = pr[k];
+    float *dp = F.dparam + 6 * (size_t)c;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) dp[k] = gb[k] * (-2.f * x[k] * band[k] * band[k]);
+    for (int k = 0; k < 5; ++k)
+    {
+        const float band = 1.f / (x[k] * x[k] + 1e-4f);
+        dp[k] = gb[k] * (-2.f * x[k] * band * band);
+    }
     dp[5] = s * ga * alpha * (1.f - alpha);
     return lsum;
 }
@@ -213,12 +216,13 @@ __device__ __forceinline__ void sf_tile_body(const Batch &b, float *red)
     }
     __syncthreads();
     const int lx = threadIdx.x & (SF_T - 1), ly = threadIdx.x / SF_T;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int cx = x0 + lx, cy = y0 + ly;
     const bool live = cx < W && cy < H;
     float lsum = 0.f;
     if (live)
     {
-        const size_t c = (size_t)cy * W + cx;
+        const uint32_t c = (uint32_t)cy * (uint32_t)W + (uint32_t)cx;  // a frame has < 2^32 pixels
         const int q0 = (ly + HALF) * E + lx + HALF;
         SfCenter C;
         {
@@ -230,7 +234,7 @@ __device__ __forceinline__ void sf_tile_body(const Batch &b, float *red)
         }
         if constexpr (MODE == SF_TRAIN)
         {
-            const float *pr = F.param + 6 * c;
+            const float *pr = F.param + 6 * (size_t)c;
             float bnd[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) bnd[k] = 1.f / (pr[k] * pr[k] + 1e-4f);
@@ -273,7 +277,7 @@ __device__ __forceinline__ void sf_tile_body(const Batch &b, float *red)
         }
         else if constexpr (MODE == SF_GRAD)
         {
-            float *o = F.gband + 5 * c;
+            float *o = F.gband + 5 * (size_t)c;
 #pragma unroll
             for (int k = 0; k < 5; ++k) o[k] = -acc.g[k];
         }
@@ -285,11 +289,13 @@ __device__ __forceinline__ void sf_tile_body(const Batch &b, float *red)
     }
     if constexpr (MODE == SF_TRAIN)
     {
-        // deterministic per-tile loss partial: wave reduction then 4 waves through LDS
+        // deterministic per-tile loss partial: wave reduction then 4 waves through LDS (wave and lane
+        // from a wave-uniform value and the lane count, so no thread index stays live across the taps)
         for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lsum;
+        const bool lane0 = __lane_id() == 0;
+        if (lane0) red[wave] = lsum;
         __syncthreads();
-        if (threadIdx.x == 0) F.loss_part[tile - b.map.begin[L.f]] = (red[0] + red[1]) + (red[2] + red[3]);
+        if (wave == 0 && lane0) F.loss_part[tile - b.map.begin[L.f]] = (red[0] + red[1]) + (red[2] + red[3]);
     }
 }
 
@@ -299,8 +305,8 @@ __global__ __launch_bounds__(256) void sf_tile(SfBatch b)
     sf_tile_body<HALF, MODE>(b, nullptr);
 }
 
-// Five resident blocks per CU (32.5 KB of LDS each): 96 VGPRs instead of 106, the few values the
-// compiler then spills are live only around the tap loop (center set-up, epilogue), not in it.
+// Five resident blocks per CU (32.5 KB of LDS each): 96 VGPRs instead of 106. The epilogue is kept
+// lean for it (see train_epilogue); one 8-byte value is still spilled there, none in the tap loop.
 template <int HALF>
 __global__ __launch_bounds__(256, 5) void sf_train_tile(TrBatch b)
 {
