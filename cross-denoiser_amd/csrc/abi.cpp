@@ -230,6 +230,7 @@ static int sf_common_check(int B, int H, int W, int win)
     if (win < 1) return fail(XDN_EINVAL, "win_size must be >= 1");
     if (win / 2 < 1) return fail(XDN_EINVAL, "win_size must be >= 2");
     if (H <= win / 2 || W <= win / 2) return fail(XDN_EINVAL, "Image height and width must exceed win_size/2");
+    if ((size_t)H * W >= kFilterMaxPx) return fail(XDN_EINVAL, "frames of fewer than 2^32 pixels (got %d x %d)", H, W);
     return XDN_OK;
 }
 
@@ -565,6 +566,7 @@ static int cnn5_check(int B, int H, int W, size_t ws_bytes)
 {
     if (int e = check_shape(B, H, W)) return e;
     if (B > XDN_MAX_FRAMES) return fail(XDN_EINVAL, "cnn5: batch must be <= %d", XDN_MAX_FRAMES);
+    if ((size_t)H * W >= kCnn5MaxPx) return fail(XDN_EINVAL, "cnn5: frames of fewer than 2^25 pixels (got %d x %d)", H, W);
     if (ws_bytes < xdn_cnn5_workspace_size(B, H, W)) return fail(XDN_EINVAL, "cnn5: workspace too small");
     return XDN_OK;
 }

@@ -39,6 +39,11 @@ constexpr LayerSpec kCnn5[5] = {{15, 32}, {32, 32}, {32, 32}, {32, 32}, {32, 6}}
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// The cnn5 kernels address their 64 B/px activations with 32-bit buffer offsets, so a frame has
+// fewer than 2^25 pixels (7680x4320 fits); the filter kernels index pixels with 32 bits.
+constexpr size_t kCnn5MaxPx = (size_t)1 << 25;
+constexpr size_t kFilterMaxPx = (size_t)1 << 32;
+
 // FP32 parameter offsets (floats) and packed-fragment byte offsets of cnn5.
 struct Cnn5Layout {
     size_t w_off[5], b_off[5], count;

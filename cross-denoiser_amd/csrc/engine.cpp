@@ -73,6 +73,10 @@ int check_slot(const xdn_slot &s)
     if (s.height % kSparsity || s.width % kSparsity)
         return fail(XDN_EINVAL, "Image height and width must be divisible by sparsity");
     if (s.network != XDN_NET_CNN5 && s.network != XDN_NET_UNET) return fail(XDN_EINVAL, "unknown network %d", s.network);
+    if ((size_t)s.height * s.width >= kFilterMaxPx)
+        return fail(XDN_EINVAL, "slots hold frames of fewer than 2^32 pixels (got %d x %d)", s.height, s.width);
+    if (s.network == XDN_NET_CNN5 && (size_t)s.height * s.width >= kCnn5MaxPx)
+        return fail(XDN_EINVAL, "cnn5 slots hold frames of fewer than 2^25 pixels (got %d x %d)", s.height, s.width);
     return XDN_OK;
 }
 

@@ -17,6 +17,9 @@
  *     writing any buffer, so its pointers may be NULL (the data pointers of zero-element tensors);
  *     xdn_cnn5_backward and xdn_unet_backward, whose gradient has no batch dimension, write zeros
  *     to grad.
+ *   - Frame size: the cnn5 entry points and cnn5 slots take frames of fewer than 2^25 pixels (the
+ *     network kernels address their 64 B/px activations with 32-bit offsets; 7680x4320 fits), the
+ *     spatial-filter entry points and every slot fewer than 2^32; larger ones are XDN_EINVAL.
  */
 #ifndef XDN_H
 #define XDN_H

@@ -85,6 +85,25 @@ def test_empty_batch_is_a_no_op_without_gpu():
     assert lib.xdn_unet_backward(*([N] * 8 + [N, 0, 0, H, W, N])) == -1
 
 
+def test_oversize_frames_are_rejected_without_gpu():
+    """Frames past the kernels' 32-bit offsets are refused before anything is launched: cnn5 below
+    2^25 pixels (its 64 B/px activations), the filter below 2^32 pixels."""
+    from xdn import pipeline
+    from xdn._lib import lib
+    N = None
+    assert lib.xdn_cnn5_forward(*([N] * 7 + [N, 1 << 40, 1, 8192, 4096, N])) == -1
+    assert "2^25" in lib.xdn_last_error().decode()
+    assert lib.xdn_spatial_filter(*([N] * 9 + [1, 65536, 65536, 11, N])) == -1
+    assert "2^32" in lib.xdn_last_error().decode()
+    s = pipeline.XdnSlot()
+    s.height, s.width = 8192, 4096
+    assert lib.xdn_denoise_step(ctypes.addressof(s), 1, 1e-3, None) == -1
+    assert "2^25" in lib.xdn_last_error().decode()
+    s.height, s.width = 4320, 7680  # 8K UHD fits; the next failure is the missing buffers
+    assert lib.xdn_denoise_step(ctypes.addressof(s), 1, 1e-3, None) == -1
+    assert "null" in lib.xdn_last_error().decode()
+
+
 def test_make_slot_parity_selects_history_buffers():
     """A slot for step t takes its outputs from history buffer t % 2 and its history from the other
     (host logic only: CPU tensors stand in for device buffers)."""
