@@ -344,6 +344,22 @@ __device__ __forceinline__ void recon_term(const float4 *cb, float3 dwv, float i
     acc.z += w * pz;
 }
 
+// The same prediction with the center's terms folded into the staged record: with
+// K0 = beta0 - beta2 . a_c - beta3 . n_c (per channel), beta . delta = K0 + beta1 d1 + beta2 a_p +
+// beta3 . n_p, so a (pixel, center) pair needs no albedo or normal difference. bt: K0 in place of
+// beta0, then beta1..beta5 as in recon_term; ca, cn: the pixel's albedo and normal.
+__device__ __forceinline__ void recon_term_k0(const float *bt, float3 dwv, float inv, float3 ca, float3 cn, float w,
+                                              float3 &acc)
+{
+    const float d1x = dwv.x * inv, d1y = dwv.y * inv, d1z = dwv.z * inv;
+    const float px = bt[0] + bt[3] * d1x + bt[6] * ca.x + bt[9] * cn.x + bt[12] * cn.y + bt[15] * cn.z;
+    const float py = bt[1] + bt[4] * d1y + bt[7] * ca.y + bt[10] * cn.x + bt[13] * cn.y + bt[16] * cn.z;
+    const float pz = bt[2] + bt[5] * d1z + bt[8] * ca.z + bt[11] * cn.x + bt[14] * cn.y + bt[17] * cn.z;
+    acc.x += w * px;
+    acc.y += w * py;
+    acc.z += w * pz;
+}
+
 __global__ __launch_bounds__(256) void recon_tile(RecBatch b)
 {
     // staged center record (13 float4): beta A (18) + beta B (18) = 9 float4,
@@ -380,6 +396,20 @@ __global__ __launch_bounds__(256) void recon_tile(RecBatch b)
         d[12] = make_float4(nr.x, nr.y, nr.z, sqrtf(v2));
     }
     __syncthreads();
+    // fold each center's albedo and normal into its constant term, per half and channel
+    for (int i = threadIdx.x; i < ncx * ncy * 2; i += 256)
+    {
+        const int cidx = i >> 1, hb = i & 1;
+        const int ry = cidx / ncx, rx = cidx - ry * ncx;
+        float4 *d = cs[ry * RC_C + rx];
+        float *bt = reinterpret_cast<float *>(d) + 18 * hb;
+        const float4 A = d[11], N = d[12];
+        const float av[3] = {A.x, A.y, A.z};
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+            bt[ch] = bt[ch] - bt[6 + ch] * av[ch] - (bt[9 + ch] * N.x + bt[12 + ch] * N.y + bt[15 + ch] * N.z);
+    }
+    __syncthreads();
     const int cx = x0 + (threadIdx.x & 15), cy = y0 + (threadIdx.x >> 4);
     if (cx >= W || cy >= H) return;
     const size_t c = (size_t)cy * W + cx;
@@ -395,15 +425,14 @@ __global__ __launch_bounds__(256) void recon_tile(RecBatch b)
         for (int ix = sx; ix <= ex; ++ix)
         {
             const float4 *rec = cs[(iy - rlo) * RC_C + (ix - clo)];
-            const float4 P1 = rec[9], P2 = rec[10], A = rec[11], N = rec[12];
-            const float3 d2 = make_float3(ca.x - A.x, ca.y - A.y, ca.z - A.z);
-            const float3 n = make_float3(cn.x - N.x, cn.y - N.y, cn.z - N.z);
+            const float4 P1 = rec[9], P2 = rec[10];
+            const float std1 = rec[11].w, std2 = rec[12].w;
             const float3 dwA = make_float3(c1.x - P1.x, c1.y - P1.y, c1.z - P1.z);
             const float3 dwB = make_float3(c2.x - P2.x, c2.y - P2.y, c2.z - P2.z);
             const float wA = reg_weight(dwA, v1e, P1.w), wB = reg_weight(dwB, v2e, P2.w);
-            recon_term(rec, dwA, fast_rcp(s1e + A.w), d2, n, wA, accA);
-            recon_term(reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(rec) + 18), dwB,
-                       fast_rcp(s2e + N.w), d2, n, wB, accB);
+            const float *bt = reinterpret_cast<const float *>(rec);
+            recon_term_k0(bt, dwA, fast_rcp(s1e + std1), ca, cn, wA, accA);
+            recon_term_k0(bt + 18, dwB, fast_rcp(s2e + std2), ca, cn, wB, accB);
             sumA += wA;
             sumB += wB;
         }
