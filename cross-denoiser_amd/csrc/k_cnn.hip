@@ -340,7 +340,9 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
     bst16(zero4, rout, kOob + 96u); bst16(zero4, rout, kOob + 128u); bst8(u32x2{0u, 0u}, rout, kOob + 160u);
     __syncthreads();
 
-    const uint16_t *in = &ring[wv][0][0];
+    // the lane's B-fragment address in slot 0 of its input ring: a slot adds a wave-uniform offset
+    // and the column shift / channel chunk an immediate one
+    const uint16_t *in = &ring[wv][0][0] + px * FS_PS + 8 * h;
     const int gx = xw + px;
     const bool cin = gx >= 0 && gx < W;
     const bool ccol = cin && px >= 3 && px < 3 + FS_VALID;  // a column this strip stores
@@ -348,6 +350,9 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
         put(ys - 1 + s, pf);                // layer 1's row ys - 1 + s, loaded two steps ago
         pf = bld16(rx1, xoff(ys + 1 + s));  // ... and the one two steps ahead
         const int r = ys - 3 + s - 2 * wv;
+        const uint16_t *rows[3];
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) rows[dy] = in + ((r - 1 + dy) & 3) * (FS_PX * FS_PS);
         f32x16 acc;
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc[k] = 0.f;
@@ -358,8 +363,7 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy)
                 {
-                    const u32x4 bfr = *reinterpret_cast<const u32x4 *>(
-                        &in[(((r - 1 + dy) & 3) * FS_PX + px + dx) * FS_PS + kc * 16 + 8 * h]);
+                    const u32x4 bfr = *reinterpret_cast<const u32x4 *>(rows[dy] + dx * FS_PS + kc * 16);
                     acc = mfma(wreg[(dy * 3 + dx) * 2 + kc], bfr, acc);
                 }
         const bool rin = r >= 0 && r < H;
@@ -367,10 +371,12 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
         const uint32_t p = (uint32_t)(r * W + gx);
         // Every wave issues the same four stores (the unused ones at kOob), so the number of memory
         // accesses behind a prefetch is the same on every path and its wait excludes the stores.
-        const bool live = rin && cin;  // zero outside the image: the next layer's padding
+        // Outside the image the row is zero (the next layer's padding): ReLU and that mask are one
+        // median, med3(x, 0, +inf) = max(x, 0) and med3(x, 0, 0) = 0.
+        const float lim = rin && cin ? __builtin_inff() : 0.f;
         float v[16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = live ? fmaxf(acc[k] + bias[k], 0.f) : 0.f;
+        for (int k = 0; k < 16; ++k) v[k] = __builtin_amdgcn_fmed3f(acc[k] + bias[k], 0.f, lim);
         uint32_t d[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) d[q] = pack2(v[2 * q], v[2 * q + 1]);
