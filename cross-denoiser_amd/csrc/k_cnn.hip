@@ -40,8 +40,6 @@ __device__ __forceinline__ uint32_t pack2(float a, float b)
 {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
-__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
-__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
 __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
 {
@@ -443,6 +441,36 @@ __device__ __forceinline__ uint2 ld_tr(const uint16_t *p)
 {
     return __builtin_bit_cast(uint2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)p));
 }
+// The same on an LDS-address-space pointer: offsets added to it stay 32-bit and constant ones go
+// into the instruction's offset field.
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+__device__ __forceinline__ uint2 ld_tr(lds_u16 *p)
+{
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)p));
+}
+
+// acc + the two bf16 halves of w, in FP32 (v_dot2_f32_bf16 against (1, 1)).
+__device__ __forceinline__ float add_bf2(uint32_t w, float acc)
+{
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), bf16x2{(__bf16)1.f, (__bf16)1.f}, acc, false);
+}
+__device__ __forceinline__ float add_bf8(u32x4 w, float acc)
+{
+    return add_bf2(w.w, add_bf2(w.z, add_bf2(w.y, add_bf2(w.x, acc))));
+}
+
+// 0xFFFF in each 16-bit half of w that holds a nonzero bf16 (+0 and -0 count as zero), else 0:
+// the half's magnitude plus 0x7FFF sets its bit 15 exactly when the magnitude is nonzero, and an
+// arithmetic shift by 15 spreads that bit (packed 16-bit integer ops).
+__device__ __forceinline__ uint32_t nz_mask(uint32_t w)
+{
+    typedef __attribute__((ext_vector_type(2))) unsigned short u16x2;
+    typedef __attribute__((ext_vector_type(2))) short s16x2;
+    const u16x2 t = __builtin_bit_cast(u16x2, w & 0x7FFF7FFFu) + u16x2{0x7FFF, 0x7FFF};
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(s16x2, t) >> s16x2{15, 15});
+}
+
+template <int N> struct IntC { static constexpr int value = N; };
 
 // Weight gradient of layer 1, dW[t][ci][co] = sum_p X[p + shift_t][ci] dY[p][co] over 8x32-px
 // strips, X = the five FP32 3-channel input images (15 channels, channel 15 zero), dY = the bf16
@@ -559,23 +587,23 @@ __global__ __launch_bounds__(256, 2) void conv3x3_wgrad_l1(WgBatch b)
         for (int r = 0; r < 2; ++r)
         {
             const int row = 2 * wv + r;
+            lds_u16 *dp = (lds_u16 *)ds + (row * WS_W + 8 * kg + jq) * DS + 4 * qq;
+            lds_u16 *xp = (lds_u16 *)xs + (row * WX_C + 8 * kg + jq) * XS + 4 * qq;
             u32x4 Bn[2];
 #pragma unroll
             for (int n = 0; n < 2; ++n)
             {
-                const uint16_t *dp = &ds[(row * WS_W + 8 * kg + jq) * DS + 16 * n + 4 * qq];
-                const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * DS);
+                const uint2 b0 = ld_tr(dp + 16 * n), b1 = ld_tr(dp + 16 * n + 4 * DS);
                 Bn[n] = u32x4{b0.x, b0.y, b1.x, b1.y};
-                db16[n] += (bf_lo(Bn[n].x) + bf_hi(Bn[n].x)) + (bf_lo(Bn[n].y) + bf_hi(Bn[n].y)) +
-                           (bf_lo(Bn[n].z) + bf_hi(Bn[n].z)) + (bf_lo(Bn[n].w) + bf_hi(Bn[n].w));
+                db16[n] = add_bf8(Bn[n], db16[n]);
             }
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
                 for (int sh = 0; sh < 3; ++sh)
                 {
-                    const uint16_t *xp = &xs[((row + dy) * WX_C + 8 * kg + sh + jq) * XS + 4 * qq];
-                    const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
+                    const int o = (dy * WX_C + sh) * XS;
+                    const uint2 a0 = ld_tr(xp + o), a1 = ld_tr(xp + o + 4 * XS);
                     const bf16x8 A = __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y});
 #pragma unroll
                     for (int n = 0; n < 2; ++n)
@@ -770,62 +798,70 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
 
         if constexpr (!L5)
         {
-            // ---- weight gradient: taps 5 tg .. over rows 4 rg + 2 pr + h, 8-px K chunks c ----
+            // ---- weight gradient: taps 5 TG .. over rows 4 rg + 2 pr + h, 8-px K chunks c ----
+            // (the tap group a compile-time constant on each side of a wave-uniform branch, so every
+            // tap's shift is an immediate address offset)
+            auto wgrad = [&](auto tgc) {
+                constexpr int TG = decltype(tgc)::value, NJ = TG == 0 ? 5 : 4;
 #pragma unroll 1
-            for (int pr = 0; pr < 2; ++pr)
-            {
-                const int rr = 4 * rg + 2 * pr + h;
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
+                for (int pr = 0; pr < 2; ++pr)
                 {
-                    const uint16_t *dp = &dys[((rr + 1) * LW + 8 * c + jq + 1) * PS + cb];
-                    const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * PS);
-                    const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
-                    if (tg == 0)
-                        db += (bf_lo(B.x) + bf_hi(B.x)) + (bf_lo(B.y) + bf_hi(B.y)) + (bf_lo(B.z) + bf_hi(B.z)) +
-                              (bf_lo(B.w) + bf_hi(B.w));
+                    const int rr = 4 * rg + 2 * pr + h;
+                    lds_u16 *dp = (lds_u16 *)dys + ((rr + 1) * LW + jq + 1) * PS + cb;
+                    lds_u16 *xp = (lds_u16 *)xs + (rr * LW + jq) * XS + cb;
 #pragma unroll
-                    for (int j = 0; j < 5; ++j)
+                    for (int c = 0; c < 4; ++c)
                     {
-                        if (tg == 1 && j == 4) continue;  // taps 5..8 only
-                        const int tap = 5 * tg + j, dy = tap / 3, sh = tap - 3 * dy;
-                        const uint16_t *xp = &xs[((rr + dy) * LW + 8 * c + sh + jq) * XS + cb];
-                        const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
-                        const u32x4 A = {a0.x, a0.y, a1.x, a1.y};
-                        wacc[j] = mfma(A, B, wacc[j]);
+                        const uint2 b0 = ld_tr(dp + 8 * c * PS), b1 = ld_tr(dp + (8 * c + 4) * PS);
+                        const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
+                        if constexpr (TG == 0) db = add_bf8(B, db);
+#pragma unroll
+                        for (int j = 0; j < NJ; ++j)
+                        {
+                            const int tap = 5 * TG + j, dy = tap / 3, sh = tap - 3 * dy;
+                            const int o = (dy * LW + 8 * c + sh) * XS;
+                            const uint2 a0 = ld_tr(xp + o), a1 = ld_tr(xp + o + 4 * XS);
+                            const u32x4 A = {a0.x, a0.y, a1.x, a1.y};
+                            wacc[j] = mfma(A, B, wacc[j]);
+                        }
                     }
                 }
-            }
+            };
+            if (tg == 0) wgrad(IntC<0>{});
+            else wgrad(IntC<1>{});
         }
         else
         {
             // ---- weight gradient on 16x16x32: tile rows 4 rg + r, K = the row's 32 px ----
+            auto wgrad = [&](auto tgc) {
+                constexpr int TG = decltype(tgc)::value, NJ = TG == 0 ? 5 : 4;
 #pragma unroll 1
-            for (int r = 0; r < 4; ++r)
-            {
-                const int row = 4 * rg + r;
-                const uint16_t *dp = &dys[((row + 1) * LW + 8 * kg + jq + 1) * PS + 4 * qq];
-                const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * PS);
-                const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
-                if (tg == 0)
-                    db += (bf_lo(B.x) + bf_hi(B.x)) + (bf_lo(B.y) + bf_hi(B.y)) + (bf_lo(B.z) + bf_hi(B.z)) +
-                          (bf_lo(B.w) + bf_hi(B.w));
-#pragma unroll
-                for (int j = 0; j < 5; ++j)
+                for (int r = 0; r < 4; ++r)
                 {
-                    if (tg == 1 && j == 4) continue;  // taps 5..8 only
-                    const int tap = 5 * tg + j, dy = tap / 3, sh = tap - 3 * dy;
+                    const int row = 4 * rg + r;
+                    lds_u16 *dp = (lds_u16 *)dys + ((row + 1) * LW + 8 * kg + jq + 1) * PS + 4 * qq;
+                    lds_u16 *xp = (lds_u16 *)xs + (row * LW + 8 * kg + jq) * XS + 4 * qq;
+                    const uint2 b0 = ld_tr(dp), b1 = ld_tr(dp + 4 * PS);
+                    const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
+                    if constexpr (TG == 0) db = add_bf8(B, db);
 #pragma unroll
-                    for (int m = 0; m < 2; ++m)
+                    for (int j = 0; j < NJ; ++j)
                     {
-                        const uint16_t *xp = &xs[((row + dy) * LW + 8 * kg + sh + jq) * XS + 16 * m + 4 * qq];
-                        const uint2 a0 = ld_tr(xp), a1 = ld_tr(xp + 4 * XS);
-                        wacc16[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y}), __builtin_bit_cast(bf16x8, B),
-                            wacc16[j][m], 0, 0, 0);
+                        const int tap = 5 * TG + j, dy = tap / 3, sh = tap - 3 * dy;
+#pragma unroll
+                        for (int m = 0; m < 2; ++m)
+                        {
+                            const int o = (dy * LW + sh) * XS + 16 * m;
+                            const uint2 a0 = ld_tr(xp + o), a1 = ld_tr(xp + o + 4 * XS);
+                            wacc16[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y}),
+                                __builtin_bit_cast(bf16x8, B), wacc16[j][m], 0, 0, 0);
+                        }
                     }
                 }
-            }
+            };
+            if (tg == 0) wgrad(IntC<0>{});
+            else wgrad(IntC<1>{});
         }
 
         // ---- data gradient: output rows 2wv, 2wv+1 (the conv3x3 loop on the dY tile) ----
@@ -870,12 +906,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
             {
                 const u32x2 xw = *reinterpret_cast<const u32x2 *>(xp + 8 * j);
 #pragma unroll
-                for (int e = 0; e < 2; ++e)
-                {
-                    const uint32_t lo = xw[e] & 0x00007FFFu ? 0x0000FFFFu : 0u;
-                    const uint32_t hi = xw[e] & 0x7FFF0000u ? 0xFFFF0000u : 0u;
-                    d[2 * j + e] &= lo | hi;
-                }
+                for (int e = 0; e < 2; ++e) d[2 * j + e] &= nz_mask(xw[e]);
             }
             u32x4 v[2];
 #pragma unroll
