@@ -176,8 +176,11 @@ __global__ __launch_bounds__(256, 2) void conv3x3_l1(ConvBatch b)
     const int x0 = L.tx * CT_W, y0 = L.ty * CT_H;
     __syncthreads();  // tile `it` and its frame's fragments are in LDS
     const int nxt = it + (int)gridDim.x;
-    // the next tile's loads (the last iteration reloads its own tile: no branch around the loads)
+    // the next tile's loads (the last iteration reloads its own tile: no branch around the loads),
+    // kept ahead of the MFMAs: left to itself the scheduler sinks them behind the whole tile, and
+    // the commit at the end of the iteration then waits out their full latency
     fetch(nxt < total ? nxt : it);
+    __builtin_amdgcn_sched_barrier(0);
     float bias[16];  // bias of the lane's channels c(k) = 8(k/4) + 4h + k%4
 #pragma unroll
     for (int j = 0; j < 4; ++j)
