@@ -47,6 +47,15 @@ __device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
                                                    0, 0, 0);
 }
 
+// ReLU of two packed bf16 values: a negative bf16 (sign bit set) reads as a negative int16, so a
+// packed signed 16-bit max with 0 turns it into +0 and keeps every other value (one v_pk_max_i16
+// instead of two FP32 maxima before the conversion; rounding is monotonic, so the result is the same).
+__device__ __forceinline__ uint32_t relu_bf2(uint32_t w)
+{
+    typedef __attribute__((ext_vector_type(2))) short s16x2;
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, w), s16x2{0, 0}));
+}
+
 // Raw buffer accesses: a load outside [0, bytes) of its buffer returns zero and a store there is
 // dropped, so guarded accesses (image borders, partial tiles, optional outputs) need no branch. With
 // no branch around any vector memory access, the compiler knows exactly how many of them were issued
@@ -226,7 +235,7 @@ __global__ __launch_bounds__(256, 2) void conv3x3_l1(ConvBatch b)
         uint32_t d[8];  // d[2j + e] = channels 8j + 4h + 2e, +1
 #pragma unroll
         for (int q = 0; q < 8; ++q)
-            d[q] = pack2(fmaxf(acc[orow][2 * q] + bias[2 * q], 0.f), fmaxf(acc[orow][2 * q + 1] + bias[2 * q + 1], 0.f));
+            d[q] = relu_bf2(pack2(acc[orow][2 * q] + bias[2 * q], acc[orow][2 * q + 1] + bias[2 * q + 1]));
         // lane h gathers channels 16g + 8h .. 16g + 8h + 7 (g = 0, 1): the low half-wave keeps its
         // group 2g and takes the high half's; the high half keeps group 2g + 1 and takes the low's
         u32x4 v[2];
