@@ -715,9 +715,15 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
 
     for (int e = tid; e < 9 * KC * 64; e += NT) wl[e] = static_cast<const u32x4 *>(F.wfrag)[e];
 
+    // The rest of the kernel once per tap group, a compile-time constant on each side of a
+    // wave-uniform branch: every tap's shift is then an immediate LDS offset, and each group keeps
+    // its own weight-gradient accumulators across the tile loop (one shared loop would have to
+    // merge the two groups' registers after every tile). Both sides run the same barriers.
+    auto run = [&](auto tgc) {
+    constexpr int TG = decltype(tgc)::value;
     typedef __attribute__((ext_vector_type(4))) float f32x4;
-    f32x16 wacc[L5 ? 1 : 5];         // taps 5 tg + j (middle layers)
-    f32x4 wacc16[L5 ? 5 : 1][2];     // L5: taps 5 tg + j, input-channel tiles m
+    f32x16 wacc[L5 ? 1 : 5];         // taps 5 TG + j (middle layers)
+    f32x4 wacc16[L5 ? 5 : 1][2];     // L5: taps 5 TG + j, input-channel tiles m
 #pragma unroll
     for (int j = 0; j < (L5 ? 1 : 5); ++j)
 #pragma unroll
@@ -811,10 +817,8 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         if constexpr (!L5)
         {
             // ---- weight gradient: taps 5 TG .. over rows 4 rg + 2 pr + h, 8-px K chunks c ----
-            // (the tap group a compile-time constant on each side of a wave-uniform branch, so every
-            // tap's shift is an immediate address offset)
-            auto wgrad = [&](auto tgc) {
-                constexpr int TG = decltype(tgc)::value, NJ = TG == 0 ? 5 : 4;
+            {
+                constexpr int NJ = TG == 0 ? 5 : 4;
 #pragma unroll 1
                 for (int pr = 0; pr < 2; ++pr)
                 {
@@ -838,15 +842,13 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
                         }
                     }
                 }
-            };
-            if (tg == 0) wgrad(IntC<0>{});
-            else wgrad(IntC<1>{});
+            }
         }
         else
         {
             // ---- weight gradient on 16x16x32: tile rows 4 rg + r, K = the row's 32 px ----
-            auto wgrad = [&](auto tgc) {
-                constexpr int TG = decltype(tgc)::value, NJ = TG == 0 ? 5 : 4;
+            {
+                constexpr int NJ = TG == 0 ? 5 : 4;
 #pragma unroll 1
                 for (int r = 0; r < 4; ++r)
                 {
@@ -871,9 +873,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
                         }
                     }
                 }
-            };
-            if (tg == 0) wgrad(IntC<0>{});
-            else wgrad(IntC<1>{});
+            }
         }
 
         // ---- data gradient: output rows 2wv, 2wv+1 (the conv3x3 loop on the dY tile) ----
@@ -882,7 +882,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         for (int r = 0; r < 2; ++r)
 #pragma unroll
             for (int k = 0; k < 16; ++k) acc[r][k] = 0.f;
-#pragma unroll 1
+#pragma unroll
         for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
             for (int kc = 0; kc < KC; ++kc)
@@ -949,7 +949,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
     {
         const int g = t < 5 ? 0 : 1, j = t - 5 * g;
         __syncthreads();
-        if (tg == g)
+        if (TG == g)
         {
             if constexpr (!L5)
 #pragma unroll
@@ -976,16 +976,19 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
     if constexpr (!L5)
     {
         db += __shfl_xor(db, 32);
-        if (tg == 0 && h == 0) dbs[rg][ch] = db;
+        if (TG == 0 && h == 0) dbs[rg][ch] = db;
     }
     else
     {
         db += __shfl_xor(db, 16);
         db += __shfl_xor(db, 32);
-        if (tg == 0 && lane < 32) dbs[rg][lane] = lane < 16 ? db : 0.f;
+        if (TG == 0 && lane < 32) dbs[rg][lane] = lane < 16 ? db : 0.f;
     }
     __syncthreads();
     if (tid < CO) out[9 * 32 * CO + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
+    };
+    if (tg == 0) run(IntC<0>{});
+    else run(IntC<1>{});
 }
 
 // Two-stage deterministic reduction of the per-block partials, batched over jobs (a job is one
