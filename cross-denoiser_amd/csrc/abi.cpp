@@ -621,10 +621,10 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
     std::vector<xdn::RedJob> jobs;
     std::vector<xdn::BwdFrame> bf(batch);
     int cur = 0;  // dY ping-pong index holding the gradient of the current layer's output
-    // layers 5..2: weight gradient and data gradient into layer l-1 in one kernel (the last layer's
+    // layers 5..3: weight gradient and data gradient into layer l-1 in one kernel (the last layer's
     // dY is the FP32 gradient of the network output); the B images' partials of a layer are reduced
     // together into the one gradient
-    for (int l = 4; l >= 1; --l)
+    for (int l = 4; l >= 2; --l)
     {
         const int nb = ws.nblk[l];
         for (int b = 0; b < batch; ++b)
@@ -639,17 +639,20 @@ int xdn_cnn5_backward(const float *ols1, const float *ols2, const float *prev_ou
         jobs.push_back(cnn5_red_job(ws, l, grad, batch * nb));
         cur = 1 - cur;
     }
-    // layer 1's weight gradient: x = the five input images, dy = the gradient of its pre-activation
-    std::vector<xdn::WgFrame> wf(batch);
-    const int nb0 = ws.nblk[0];
+    // layer 2's backward with layer 1's weight gradient (x = the five input images; layer 1's
+    // pre-activation gradient stays on chip)
+    std::vector<xdn::Bwd12Frame> f12(batch);
+    const int nb1 = ws.nblk[1], nb0 = ws.nblk[0];
     for (int b = 0; b < batch; ++b)
     {
-        const size_t o3 = (size_t)b * N * 3;
-        wf[b] = xdn::WgFrame{{ols1 + o3, ols2 + o3, prev_out + o3, albedo + o3, normal + o3},
-                             ws.dY[cur] + (size_t)b * N * 32, ws.part[0] + (size_t)b * nb0 * ws.pstride[0], height,
-                             width, nb0};
+        const size_t o3 = (size_t)b * N * 3, o32 = (size_t)b * N * 32;
+        f12[b] = xdn::Bwd12Frame{ws.dY[cur] + o32, ws.X[0] + o32, pk + kLay.dg_off[1],
+                                 ws.part[1] + (size_t)b * nb1 * ws.pstride[1],
+                                 {ols1 + o3, ols2 + o3, prev_out + o3, albedo + o3, normal + o3},
+                                 ws.part[0] + (size_t)b * nb0 * ws.pstride[0], height, width, nb1};
     }
-    if (int e = hip_check(xdn::launch_wgrad_l1(batch, wf.data(), S(stream)), "cnn5 wgrad")) return e;
+    if (int e = hip_check(xdn::launch_bwd12(batch, f12.data(), S(stream)), "cnn5 backward")) return e;
+    jobs.push_back(cnn5_red_job(ws, 1, grad, batch * nb1));
     jobs.push_back(cnn5_red_job(ws, 0, grad, batch * nb0));
     return hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), S(stream)), "cnn5 wgrad reduce");
 }

@@ -90,8 +90,9 @@ struct Cnn5Ws {
         for (int i = 0; i < 2; ++i) dY[i] = reinterpret_cast<uint16_t *>(p + o), o += act;
         for (int l = 0; l < 5; ++l)
         {
-            // the weight gradients of layers 2..5 come from the fused backward kernel
-            nblk[l] = l >= 1 ? xdn::bwd_blocks(H, W) : xdn::wgrad_blocks(H, W);
+            // every weight gradient comes from a fused backward kernel on the same tile grid (layer
+            // 1's from layer 2's backward)
+            nblk[l] = xdn::bwd_blocks(H, W);
             pstride[l] = l == 0 ? xdn::WG_PART_L1 : (l == 4 ? xdn::WG_PART_L5 : xdn::WG_PART);
             part[l] = reinterpret_cast<float *>(p + o);
             o += al256((size_t)B * nblk[l] * pstride[l] * 4);

@@ -294,9 +294,10 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
                               "loss"))
             return e;
     }
-    // 6. network backward: layers 5..2 each in one fused launch (weight gradient and data gradient
-    //    into the layer below), then layer 1's weight gradient; the per-block weight-gradient
-    //    partials of every (slot, layer) are reduced by one launch pair
+    // 6. network backward: layers 5..3 each in one fused launch (weight gradient and data gradient
+    //    into the layer below), then layer 2's backward fused with layer 1's weight gradient (layer
+    //    1's pre-activation gradient stays on chip); the per-block weight-gradient partials of every
+    //    (slot, layer) are reduced by one launch pair
     if (unet)
     {
         if (int e = unet_backward_frames(uf.data(), n, st)) return e;
@@ -311,7 +312,7 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
         std::vector<xdn::RedJob> jobs;
         std::vector<xdn::BwdFrame> bf(n);
         int cur = 0;  // dY ping-pong index holding the gradient of the current layer's output
-        for (int l = 4; l >= 1; --l)
+        for (int l = 4; l >= 2; --l)
         {
             // the last layer's dY is the FP32 gradient of the network output
             for (int i = 0; i < n; ++i)
@@ -326,15 +327,17 @@ int xdn_denoise_stages(const xdn_slot *slots, int n_slots, float lr, int stages,
             if (int e = hip_check(xdn::launch_bwd(n, bf.data(), l == 4, st), "cnn backward")) return e;
             cur = 1 - cur;
         }
-        std::vector<xdn::WgFrame> wf(n);
+        std::vector<xdn::Bwd12Frame> f12(n);
         for (int i = 0; i < n; ++i)
         {
             const xdn_slot &s = slots[i];
-            wf[i] = xdn::WgFrame{{s.ols1, s.ols2, sc[i].prev_out, s.albedo, s.normal}, cw[i].dY[cur], cw[i].part[0],
-                                 s.height, s.width, cw[i].nblk[0]};
+            f12[i] = xdn::Bwd12Frame{cw[i].dY[cur], cw[i].X[0], static_cast<const char *>(s.packed) + kLay.dg_off[1],
+                                     cw[i].part[1], {s.ols1, s.ols2, sc[i].prev_out, s.albedo, s.normal},
+                                     cw[i].part[0], s.height, s.width, cw[i].nblk[1]};
+            jobs.push_back(cnn5_red_job(cw[i], 1, s.grad, cw[i].nblk[1]));
             jobs.push_back(cnn5_red_job(cw[i], 0, s.grad, cw[i].nblk[0]));
         }
-        if (int e = hip_check(xdn::launch_wgrad_l1(n, wf.data(), st), "cnn wgrad")) return e;
+        if (int e = hip_check(xdn::launch_bwd12(n, f12.data(), st), "cnn backward")) return e;
         if (int e = hip_check(xdn::launch_wgrad_reduce((int)jobs.size(), jobs.data(), st), "cnn wgrad reduce")) return e;
     }
     // 7. Keras Adam on every slot's parameters and the bf16 fragments: the cnn5 update writes each

@@ -12,14 +12,16 @@
 //                     its four waves computing one layer's row per step from 4-row LDS rings
 //                     of the layer below (the hidden activations are written once and never
 //                     read back); kernel fragments in registers.
-//   conv3x3_bwd       backward of layers 5..2: the data gradient into the layer below (conv3x3
+//   conv3x3_bwd       backward of layers 5..3: the data gradient into the layer below (conv3x3
 //                     scheme on the flipped, transposed kernel) and the weight gradient of one
 //                     16x32-px tile from one staging of dY and the layer input X, eight waves per
 //                     block. The ReLU derivative of the data gradient is read off X itself
-//                     (x > 0 in bf16), so the forward stores no mask.
-//   conv3x3_wgrad_l1  layer 1 weight gradient: M = input channel, N = output channel, K = pixels;
-//                     both operands staged pixel-major in LDS as they lie in HBM and read
+//                     (x > 0 in bf16), so the forward stores no mask. Weight gradients take both
+//                     operands staged pixel-major in LDS as they lie in HBM and read them
 //                     channel-major with the transposing ds_read_b64_tr_b16.
+//   conv3x3_bwd12     backward of layer 2 the same way, with layer 1's weight gradient computed
+//                     on the same tile from layer 1's pre-activation gradient kept in LDS and the
+//                     five input images (so that gradient is never written to HBM).
 //   wgrad_reduce1/2   deterministic two-stage reduction of the per-block weight-gradient partials.
 //   pack_weights      FP32 master weights -> bf16 MFMA B fragments for the forward and the data
 //                     gradient (flipped/transposed) convolutions.
@@ -432,16 +434,6 @@ __global__ __launch_bounds__(256, 3) void cnn5_fwd_stream(FwdStreamBatch b)
 // ---------------------------------------------------------------------------------------------
 // Weight gradient
 // ---------------------------------------------------------------------------------------------
-struct WgBatch {
-    int n;
-    int blk_begin[XDN_MAX_FRAMES + 1];
-    WgFrame f[XDN_MAX_FRAMES];
-};
-static_assert(sizeof(WgBatch) <= 4096, "kernel argument too large");
-
-constexpr int WS_W = 32, WS_H = 8;              // strip: 8 rows x 32 px
-constexpr int WX_C = WS_W + 2, WX_R = WS_H + 2;  // staged X: 10 rows x 34 px (1-px halo)
-
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) v4s lds_v4s;
 
@@ -484,176 +476,6 @@ __device__ __forceinline__ uint32_t nz_mask(uint32_t w)
 
 template <int N> struct IntC { static constexpr int value = N; };
 
-// Weight gradient of layer 1, dW[t][ci][co] = sum_p X[p + shift_t][ci] dY[p][co] over 8x32-px
-// strips, X = the five FP32 3-channel input images (15 channels, channel 15 zero), dY = the bf16
-// gradient of the layer's 32-channel pre-activation. X (with a 1-px halo) and dY are staged
-// pixel-major (16-B stores); the channel-major MFMA operands come from transposing reads, the
-// +-1-px shifts of the taps being address offsets. The product runs on v_mfma_f32_16x16x32_bf16:
-// M = the 16 input channels, N = 16 output channels (two tiles), K = 32 px = one strip row (8 px
-// per lane group kg = lane / 16). Wave wv accumulates strip rows 2wv and 2wv+1; the four waves'
-// accumulators are summed through LDS before the block writes its partial.
-__global__ __launch_bounds__(256, 2) void conv3x3_wgrad_l1(WgBatch b)
-{
-    constexpr int XS = 16, DS = 32;  // staged X / dY channels (pixel strides)
-    constexpr int X_ELEMS = WX_R * WX_C * XS, D_ELEMS = WS_H * WS_W * DS;
-    static_assert(X_ELEMS + D_ELEMS >= 4 * 1024 * 2, "partial reduction space");
-    __shared__ __attribute__((aligned(16))) uint16_t lds[X_ELEMS + D_ELEMS];
-    __shared__ float dbs[4][32];
-    uint16_t *xs = lds, *ds = lds + X_ELEMS;
-    int f = 0;
-    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
-    const WgFrame &F = b.f[f];
-    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
-    const int H = F.H, W = F.W;
-    const int sx = (W + WS_W - 1) / WS_W, sy = (H + WS_H - 1) / WS_H;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3, kg = lane >> 4;
-
-    typedef __attribute__((ext_vector_type(4))) float f32x4;
-    f32x4 acc16[9][2];  // [tap][output-channel tile]
-#pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-        for (int n = 0; n < 2; ++n)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) acc16[t][n][k] = 0.f;
-    float db16[2] = {0.f, 0.f};
-
-    // Software pipeline: the global loads of strip s + nblk are issued into registers before the
-    // MFMAs of strip s and land in LDS after them.
-    constexpr int NX = (WX_R * WX_C + 255) / 256;
-    constexpr int ND = WS_H * WS_W * 4 / 256;
-    u32x4 rd[ND];
-    float fx[NX][15];
-    auto fetch = [&](int s) {
-        const int y0 = (s / sx) * WS_H, x0 = (s % sx) * WS_W;
-        // X: LDS pixel (r, col) <-> (y0 - 1 + r, x0 - 1 + col), zero outside the image; one pixel per
-        // item: the 15 channels of the five 3-channel images
-#pragma unroll
-        for (int u = 0; u < NX; ++u)
-        {
-            const int pix = tid + 256 * u;
-            const int r = pix / WX_C, col = pix - r * WX_C;
-            const int gy = y0 - 1 + r, gx = x0 - 1 + col;
-#pragma unroll
-            for (int j = 0; j < 15; ++j) fx[u][j] = 0.f;
-            if (pix < WX_R * WX_C && gy >= 0 && gy < H && gx >= 0 && gx < W)
-            {
-                const size_t p = (size_t)gy * W + gx;
-#pragma unroll
-                for (int im = 0; im < 5; ++im)
-                {
-                    const float *sp = static_cast<const float *>(F.x[im]) + 3 * p;
-                    fx[u][3 * im] = sp[0]; fx[u][3 * im + 1] = sp[1]; fx[u][3 * im + 2] = sp[2];
-                }
-            }
-        }
-        // dY: LDS pixel (r, col) <-> (y0 + r, x0 + col)
-#pragma unroll
-        for (int u = 0; u < ND; ++u)
-        {
-            const int i = tid + 256 * u;
-            const int pix = i >> 2, q = i & 3;
-            const int r = pix / WS_W, col = pix - r * WS_W;
-            const int gy = y0 + r, gx = x0 + col;
-            rd[u] = u32x4{0u, 0u, 0u, 0u};
-            if (gy < H && gx < W)
-                rd[u] = *reinterpret_cast<const u32x4 *>(static_cast<const uint16_t *>(F.dy) + ((size_t)gy * W + gx) * 32 +
-                                                         q * 8);
-        }
-    };
-    auto commit = [&]() {
-#pragma unroll
-        for (int u = 0; u < NX; ++u)
-        {
-            const int pix = tid + 256 * u;
-            if (pix < WX_R * WX_C)
-            {
-                u32x4 a, c;
-                a.x = pack2(fx[u][0], fx[u][1]); a.y = pack2(fx[u][2], fx[u][3]);
-                a.z = pack2(fx[u][4], fx[u][5]); a.w = pack2(fx[u][6], fx[u][7]);
-                c.x = pack2(fx[u][8], fx[u][9]); c.y = pack2(fx[u][10], fx[u][11]);
-                c.z = pack2(fx[u][12], fx[u][13]); c.w = pack2(fx[u][14], 0.f);
-                *reinterpret_cast<u32x4 *>(&xs[pix * XS]) = a;
-                *reinterpret_cast<u32x4 *>(&xs[pix * XS + 8]) = c;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < ND; ++u)
-        {
-            const int i = tid + 256 * u;
-            *reinterpret_cast<u32x4 *>(&ds[(i >> 2) * DS + (i & 3) * 8]) = rd[u];
-        }
-    };
-
-    const int nstrip = sx * sy;
-    const StripWalk sw = xcd_strips(li, nblk, nstrip);
-    if (sw.first < sw.end) fetch(sw.first);
-    for (int s = sw.first; s < sw.end; s += sw.step)
-    {
-        __syncthreads();  // the previous strip's reads are done
-        commit();
-        __syncthreads();
-        if (s + sw.step < sw.end) fetch(s + sw.step);
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-        {
-            const int row = 2 * wv + r;
-            lds_u16 *dp = (lds_u16 *)ds + (row * WS_W + 8 * kg + jq) * DS + 4 * qq;
-            lds_u16 *xp = (lds_u16 *)xs + (row * WX_C + 8 * kg + jq) * XS + 4 * qq;
-            u32x4 Bn[2];
-#pragma unroll
-            for (int n = 0; n < 2; ++n)
-            {
-                const uint2 b0 = ld_tr(dp + 16 * n), b1 = ld_tr(dp + 16 * n + 4 * DS);
-                Bn[n] = u32x4{b0.x, b0.y, b1.x, b1.y};
-                db16[n] = add_bf8(Bn[n], db16[n]);
-            }
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                for (int sh = 0; sh < 3; ++sh)
-                {
-                    const int o = (dy * WX_C + sh) * XS;
-                    const uint2 a0 = ld_tr(xp + o), a1 = ld_tr(xp + o + 4 * XS);
-                    const bf16x8 A = __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y});
-#pragma unroll
-                    for (int n = 0; n < 2; ++n)
-                        acc16[dy * 3 + sh][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            A, __builtin_bit_cast(bf16x8, Bn[n]), acc16[dy * 3 + sh][n], 0, 0, 0);
-                }
-        }
-    }
-    // compact partial: [t][15 ci][32 co] then [32 co]; the four waves' sums are added in LDS one tap
-    // at a time
-    float *out = F.part + (size_t)li * WG_PART_L1;
-    float *red = reinterpret_cast<float *>(lds);  // [4 waves][1024]
-#pragma unroll
-    for (int t = 0; t < 9; ++t)
-    {
-        __syncthreads();
-        // 16x16 accumulators: column co = 16n + lane % 16, rows ci = 4kg + i
-#pragma unroll
-        for (int n = 0; n < 2; ++n)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) red[wv * 1024 + (4 * kg + i) * 32 + 16 * n + (lane & 15)] = acc16[t][n][i];
-        __syncthreads();
-        // input channel 15 is padding
-        for (int e = tid; e < 15 * 32; e += 256)
-            out[t * 15 * 32 + e] = (red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e]);
-    }
-#pragma unroll
-    for (int n = 0; n < 2; ++n)
-    {
-        float v = db16[n];
-        v += __shfl_xor(v, 16);
-        v += __shfl_xor(v, 32);
-        if (lane < 16) dbs[wv][16 * n + lane] = v;
-    }
-    __syncthreads();
-    if (tid < 32) out[9 * 15 * 32 + tid] = (dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid]);
-}
-
 // ---------------------------------------------------------------------------------------------
 // Fused backward of a 32 -> 32 layer
 // ---------------------------------------------------------------------------------------------
@@ -670,8 +492,8 @@ static_assert(sizeof(BwdBatch) <= 4096, "kernel argument too large");
 //                  ReLU derivative, read from the staged X tile); B operands read from the dY tile
 //                  with ds_read_b128, as conv3x3_l1 reads its input tile;
 //   weight gradient dW[t] += X(p + shift_t)^T dY(p) over the tile's pixels: transposing reads of
-//                  both tiles, as in conv3x3_wgrad_l1, accumulators kept in registers across the
-//                  block's tiles.
+//                  both tiles (M = input channel, N = output channel, K = pixels), accumulators
+//                  kept in registers across the block's tiles.
 // dY is read from HBM once instead of once per product. Eight waves share the work of a tile:
 // wave w computes the data gradient of tile rows 2w, 2w+1 and the weight gradient of taps
 // 0..4 (w < 4) or 5..8 (w >= 4) over tile rows 4(w%4) .. 4(w%4)+3, so each wave holds 80 + 32
@@ -991,6 +813,334 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
     else run(IntC<1>{});
 }
 
+// ---------------------------------------------------------------------------------------------
+// Backward of layer 2 fused with layer 1's weight gradient
+// ---------------------------------------------------------------------------------------------
+struct Bwd12Batch {
+    int n;
+    int blk_begin[XDN_MAX_FRAMES + 1];
+    Bwd12Frame f[XDN_MAX_FRAMES];
+};
+static_assert(sizeof(Bwd12Batch) <= 4096, "kernel argument too large");
+
+// Layer 2's backward on one staging of a 16x32-px tile of its dY and its input X (both with the
+// 1-px halo, as in conv3x3_bwd) and of layer 1's five input images (1-px halo, bf16, 15 channels
+// and a zero one):
+//   layer 2 weight gradient: wave w takes tap w over the whole tile (32 MFMAs; it holds the block's
+//     only accumulator of that tap) and tap 8 over tile rows 2w, 2w+1 (4 MFMAs; the eight waves'
+//     sums of tap 8 are added when the block writes its partial);
+//   layer 2 data gradient: wave w computes tile rows 2w, 2w+1 as conv3x3_bwd does and keeps it
+//     where X is nonzero: that is layer 1's pre-activation gradient dY1, which goes into LDS over
+//     the dY tile (read by then) instead of to HBM;
+//   layer 1 weight gradient (v_mfma_f32_16x16x32_bf16: M = 16 input channels, N = 16 output
+//     channels, two tiles, K = one tile row's 32 px): wave w takes tap w over the tile and tap 8
+//     over rows 2w, 2w+1, A read from the image tile and B from dY1, both with transposing reads.
+// The tap-major split keeps 48 accumulator registers per wave for the two weight gradients (a
+// row-group split would need 80 + 72), so the three products and the prefetch of the three staged
+// tiles fit two waves per SIMD. dY1 never reaches HBM, and layer 1's weight gradient needs no
+// launch of its own. Four barriers per tile: the tile is staged; every wave is done with the dY
+// tile; dY1 is in LDS; every wave is done with all tiles.
+__global__ __launch_bounds__(512, 1) void conv3x3_bwd12(Bwd12Batch b)
+{
+    constexpr int PS = 40, XS = 40;   // dY / X pixel strides in bf16 (as conv3x3_bwd)
+    constexpr int IS = 16;            // image-tile pixel stride: 15 channels + 1 zero
+    constexpr int D1S = 40;           // dY1 pixel stride ([16 rows][32 px], over the dY tile)
+    constexpr int LW = CT_W + 2, LH = CT_H + 2;
+    constexpr int NCH = LH * LW * 4;  // 16-B chunks of a staged dY or X tile
+    constexpr int NT = 512;
+    constexpr int NV = (NCH + NT - 1) / NT;
+    constexpr int NI = (LH * LW + NT - 1) / NT;  // image-tile pixels per thread
+    __shared__ __attribute__((aligned(16))) uint16_t dys[LH * LW * PS];
+    __shared__ __attribute__((aligned(16))) uint16_t xs[LH * LW * XS];
+    __shared__ __attribute__((aligned(16))) uint16_t ims[LH * LW * IS];
+    __shared__ u32x4 wl[9 * 2 * 64];
+    __shared__ float dbs[8][32], dbs1[8][32];
+    static_assert(LH * LW * PS >= CT_H * CT_W * D1S, "dY1 tile space");
+    static_assert(LH * LW * PS * 2 >= 8 * 1024 * 4, "partial reduction space");
+    static_assert(LH * LW * XS * 2 >= 8 * 512 * 4, "layer-1 partial reduction space");
+    int f = 0;
+    while (f + 1 < b.n && (int)blockIdx.x >= b.blk_begin[f + 1]) ++f;
+    const Bwd12Frame &F = b.f[f];
+    const int li = blockIdx.x - b.blk_begin[f], nblk = b.blk_begin[f + 1] - b.blk_begin[f];
+    const int H = F.H, W = F.W;
+    const int ntx = (W + CT_W - 1) / CT_W, nty = (H + CT_H - 1) / CT_H;
+    const int tid = threadIdx.x, lane = tid & 63, px = lane & 31, h = lane >> 5, ch = lane & 31;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: this wave's tap
+    const int grp = (lane >> 4) & 1, s16 = lane & 15, jq = s16 >> 2, qq = s16 & 3, kg = lane >> 4;
+    const int cb = 16 * grp + 4 * qq;  // channel offset addressed by this lane in transposing reads
+    const int tdy = wv / 3, tsh = wv - 3 * tdy;
+
+    for (int e = tid; e < 9 * 2 * 64; e += NT) wl[e] = static_cast<const u32x4 *>(F.wfrag)[e];
+
+    // per-lane LDS bases of the transposing reads; a row, chunk or tap-8 shift adds an immediate
+    lds_u16 *const dys3 = (lds_u16 *)dys, *const xs3 = (lds_u16 *)xs, *const ims3 = (lds_u16 *)ims;
+    lds_u16 *const dB = dys3 + ((h + 1) * LW + jq + 1) * PS + cb;                 // layer 2 dY, rows 2p + h
+    lds_u16 *const xA = xs3 + ((h + tdy) * LW + jq + tsh) * XS + cb;              // layer 2 X, tap wv
+    lds_u16 *const dB8 = dys3 + ((2 * wv + h + 1) * LW + jq + 1) * PS + cb;       // rows 2wv + h
+    lds_u16 *const xA8 = xs3 + ((2 * wv + h + 2) * LW + jq + 2) * XS + cb;        // tap 8 there
+    lds_u16 *const d1B = dys3 + (8 * kg + jq) * D1S + 4 * qq;                    // dY1, row 0
+    lds_u16 *const iA = ims3 + ((tdy * LW) + 8 * kg + jq + tsh) * IS + 4 * qq;    // images, tap wv
+    lds_u16 *const iA8 = ims3 + ((2 * wv + 2) * LW + 8 * kg + jq + 2) * IS + 4 * qq;  // tap 8, row 2wv
+
+    typedef __attribute__((ext_vector_type(4))) float f32x4;
+    f32x16 wa2, wa28;    // layer 2: tap wv over the tile; tap 8 over rows 2wv, 2wv + 1
+    f32x4 wa1[2], wa18[2];  // layer 1 (two output-channel tiles): the same split
+#pragma unroll
+    for (int k = 0; k < 16; ++k) wa2[k] = wa28[k] = 0.f;
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wa1[n][k] = wa18[n][k] = 0.f;
+    float db2 = 0.f, db1[2] = {0.f, 0.f};
+
+    // staging: branch-free buffer loads (zero outside the image)
+    const size_t npx = (size_t)H * W;
+    const rsrc_t rdy = make_rsrc(F.dy, npx * 64), rxx = make_rsrc(F.x, npx * 64);
+    rsrc_t rim[5];
+#pragma unroll
+    for (int im = 0; im < 5; ++im) rim[im] = make_rsrc(F.img[im], npx * 12);
+    u32x4 rd[NV], rx[NV];
+    u32x3 fi[NI][5];
+    auto fetch = [&](int t) {
+        const int y0 = (t / ntx) * CT_H, x0 = (t % ntx) * CT_W;
+#pragma unroll
+        for (int u = 0; u < NV; ++u)
+        {
+            const int i = tid + NT * u;
+            const int pix = i >> 2, q = i & 3;
+            const int ly = pix / LW, lx = pix - ly * LW;
+            const int gy = y0 + ly - 1, gxx = x0 + lx - 1;
+            const bool ok = i < NCH && gy >= 0 && gy < H && gxx >= 0 && gxx < W;
+            const uint32_t o = ok ? (uint32_t)((gy * W + gxx) * 32 + q * 8) * 2u : kOob;
+            rd[u] = bld16(rdy, o);
+            rx[u] = bld16(rxx, o);
+        }
+#pragma unroll
+        for (int u = 0; u < NI; ++u)
+        {
+            const int pix = tid + NT * u;
+            const int ly = pix / LW, lx = pix - ly * LW;
+            const int gy = y0 + ly - 1, gxx = x0 + lx - 1;
+            const bool ok = pix < LH * LW && gy >= 0 && gy < H && gxx >= 0 && gxx < W;
+            const uint32_t o = ok ? (uint32_t)(gy * W + gxx) * 12u : kOob;
+#pragma unroll
+            for (int im = 0; im < 5; ++im) fi[u][im] = bld12(rim[im], o);
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int u = 0; u < NV; ++u)
+        {
+            const int i = tid + NT * u;
+            if (i < NCH)
+            {
+                const int pix = i >> 2, q = i & 3;
+                *reinterpret_cast<u32x4 *>(&dys[pix * PS + q * 8]) = rd[u];
+                *reinterpret_cast<u32x4 *>(&xs[pix * XS + q * 8]) = rx[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NI; ++u)
+        {
+            const int pix = tid + NT * u;
+            if (pix < LH * LW)
+            {
+                float v[15];
+#pragma unroll
+                for (int im = 0; im < 5; ++im)
+                {
+                    v[3 * im] = __uint_as_float(fi[u][im].x);
+                    v[3 * im + 1] = __uint_as_float(fi[u][im].y);
+                    v[3 * im + 2] = __uint_as_float(fi[u][im].z);
+                }
+                const u32x4 a = {pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7])};
+                const u32x4 c = {pack2(v[8], v[9]), pack2(v[10], v[11]), pack2(v[12], v[13]), pack2(v[14], 0.f)};
+                *reinterpret_cast<u32x4 *>(&ims[pix * IS]) = a;
+                *reinterpret_cast<u32x4 *>(&ims[pix * IS + 8]) = c;
+            }
+        }
+    };
+
+    const StripWalk sw = xcd_strips(li, nblk, ntx * nty);
+    if (sw.first < sw.end)
+    {
+        fetch(sw.first);
+        commit();
+    }
+    for (int t = sw.first; t < sw.end; t += sw.step)
+    {
+        __syncthreads();  // tile t is in LDS
+        const int nxt = t + sw.step;
+        fetch(nxt < sw.end ? nxt : t);  // the last tile reloads itself: no branch around the loads
+
+        // ---- layer 2 weight gradient: tap wv over rows 2p + h, 8-px K chunks c; tap 8 over 2wv + h ----
+#pragma unroll 2
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                const int od = (2 * p * LW + 8 * c) * PS, ox = (2 * p * LW + 8 * c) * XS;
+                const uint2 b0 = ld_tr(dB + od), b1 = ld_tr(dB + od + 4 * PS);
+                const uint2 a0 = ld_tr(xA + ox), a1 = ld_tr(xA + ox + 4 * XS);
+                wa2 = mfma(u32x4{a0.x, a0.y, a1.x, a1.y}, u32x4{b0.x, b0.y, b1.x, b1.y}, wa2);
+            }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+        {
+            const uint2 b0 = ld_tr(dB8 + 8 * c * PS), b1 = ld_tr(dB8 + (8 * c + 4) * PS);
+            const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
+            db2 = add_bf8(B, db2);
+            const uint2 a0 = ld_tr(xA8 + 8 * c * XS), a1 = ld_tr(xA8 + (8 * c + 4) * XS);
+            wa28 = mfma(u32x4{a0.x, a0.y, a1.x, a1.y}, B, wa28);
+        }
+
+        // ---- layer 2 data gradient: rows 2wv, 2wv + 1, kept where X is nonzero -> dY1 ----
+        f32x16 acc[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int k = 0; k < 16; ++k) acc[r][k] = 0.f;
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+            for (int kc = 0; kc < 2; ++kc)
+            {
+                u32x4 wk[3], bx[4];
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) wk[dy] = wl[((dy * 3 + dx) * 2 + kc) * 64 + lane];
+#pragma unroll
+                for (int lr = 0; lr < 4; ++lr)
+                    bx[lr] = *reinterpret_cast<const u32x4 *>(&dys[((2 * wv + lr) * LW + px + dx) * PS + kc * 16 + 8 * h]);
+#pragma unroll
+                for (int lr = 0; lr < 4; ++lr)
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy)
+                    {
+                        const int orow = lr - dy;
+                        if (orow >= 0 && orow < 2) acc[orow] = mfma(wk[dy], bx[lr], acc[orow]);
+                    }
+            }
+        u32x4 g1[2][2];  // dY1 of rows 2wv + orow: lane (px, h) holds channels 8h .. 8h + 7 and 16 + 8h ..
+#pragma unroll
+        for (int orow = 0; orow < 2; ++orow)
+        {
+            uint32_t d[8];  // d[2j + e] = channels 8j + 4h + 2e, +1
+#pragma unroll
+            for (int q = 0; q < 8; ++q) d[q] = pack2(acc[orow][2 * q], acc[orow][2 * q + 1]);
+            const uint16_t *xp = &xs[((2 * wv + orow + 1) * LW + px + 1) * XS + 4 * h];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const u32x2 xw = *reinterpret_cast<const u32x2 *>(xp + 8 * j);
+#pragma unroll
+                for (int e = 0; e < 2; ++e) d[2 * j + e] &= nz_mask(xw[e]);
+            }
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                {
+                    const auto sp = __builtin_amdgcn_permlane32_swap(d[4 * g + e], d[4 * g + 2 + e], false, false);
+                    g1[orow][g][e] = (uint32_t)sp[0];
+                    g1[orow][g][2 + e] = (uint32_t)sp[1];
+                }
+        }
+        __syncthreads();  // every wave is done with the dY tile
+#pragma unroll
+        for (int orow = 0; orow < 2; ++orow)
+        {
+            uint16_t *dp = &dys[((2 * wv + orow) * CT_W + px) * D1S + 8 * h];
+            *reinterpret_cast<u32x4 *>(dp) = g1[orow][0];
+            *reinterpret_cast<u32x4 *>(dp + 16) = g1[orow][1];
+        }
+        __syncthreads();  // dY1 is in LDS
+
+        // ---- layer 1 weight gradient: tap wv over the 16 rows; tap 8 over rows 2wv, 2wv + 1 ----
+#pragma unroll 4
+        for (int r = 0; r < CT_H; ++r)
+        {
+            const uint2 a0 = ld_tr(iA + r * LW * IS), a1 = ld_tr(iA + r * LW * IS + 4 * IS);
+            const bf16x8 A = __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y});
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+            {
+                const int od = r * CT_W * D1S + 16 * n;
+                const uint2 b0 = ld_tr(d1B + od), b1 = ld_tr(d1B + od + 4 * D1S);
+                wa1[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    A, __builtin_bit_cast(bf16x8, u32x4{b0.x, b0.y, b1.x, b1.y}), wa1[n], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+        {
+            const uint2 a0 = ld_tr(iA8 + r * LW * IS), a1 = ld_tr(iA8 + r * LW * IS + 4 * IS);
+            const bf16x8 A = __builtin_bit_cast(bf16x8, u32x4{a0.x, a0.y, a1.x, a1.y});
+#pragma unroll
+            for (int n = 0; n < 2; ++n)
+            {
+                lds_u16 *const bp = d1B + (2 * wv + r) * CT_W * D1S + 16 * n;
+                const uint2 b0 = ld_tr(bp), b1 = ld_tr(bp + 4 * D1S);
+                const u32x4 B = {b0.x, b0.y, b1.x, b1.y};
+                db1[n] = add_bf8(B, db1[n]);
+                wa18[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, __builtin_bit_cast(bf16x8, B), wa18[n], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wave is done reading tile t
+        if (nxt < sw.end) commit();
+    }
+
+    // partials: layer 2 [t][ci][co] then [co]; layer 1 compact [t][15 ci][32 co] then [32 co]. Taps
+    // 0..7 come whole from wave t; tap 8 is the sum of the eight waves' row pairs, added in LDS.
+    float *out = F.part + (size_t)li * WG_PART;
+    float *out1 = F.part1 + (size_t)li * WG_PART_L1;
+    float *red = reinterpret_cast<float *>(dys);   // [8 waves][1024]
+    float *red1 = reinterpret_cast<float *>(xs);   // [8 waves][512]
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+    {
+        const int ci = 8 * (k >> 2) + 4 * h + (k & 3);
+        out[wv * 1024 + ci * 32 + ch] = wa2[k];
+        red[wv * 1024 + ci * 32 + ch] = wa28[k];
+    }
+    // 16x16 accumulators: column co = 16n + lane % 16, rows ci = 4kg + i (input channel 15 is padding)
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+            const int ci = 4 * kg + i, co = 16 * n + (lane & 15);
+            if (ci < 15) out1[wv * 15 * 32 + ci * 32 + co] = wa1[n][i];
+            red1[wv * 512 + ci * 32 + co] = wa18[n][i];
+        }
+    db2 += __shfl_xor(db2, 32);
+    if (h == 0) dbs[wv][ch] = db2;
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+    {
+        float v = db1[n];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (lane < 16) dbs1[wv][16 * n + lane] = v;
+    }
+    __syncthreads();
+    for (int e = tid; e < 1024; e += NT)
+        out[8 * 1024 + e] = ((red[e] + red[1024 + e]) + (red[2048 + e] + red[3072 + e])) +
+                            ((red[4096 + e] + red[5120 + e]) + (red[6144 + e] + red[7168 + e]));
+    if (tid < 15 * 32)
+        out1[8 * 15 * 32 + tid] = ((red1[tid] + red1[512 + tid]) + (red1[1024 + tid] + red1[1536 + tid])) +
+                                  ((red1[2048 + tid] + red1[2560 + tid]) + (red1[3072 + tid] + red1[3584 + tid]));
+    if (tid < 32)
+        out[9 * 1024 + tid] = ((dbs[0][tid] + dbs[1][tid]) + (dbs[2][tid] + dbs[3][tid])) +
+                              ((dbs[4][tid] + dbs[5][tid]) + (dbs[6][tid] + dbs[7][tid]));
+    else if (tid < 64)
+    {
+        const int c = tid - 32;
+        out1[9 * 15 * 32 + c] = ((dbs1[0][c] + dbs1[1][c]) + (dbs1[2][c] + dbs1[3][c])) +
+                                ((dbs1[4][c] + dbs1[5][c]) + (dbs1[6][c] + dbs1[7][c]));
+    }
+}
+
 // Two-stage deterministic reduction of the per-block partials, batched over jobs (a job is one
 // layer of one frame). Stage 1: stage[j][split][e] = sum of the partials split*P/S .. of job j;
 // stage 2 writes the flat gradient of the layer ([9][CI][CO] then [CO], unpadded).
@@ -1135,35 +1285,6 @@ hipError_t launch_conv_l1(int n, const ConvFrame *fr, hipStream_t st)
     return hipGetLastError();
 }
 
-int wgrad_blocks(int H, int W)
-{
-    const int strips = div_up(W, WS_W) * div_up(H, WS_H);
-    // ~16 strips per block (about two blocks per CU at 1080p): the strip loop is software-pipelined;
-    // a smaller image still gets two blocks per CU where it has the strips (1280x720: 512 blocks of
-    // about 7 strips instead of 225 of 16)
-    int nb = div_up(strips, 16);
-    if (nb < 512) nb = 512;
-    if (nb > strips) nb = strips;
-    return nb < 1 ? 1 : nb;
-}
-
-hipError_t launch_wgrad_l1(int n, const WgFrame *fr, hipStream_t st)
-{
-    WgBatch b{};
-    b.n = n;
-    int acc = 0;
-    for (int i = 0; i < n; ++i)
-    {
-        b.f[i] = fr[i];
-        b.blk_begin[i] = acc;
-        acc += fr[i].nblk;
-    }
-    b.blk_begin[n] = acc;
-    if (acc == 0) return hipSuccess;
-    hipLaunchKernelGGL(conv3x3_wgrad_l1, dim3(acc), dim3(256), 0, st, b);
-    return hipGetLastError();
-}
-
 int bwd_blocks(int H, int W)
 {
     const int tiles = div_up(W, CT_W) * div_up(H, CT_H);
@@ -1194,6 +1315,25 @@ hipError_t launch_bwd(int n, const BwdFrame *fr, bool last, hipStream_t st)
         hipLaunchKernelGGL(conv3x3_bwd<true>, dim3(acc), dim3(512), 0, st, b);
     else
         hipLaunchKernelGGL(conv3x3_bwd<false>, dim3(acc), dim3(512), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_bwd12(int n, const Bwd12Frame *fr, hipStream_t st)
+{
+    for (int i = 0; i < n; ++i)  // 32-bit buffer offsets, kOob = 2^31 past every buffer
+        if ((size_t)fr[i].H * fr[i].W * 64 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
+    Bwd12Batch b{};
+    b.n = n;
+    int acc = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        b.f[i] = fr[i];
+        b.blk_begin[i] = acc;
+        acc += fr[i].nblk;
+    }
+    b.blk_begin[n] = acc;
+    if (acc == 0) return hipSuccess;
+    hipLaunchKernelGGL(conv3x3_bwd12, dim3(acc), dim3(512), 0, st, b);
     return hipGetLastError();
 }
 

@@ -8,7 +8,7 @@
 //     plain for concat and pooled inputs). Channel counts are the stored ones (8, 16, 24, 32, 48).
 //   uwgrad<SRC, CA, CB, CO, MT>  weight gradient over 8x32-px strips, operands from pixel-major
 //     LDS tiles through ds_read_b64_tr_b16, one 32-channel input slice (MT) per instantiation,
-//     software-pipelined strip loop (as conv3x3_wgrad_l1 of k_cnn.hip).
+//     software-pipelined strip loop.
 //   maxpool2 / pool_bwd          2x2 max pool and its gradient (to the first maximum, plus the
 //     skip-connection gradient, ReLU-masked).
 //   convT1x1s2 / convT_bwd       transposed 1x1 stride-2 convolution and its gradients.

@@ -19,14 +19,6 @@ struct ConvFrame {
     int H, W;
 };
 
-// Layer 1 weight gradient.
-struct WgFrame {
-    const float *x[5];       // the five FP32 input images
-    const void *dy;          // bf16 [H][W][32]: gradient of the layer's pre-activation
-    float *part;             // [nblk][WG_PART_L1], compact
-    int H, W, nblk;
-};
-
 // Fused backward of one 32 -> 32 layer: its weight gradient and its data gradient into the layer
 // below, both from one staging of dY and x.
 struct BwdFrame {
@@ -35,6 +27,19 @@ struct BwdFrame {
     void *dx;                // bf16 [H][W][32]: gradient of x where x > 0, else 0 (dY of the layer below)
     const void *wfrag;       // data-gradient B fragments [9][KC][64] (16 B each; KC 2, last layer 1)
     float *part;             // [nblk][WG_PART] weight-gradient partials (last layer: WG_PART_L5, compact)
+    int H, W, nblk;
+};
+
+// Backward of layer 2 fused with layer 1's weight gradient: layer 2's weight gradient and its data
+// gradient, which (masked by x > 0) is layer 1's pre-activation gradient; that one stays in LDS and
+// meets the five input images there for layer 1's weight and bias gradient, so it is never written.
+struct Bwd12Frame {
+    const void *dy;          // bf16 [H][W][32]: gradient of layer 2's pre-activation
+    const void *x;           // bf16 [H][W][32]: layer 2's input (layer 1's ReLU output)
+    const void *wfrag;       // layer 2's data-gradient B fragments [9][2][64]
+    float *part;             // [nblk][WG_PART] layer 2's weight-gradient partials
+    const float *img[5];     // FP32 [H][W][3]: layer 1's input images
+    float *part1;            // [nblk][WG_PART_L1] layer 1's weight-gradient partials, compact
     int H, W, nblk;
 };
 
@@ -58,7 +63,6 @@ struct PackJob {
 };
 
 hipError_t launch_conv_l1(int n, const ConvFrame *fr, hipStream_t st);
-hipError_t launch_wgrad_l1(int n, const WgFrame *fr, hipStream_t st);
 // One reduction job: the nblk partials of one layer of one frame -> its flat gradient.
 // The partials hold an input-channel slice ci_off .. ci_off + ci_cnt - 1 (ci_cnt < 0: all ci),
 // [taps][32 ci][32 co] + [32 co] in WG_PART floats each, or when compact [taps][ci_cnt][co] +
@@ -80,9 +84,9 @@ constexpr int WG_PART_L5 = compact_part(9, 32, 6);
 constexpr int WG_MAX_JOBS = 48;  // per launch; longer job lists are split
 hipError_t launch_wgrad_reduce(int n, const RedJob *jobs, hipStream_t st);
 hipError_t launch_pack(int n, const PackJob *jobs, hipStream_t st);
-int wgrad_blocks(int H, int W);
 // last: the network's last layer (dy = the FP32 6-channel gradient of the output, L5 fragments)
 hipError_t launch_bwd(int n, const BwdFrame *fr, bool last, hipStream_t st);
+hipError_t launch_bwd12(int n, const Bwd12Frame *fr, hipStream_t st);  // nblk = bwd_blocks(H, W)
 hipError_t launch_fwd_stream(int n, const FwdStreamFrame *fr, hipStream_t st);
 int bwd_blocks(int H, int W);  // blocks (= weight-gradient partials) per image of launch_bwd
 

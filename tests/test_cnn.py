@@ -103,31 +103,28 @@ def _ws_act(net, i, B, H, W):
 
 
 def check_kernels_on_own_operands(net, imgs, layers_cpu, B, H, W):
-    """After backward: ping-pong slot 0 holds dY of layer 1, slot 1 dY of layer 2 (pre-activation
-    gradients), slot X[0] the layer-1 activation. Checks wgrad (image input and bf16 input) and
-    the masked dgrad of layer 2 on exactly the operands the kernels read."""
+    """After backward: ping-pong slot 1 holds dY of layer 2 (its pre-activation gradient) and X[0]
+    the layer-1 activation, the operands of the kernel that runs layer 2's backward together with
+    layer 1's weight gradient. Layer 2's weight gradient is checked on exactly those operands. Layer
+    1's pre-activation gradient (layer 2's data gradient where the activation is nonzero) stays in
+    that kernel's LDS, so layer 1's weight gradient is checked against the one computed from its
+    exact value rounded to bf16 once: the kernel's FP32 sums round a few elements to the
+    neighbouring bf16 value (<= 1e-3 of them, one ulp = 2^-8 relative, random signs), which moves
+    the weight and bias gradient by about sqrt(1e-3) * 2^-8 ~ 1e-4 relative at most; a wrong tap,
+    channel, row or mask moves them by O(1)."""
     from xdn.cnn import unflatten_params
     got = unflatten_params(net.grad.cpu())
     x0 = bf(torch.cat(imgs, -1).double()).permute(0, 3, 1, 2)
-    a1, dy1, dy2 = _ws_act(net, 0, B, H, W), _ws_act(net, 4, B, H, W), _ws_act(net, 5, B, H, W)
-    for l, (x, dy) in enumerate(((x0, dy1), (a1, dy2))):
+    a1, dy2 = _ws_act(net, 0, B, H, W), _ws_act(net, 5, B, H, W)
+    w1 = bf(layers_cpu[1][0].double()).permute(3, 2, 0, 1)
+    dy1 = (torch.nn.grad.conv2d_input(a1.shape, w1, dy2, padding=1) * (a1 > 0)).to(torch.bfloat16).to(torch.float64)
+    for l, (x, dy, tol) in enumerate(((x0, dy1, 1e-3), (a1, dy2, 1e-5))):
         w = layers_cpu[l][0]
         dw = torch.nn.grad.conv2d_weight(x, bf(w.double()).permute(3, 2, 0, 1).shape, dy, padding=1)
         ew = rel_l2(got[l][0].numpy(), dw.permute(2, 3, 1, 0).numpy())
         eb = rel_l2(got[l][1].numpy(), dy.sum((0, 2, 3)).numpy())
-        print(f"  wgrad layer {l + 1} on its own operands: dW {ew:.2e} db {eb:.2e}", flush=True)
-        assert ew <= 1e-5 and eb <= 1e-5, l
-    w1 = bf(layers_cpu[1][0].double()).permute(3, 2, 0, 1)
-    dx = torch.nn.grad.conv2d_input(a1.shape, w1, dy2, padding=1) * (a1 > 0)
-    ref = dx.to(torch.bfloat16).to(torch.float64)
-    diff = (ref != dy1)
-    ulp = (ref.abs() * 2.0 ** -7).clamp_min(1e-30)
-    far = ((ref - dy1).abs() > 1.01 * ulp) & diff
-    print(f"  dgrad layer 2 on its own operands: {int(diff.sum())} of {diff.numel()} differ by 1 ulp, "
-          f"{int(far.sum())} by more", flush=True)
-    # FP32 accumulation (with cancellation) against the exact sum can move a result across more
-    # than one bf16 rounding boundary now and then
-    assert int(far.sum()) <= 2 + 1e-5 * diff.numel() and int(diff.sum()) <= 1e-3 * diff.numel()
+        print(f"  wgrad layer {l + 1} on its operands: dW {ew:.2e} db {eb:.2e}", flush=True)
+        assert ew <= tol and eb <= tol, l
 
 
 def test_cnn5_adam_and_pack_roundtrip(dev):
