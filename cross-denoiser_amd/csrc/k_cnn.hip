@@ -635,6 +635,10 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
         __syncthreads();  // tile t is in LDS
         const int nxt = t + sw.step;
         fetch(nxt < sw.end ? nxt : t);  // the last tile reloads itself: no branch around the loads
+        // The MFMA phases run at a raised wave priority, so a SIMD issues the MFMAs of one wave
+        // ahead of the other wave's epilogue, staging and address work, which then fills the
+        // cycles the matrix pipe is busy (measured: cnn5 backward -24 us with conv3x3_bwd12).
+        __builtin_amdgcn_s_setprio(1);
 
         if constexpr (!L5)
         {
@@ -724,6 +728,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd(BwdBatch b)
                         if (orow >= 0 && orow < 2) acc[orow] = mfma(wk[dy], bx[lr], acc[orow]);
                     }
             }
+        __builtin_amdgcn_s_setprio(0);
         // epilogue: keep the gradient where X's bf16 value is > 0, 16-B stores after permlane32 swaps
 #pragma unroll
         for (int orow = 0; orow < 2; ++orow)
@@ -972,6 +977,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd12(Bwd12Batch b)
         __syncthreads();  // tile t is in LDS
         const int nxt = t + sw.step;
         fetch(nxt < sw.end ? nxt : t);  // the last tile reloads itself: no branch around the loads
+        __builtin_amdgcn_s_setprio(1);  // MFMA phases at a raised wave priority (as in conv3x3_bwd)
 
         // ---- layer 2 weight gradient: tap wv over rows 2p + h, 8-px K chunks c; tap 8 over 2wv + h ----
 #pragma unroll 2
@@ -1020,6 +1026,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd12(Bwd12Batch b)
                         if (orow >= 0 && orow < 2) acc[orow] = mfma(wk[dy], bx[lr], acc[orow]);
                     }
             }
+        __builtin_amdgcn_s_setprio(0);
         u32x4 g1[2][2];  // dY1 of rows 2wv + orow: lane (px, h) holds channels 8h .. 8h + 7 and 16 + 8h ..
 #pragma unroll
         for (int orow = 0; orow < 2; ++orow)
@@ -1056,6 +1063,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd12(Bwd12Batch b)
         __syncthreads();  // dY1 is in LDS
 
         // ---- layer 1 weight gradient: tap wv over the 16 rows; tap 8 over rows 2wv, 2wv + 1 ----
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll 4
         for (int r = 0; r < CT_H; ++r)
         {
@@ -1085,6 +1093,7 @@ __global__ __launch_bounds__(512, 1) void conv3x3_bwd12(Bwd12Batch b)
                 wa18[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, __builtin_bit_cast(bf16x8, B), wa18[n], 0, 0, 0);
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();  // every wave is done reading tile t
         if (nxt < sw.end) commit();
     }
