@@ -193,6 +193,9 @@ __global__ __launch_bounds__(256) void regression_rows(RegBatch b)
     acc.zero();
 
     const int ncols = min(RG_ROW, W + 2 * RG_H - bx0 * RG_S);  // columns actually needed
+    // a wave whose 32 centers all lie past the row end (the last block of a row of 480 centers
+    // holds 32) only stages rows and meets the barriers; its lanes' results are never stored
+    const bool wave_busy = __builtin_amdgcn_readfirstlane((bx0 + (j & ~63) / 2) < Ws);
     const int own = half, oth = 1 - half;
     // first tap column of this lane: split 0 covers dx in [-8, 0], split 1 covers [1, 8]
     const int col0 = ci * RG_S + (split ? RG_H + 1 : 0);
@@ -214,6 +217,7 @@ __global__ __launch_bounds__(256) void regression_rows(RegBatch b)
             lds[buf][3][i] = make_float4(nr.x, nr.y, nr.z, sqrtf(v2));
         }
         __syncthreads();
+        if (!wave_busy) continue;
 #pragma unroll
         for (int k = 0; k < RG_H + 1; ++k)
         {
