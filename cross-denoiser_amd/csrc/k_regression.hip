@@ -132,6 +132,7 @@ struct RegFrame {
     const float *y1, *y2, *var1, *var2, *alb, *nrm;
     float *beta;  // [Hs][Ws][2][18]
     int H, W;
+    int nfull;    // fast path with paired row tails: full 64-center blocks per center row, else 0
 };
 struct RegBatch {
     TileMap map;
@@ -163,6 +164,13 @@ constexpr int RG_S = 4;         // sparsity of the fast path
 constexpr int RG_H = 8;         // half window of the fast path
 constexpr int RG_ROW = RG_CX * RG_S + 2 * RG_H;
 
+// Launch shape: one block per 64 centers of a center row; where a row's last block would hold at
+// most 32 centers (1920 px: 480 centers = 7.5 blocks), the tails of two consecutive rows share one
+// block instead (paired tail: wave w serves row 2p + (w & 1), half w >> 1, 32 centers each, over the
+// 21 tap rows the two center rows span), so a 1080p frame is 2,025 blocks instead of 2,160: two
+// rounds of the 1,024 resident at a time (four per CU) instead of three. Tile map per row pair p:
+// nfull blocks of row 2p, nfull of row 2p + 1, then the paired tail. Each lane sums its taps in the
+// same order either way.
 __global__ __launch_bounds__(256) void regression_rows(RegBatch b)
 {
     // [buffer][plane][pixel]; planes: (y1, var1), (y2, var2), (albedo, std1), (normal, std2)
@@ -170,15 +178,31 @@ __global__ __launch_bounds__(256) void regression_rows(RegBatch b)
     const int tile = xcd_tile(blockIdx.x, b.map.total);
     const TileLoc L = locate(b.map, tile);
     const RegFrame &F = b.f[L.f];
-    const int H = F.H, W = F.W, Ws = W / RG_S;
-    const int by = L.ty, bx0 = L.tx * RG_CX;
+    const int H = F.H, W = F.W, Ws = W / RG_S, Hs = H / RG_S;
     const int t = threadIdx.x;
+    int by, bx0;
+    bool paired = false;
+    if (F.nfull)
+    {
+        const int c = L.tx, nf = F.nfull;
+        paired = c == 2 * nf;
+        by = 2 * L.ty + (c >= nf && !paired);
+        bx0 = (paired ? nf : (c >= nf ? c - nf : c)) * RG_CX;
+        if (by >= Hs) return;  // odd row count: the last pair has no second row (whole block)
+    }
+    else
+    {
+        by = L.ty;
+        bx0 = L.tx * RG_CX;
+    }
     const int half = t >> 7;           // wave-uniform: waves 0-1 half A, waves 2-3 half B
     const int j = t & 127;
-    const int ci = j >> 1, split = j & 1;
+    const int o = paired ? (j >> 6) : 0;  // center row offset of this wave (paired tail only)
+    const int ci = paired ? ((j & 63) >> 1) : (j >> 1), split = j & 1;
+    const int row_c = by + o;          // this lane's center row
     const int bx = bx0 + ci;
-    const bool active = bx < Ws;
-    const int cx = (active ? bx : Ws - 1) * RG_S, cy = by * RG_S;
+    const bool active = bx < Ws && row_c < Hs;
+    const int cx = (active ? bx : Ws - 1) * RG_S, cy = min(row_c, Hs - 1) * RG_S;
     const int px0 = bx0 * RG_S - RG_H;  // pixel x of LDS column 0
 
     const float *feat = half ? F.y2 : F.y1;
@@ -193,18 +217,20 @@ __global__ __launch_bounds__(256) void regression_rows(RegBatch b)
     acc.zero();
 
     const int ncols = min(RG_ROW, W + 2 * RG_H - bx0 * RG_S);  // columns actually needed
-    // a wave whose 32 centers all lie past the row end (the last block of a row of 480 centers
-    // holds 32) only stages rows and meets the barriers; its lanes' results are never stored
-    const bool wave_busy = __builtin_amdgcn_readfirstlane((bx0 + (j & ~63) / 2) < Ws);
+    // a wave whose centers all lie past the row end (or on a missing second row) only stages rows
+    // and meets the barriers; its lanes' results are never stored
+    const bool wave_busy = __builtin_amdgcn_readfirstlane(
+        (bx0 + (paired ? 0 : (j & ~63) / 2)) < Ws && row_c < Hs);
     const int own = half, oth = 1 - half;
     // first tap column of this lane: split 0 covers dx in [-8, 0], split 1 covers [1, 8]
     const int col0 = ci * RG_S + (split ? RG_H + 1 : 0);
     const int ntap = split ? RG_H : RG_H + 1;
+    const int nrows = 2 * RG_H + 1 + (paired ? RG_S : 0);
 
-    for (int r = 0; r <= 2 * RG_H; ++r)
+    for (int r = 0; r < nrows; ++r)
     {
         const int buf = r & 1;
-        const int gy = mirror(cy + r - RG_H, H);
+        const int gy = mirror(by * RG_S + r - RG_H, H);
         const size_t row = (size_t)gy * W;
         for (int i = t; i < ncols; i += 256)
         {
@@ -217,7 +243,7 @@ __global__ __launch_bounds__(256) void regression_rows(RegBatch b)
             lds[buf][3][i] = make_float4(nr.x, nr.y, nr.z, sqrtf(v2));
         }
         __syncthreads();
-        if (!wave_busy) continue;
+        if (!wave_busy || r < o * RG_S || r > o * RG_S + 2 * RG_H) continue;
 #pragma unroll
         for (int k = 0; k < RG_H + 1; ++k)
         {
@@ -240,7 +266,7 @@ __global__ __launch_bounds__(256) void regression_rows(RegBatch b)
     reduce_pair(acc);
     if (!active) return;
     // Lane pair solves the three channels: split 0 -> channels 0 and 1, split 1 -> channel 2.
-    float *out = F.beta + (((size_t)by * Ws + bx) * 2 + half) * 18;
+    float *out = F.beta + (((size_t)row_c * Ws + bx) * 2 + half) * 18;
     {
         Sys6 s0, s2;
         acc.system(0, s0);
@@ -530,8 +556,19 @@ hipError_t launch_regression(int n, const CrFrame *fr, int win, int s, hipStream
         r.y1 = fr[i].y1; r.y2 = fr[i].y2; r.var1 = fr[i].var1; r.var2 = fr[i].var2;
         r.alb = fr[i].alb; r.nrm = fr[i].nrm; r.beta = fr[i].beta; r.H = fr[i].H; r.W = fr[i].W;
         const int Ws = r.W / s, Hs = r.H / s;
-        tx[i] = fast ? div_up(Ws, RG_CX) : div_up(Ws, 16);
-        ty[i] = fast ? Hs : div_up(Hs, 8);
+        // pair the rows' tail blocks where a tail holds at most half a block of centers
+        const int rem = Ws % RG_CX;
+        r.nfull = (fast && Ws >= RG_CX && rem > 0 && rem <= RG_CX / 2 && Hs >= 2) ? Ws / RG_CX : 0;
+        if (r.nfull)
+        {
+            tx[i] = 2 * r.nfull + 1;
+            ty[i] = div_up(Hs, 2);
+        }
+        else
+        {
+            tx[i] = fast ? div_up(Ws, RG_CX) : div_up(Ws, 16);
+            ty[i] = fast ? Hs : div_up(Hs, 8);
+        }
     }
     b.map = make_tilemap(n, tx, ty);
     b.half_win = win / 2;

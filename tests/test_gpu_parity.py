@@ -30,7 +30,10 @@ def _report(name, err):
 
 
 # ------------------------------------------------------------------------------------------------
-@pytest.mark.parametrize("B,H,W", [(1, 64, 64), (1, 48, 40), (2, 36, 52), (1, 256, 256), (1, 24, 300)])
+# 24x300 and 20x264 run the paired row tails of the regression launch (75 and 66 centers per row:
+# a last block of at most 32 centers); 20x264 with an odd count of center rows (5) in a batch of 2.
+@pytest.mark.parametrize("B,H,W", [(1, 64, 64), (1, 48, 40), (2, 36, 52), (1, 256, 256), (1, 24, 300),
+                                   (2, 20, 264)])
 def test_cross_regression(ops, dev, B, H, W):
     ya, yb = log_images(H, W, seed=H * W, B=B)
     alb, nrm = rand_img(H, W, seed=3, B=B), unit_normals(H, W, seed=4, B=B)
