@@ -444,8 +444,9 @@ def main():
                 "traffic": profiled_traffic(name.split(" ")[0], H, W), "kernel": name,
                 "kernel_ms": round(dur, 4),
                 "algorithmic_flop_per_px": ALGO[dom]["flop_per_px"] if dom == "filter_train" else None,
-                "note": "VALU-bound FP32 kernel (no MFMA); peak is the published FP32 rate (the non-packed VALU "
-                        "rate is half of it); kernel_ms = HIP events around back-to-back launches of the kernel "
+                "note": "VALU-bound FP32 kernel (no MFMA); peak is the published FP32 rate (a probe of "
+                        "independent non-packed FP32 FMAs reaches about 69% of it with eight waves per SIMD, "
+                        "and the kernel runs five); kernel_ms = HIP events around back-to-back launches of the kernel "
                         "alone on its stream; traffic = FETCH_SIZE + WRITE_SIZE bytes per launch from the "
                         "committed counter profile, copy-calibrated"}
     unet = None
